@@ -1,0 +1,431 @@
+// C-ABI implementation: contexts, plans and the host orchestration of the
+// pencil passes and the coil stage.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/disorder.h"
+#include "kernels.hpp"
+#include "transform.hpp"
+
+using namespace dis;
+
+// ---------------------------------------------------------------- context
+struct dis_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<int, float2*> tw;
+  void* ws = nullptr;
+  size_t ws_size = 0;
+  size_t ws_limit = 0;
+  void* dcoef = nullptr;
+  size_t dcoef_cap = 0;
+  // profiling
+  bool prof = false;
+  struct Rec {
+    int kclass;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+};
+
+struct dis_plan {
+  dis_ctx* ctx = nullptr;
+  dis_geom g{};
+  const float2* S = nullptr;
+  int* k1 = nullptr;
+  int* k2 = nullptr;
+  int* off = nullptr;
+  float* We = nullptr;
+  std::vector<int> hoff;
+};
+
+namespace {
+
+int fail(dis_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail((ctx), DIS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+}  // namespace
+
+namespace dis {
+
+int ctx_fail(dis_ctx* c, int code, const char* msg) { return fail(c, code, msg); }
+
+hipStream_t ctx_stream(dis_ctx* c) { return c->stream; }
+
+int ctx_twiddles(dis_ctx* c, int n, const float2** out) {
+  auto it = c->tw.find(n);
+  if (it != c->tw.end()) {
+    *out = it->second;
+    return DIS_OK;
+  }
+  float2* p = nullptr;
+  HIPCHK(c, hipMalloc(&p, sizeof(float2) * n));
+  HIPCHK(c, launch_twiddles(p, n, c->stream));
+  c->tw[n] = p;
+  *out = p;
+  return DIS_OK;
+}
+
+int ctx_workspace(dis_ctx* c, size_t bytes, void** out) {
+  if (c->ws_limit && bytes > c->ws_limit) return fail(c, DIS_E_NOMEM, "workspace request above the limit");
+  if (bytes > c->ws_size) {
+    if (c->ws) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipFree(c->ws));
+      c->ws = nullptr;
+      c->ws_size = 0;
+    }
+    if (hipMalloc(&c->ws, bytes) != hipSuccess) {
+      c->ws = nullptr;
+      (void)hipGetLastError();
+      return fail(c, DIS_E_NOMEM, "hipMalloc of the workspace failed");
+    }
+    c->ws_size = bytes;
+  }
+  *out = c->ws;
+  return DIS_OK;
+}
+
+size_t ctx_workspace_limit(dis_ctx* c) { return c->ws_limit; }
+
+// Upload the staged bytes into the device coefficient buffer (grown on
+// demand).  Stream-ordered after all earlier work; synchronous on return, so
+// the buffer may be reused by the next upload.
+int ctx_upload_bytes(dis_ctx* c, const Staging& stg, const unsigned char** dptr) {
+  const size_t nbytes = std::max<size_t>(stg.bytes.size(), 16);
+  if (nbytes > c->dcoef_cap) {
+    if (c->dcoef) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipFree(c->dcoef));
+      c->dcoef = nullptr;
+    }
+    const size_t cap = std::max<size_t>(nbytes, 1 << 16);
+    HIPCHK(c, hipMalloc(&c->dcoef, cap));
+    c->dcoef_cap = cap;
+  }
+  if (!stg.bytes.empty())
+    HIPCHK(c, hipMemcpyAsync(c->dcoef, stg.bytes.data(), stg.bytes.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *dptr = (const unsigned char*)c->dcoef;
+  return DIS_OK;
+}
+
+int prof_begin(dis_ctx* c, int kclass) {
+  if (!c->prof) return -1;
+  auto get = [&]() {
+    hipEvent_t e = nullptr;
+    if (!c->pool.empty()) {
+      e = c->pool.back();
+      c->pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  };
+  dis_ctx::Rec r;
+  r.kclass = kclass;
+  r.a = get();
+  r.b = get();
+  r.bytes = 0;
+  if (!r.a || !r.b) return -1;
+  (void)hipEventRecord(r.a, c->stream);
+  c->recs.push_back(r);
+  return (int)c->recs.size() - 1;
+}
+
+void prof_end(dis_ctx* c, int slot, double bytes) {
+  if (slot < 0 || !c->prof) return;
+  c->recs[slot].bytes = bytes;
+  (void)hipEventRecord(c->recs[slot].b, c->stream);
+}
+
+}  // namespace dis
+
+// ---------------------------------------------------------------- C-ABI
+extern "C" {
+
+int dis_version(void) { return 100; }
+
+const char* dis_status_string(int s) {
+  switch (s) {
+    case DIS_OK: return "ok";
+    case DIS_E_INVALID: return "invalid argument";
+    case DIS_E_UNSUPPORTED: return "unsupported";
+    case DIS_E_NOMEM: return "out of device memory";
+    case DIS_E_HIP: return "HIP runtime error";
+    case DIS_E_NODEVICE: return "no device";
+    default: return "unknown status";
+  }
+}
+
+int dis_ctx_create(int device, dis_ctx** out) {
+  if (!out) return DIS_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return DIS_E_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) return DIS_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return DIS_E_HIP;
+  dis_ctx* c = new dis_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return DIS_E_HIP;
+  }
+  *out = c;
+  return DIS_OK;
+}
+
+int dis_ctx_destroy(dis_ctx* c) {
+  if (!c) return DIS_E_INVALID;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->tw) (void)hipFree(kv.second);
+  for (auto& r : c->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  if (c->ws) (void)hipFree(c->ws);
+  if (c->dcoef) (void)hipFree(c->dcoef);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return DIS_OK;
+}
+
+const char* dis_last_error(const dis_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int dis_sync(dis_ctx* c) {
+  if (!c) return DIS_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+int dis_set_workspace_limit(dis_ctx* c, size_t bytes) {
+  if (!c) return DIS_E_INVALID;
+  c->ws_limit = bytes;
+  return DIS_OK;
+}
+
+int dis_profile_enable(dis_ctx* c, int32_t on) {
+  if (!c) return DIS_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& r : c->recs) {
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+  c->prof = on != 0;
+  return DIS_OK;
+}
+
+int dis_profile_read(dis_ctx* c, int32_t kclass, int64_t* launches, double* ms_total, double* bytes_total) {
+  if (!c) return DIS_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int64_t n = 0;
+  double ms = 0, by = 0;
+  for (auto& r : c->recs) {
+    if (r.kclass != kclass) continue;
+    float t = 0;
+    HIPCHK(c, hipEventElapsedTime(&t, r.a, r.b));
+    ms += t;
+    by += r.bytes;
+    ++n;
+  }
+  if (launches) *launches = n;
+  if (ms_total) *ms_total = ms;
+  if (bytes_total) *bytes_total = by;
+  return DIS_OK;
+}
+
+int dis_rigid_transform(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], const double* T, int32_t S,
+                        int32_t di, int32_t sumR, int32_t x_per_state, const void* x, void* out) {
+  if (!c) return DIS_E_INVALID;
+  if (!n || !Nfull || !T || !x || !out || S <= 0 || (di != 0 && di != 1))
+    return fail(c, DIS_E_INVALID, "dis_rigid_transform: invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  Grid gr;
+  int st = make_grid(c, n, Nfull, &gr);
+  if (st) return st;
+  std::vector<double> Th(T, T + 6 * (size_t)S);
+  if (di == 1) {
+    st = rigid_forward(c, gr, Th.data(), S, S, (const float2*)x, x_per_state ? gr.nvox : 0, (float2*)out, false);
+  } else {
+    float2* work = (float2*)out;
+    if (sumR) {
+      void* ws = nullptr;
+      st = ctx_workspace(c, sizeof(float2) * gr.nvox * S, &ws);
+      if (st) return st;
+      work = (float2*)ws;
+    }
+    st = rigid_inverse(c, gr, Th.data(), S, S, (const float2*)x, gr.nvox, work, (float2*)out, sumR != 0, false,
+                       false);
+  }
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+int dis_plan_create(dis_ctx* c, const dis_geom* g, const void* S_maps, const int32_t* k1, const int32_t* k2,
+                    const int32_t* state_off, const float* We, dis_plan** out) {
+  if (!c) return DIS_E_INVALID;
+  if (!g || !S_maps || !state_off || !out || (g->nsamples > 0 && (!k1 || !k2)))
+    return fail(c, DIS_E_INVALID, "dis_plan_create: null argument");
+  *out = nullptr;
+  if (g->ncoils <= 0 || g->nstates <= 0 || g->nmotion < 0 || g->nmotion > g->nstates || g->nsamples < 0)
+    return fail(c, DIS_E_INVALID, "dis_plan_create: bad geometry");
+  for (int d = 0; d < 3; ++d)
+    if (g->n[d] <= 0 || g->Nfull[d] <= 0) return fail(c, DIS_E_INVALID, "dis_plan_create: bad grid size");
+  if (state_off[0] != 0 || state_off[g->nstates] != g->nsamples)
+    return fail(c, DIS_E_INVALID, "dis_plan_create: state offsets must start at 0 and end at nsamples");
+  for (int s = 0; s < g->nstates; ++s)
+    if (state_off[s + 1] < state_off[s]) return fail(c, DIS_E_INVALID, "dis_plan_create: offsets not monotone");
+  for (int j = 0; j < g->nsamples; ++j)
+    if (k1[j] < 0 || k1[j] >= g->n[0] || k2[j] < 0 || k2[j] >= g->n[1])
+      return fail(c, DIS_E_INVALID, "dis_plan_create: sample index outside the phase-encode grid");
+  Grid gr;
+  int st = make_grid(c, g->n, g->Nfull, &gr);
+  if (st) return st;
+  HIPCHK(c, hipSetDevice(c->device));
+  dis_plan* p = new dis_plan();
+  p->ctx = c;
+  p->g = *g;
+  p->S = (const float2*)S_maps;
+  p->hoff.assign(state_off, state_off + g->nstates + 1);
+  const size_t ns = std::max(1, g->nsamples);
+  bool ok = hipMalloc(&p->k1, ns * sizeof(int)) == hipSuccess && hipMalloc(&p->k2, ns * sizeof(int)) == hipSuccess &&
+            hipMalloc(&p->off, (g->nstates + 1) * sizeof(int)) == hipSuccess;
+  if (ok && g->nsamples > 0) {
+    ok = hipMemcpy(p->k1, k1, g->nsamples * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(p->k2, k2, g->nsamples * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (ok) ok = hipMemcpy(p->off, state_off, (g->nstates + 1) * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    dis_plan_destroy(p);
+    return fail(c, DIS_E_NOMEM, "dis_plan_create: device allocation failed");
+  }
+  *out = p;
+  if (We) return dis_plan_set_weights(p, We);
+  return DIS_OK;
+}
+
+int dis_plan_destroy(dis_plan* p) {
+  if (!p) return DIS_E_INVALID;
+  if (p->ctx) (void)hipSetDevice(p->ctx->device);
+  if (p->k1) (void)hipFree(p->k1);
+  if (p->k2) (void)hipFree(p->k2);
+  if (p->off) (void)hipFree(p->off);
+  if (p->We) (void)hipFree(p->We);
+  delete p;
+  return DIS_OK;
+}
+
+int dis_plan_set_weights(dis_plan* p, const float* We) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!We) {
+    if (p->We) HIPCHK(c, hipFree(p->We));
+    p->We = nullptr;
+    return DIS_OK;
+  }
+  if (!p->We) HIPCHK(c, hipMalloc(&p->We, std::max(1, p->g.nsamples) * sizeof(float)));
+  if (p->g.nsamples > 0) HIPCHK(c, hipMemcpy(p->We, We, p->g.nsamples * sizeof(float), hipMemcpyHostToDevice));
+  return DIS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- operators
+namespace dis {
+
+OpView plan_view(dis_plan* p) {
+  OpView v;
+  v.S = p->S;
+  v.k1 = p->k1;
+  v.k2 = p->k2;
+  v.off = p->off;
+  v.We = p->We;
+  v.g = p->g;
+  v.hoff = p->hoff.data();
+  return v;
+}
+
+}  // namespace dis
+
+extern "C" {
+
+int dis_encode(dis_plan* p, const double* T, const void* x, void* y) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!x || !y || (p->g.nmotion > 0 && !T)) return fail(c, DIS_E_INVALID, "dis_encode: null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  int st = op_apply(c, plan_view(p), T, (const float2*)x, nullptr, (float2*)y, nullptr, COIL_ENCODE, nullptr);
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+int dis_decode(dis_plan* p, const double* T, const void* y, void* x) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!x || !y || (p->g.nmotion > 0 && !T)) return fail(c, DIS_E_INVALID, "dis_decode: null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  int st = op_apply(c, plan_view(p), T, nullptr, (const float2*)y, nullptr, (float2*)x, COIL_DECODE, nullptr);
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+int dis_encodedecode(dis_plan* p, const double* T, const void* x, void* out) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!x || !out || (p->g.nmotion > 0 && !T)) return fail(c, DIS_E_INVALID, "dis_encodedecode: null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  int st = op_apply(c, plan_view(p), T, (const float2*)x, nullptr, nullptr, (float2*)out, COIL_EHE, nullptr);
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+int dis_energy(dis_plan* p, const double* T, const void* x, const void* y, const float* Mbe, double* per_sample,
+               double* total) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!x || !y || (p->g.nmotion > 0 && !T)) return fail(c, DIS_E_INVALID, "dis_energy: null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  EnergyOut eo;
+  eo.Mbe = Mbe;
+  eo.per_sample = per_sample;
+  eo.total = total;
+  int st = op_apply(c, plan_view(p), T, (const float2*)x, (const float2*)y, nullptr, nullptr, COIL_ENERGY, &eo);
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+}  // extern "C"

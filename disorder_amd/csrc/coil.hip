@@ -1,0 +1,285 @@
+// Coil stage of E, E^H and E^H E on gfx950.
+//
+// One workgroup per (motion state s, readout plane i3).  The sampled k-space
+// value of sample j (phase-encode position (k1_j, k2_j)) for coil c is
+//   Y_c[j, i3] = 1/sqrt(n1 n2) sum_{i2} W2^{k2_j i2} sum_{i1} W1^{k1_j i1} S_c u
+// (the unitary row-DFT form of Reconstruction/encode.m:120-136).  The plane is
+// walked in tiles of G dim-2 rows: each tile is multiplied by S_c, FFT'd along
+// dim 1 in LDS, and every thread accumulates the samples it owns (registers)
+// with the dim-2 DFT weight W2^{k2 i2}.  The adjoint (Reconstruction/
+// decode.m:43-84) scatters the weighted samples of each tile row back with
+// LDS float atomics, inverse-FFTs along dim 1, multiplies by conj(S_c) and
+// sums the coils in registers.  For E^H E both halves run in the same
+// workgroup, so the state's k-space never leaves the chip.
+#include "kernels.hpp"
+
+namespace dis {
+
+constexpr int kMaxJ = 4;   // samples per thread per chunk
+constexpr int kCC = 8;     // coils per chunk
+constexpr int kTE = kMaxTile / kThreads;
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
+  constexpr bool FWD = MODE == COIL_ENCODE || MODE == COIL_EHE || MODE == COIL_ENERGY;
+  constexpr bool ADJ = MODE == COIL_DECODE || MODE == COIL_EHE;
+  __shared__ float2 lds[kMaxTile];
+  __shared__ float2 tw1[1024];
+  __shared__ float2 tw2[1024];
+  const int tid = threadIdx.x;
+  const int n1 = a.n1, n2 = a.n2;
+  const int i3 = blockIdx.x;
+  const int sl = blockIdx.y;
+  const int s = a.state0 + sl;
+  const int G = a.G;
+  const long long nplane = (long long)n1 * n2;
+  const long long nvox = nplane * a.n3;
+  const float inv_sqrt = rsqrtf((float)n1 * (float)n2);
+  const float inv_n1 = 1.0f / (float)n1;
+  for (int k = tid; k < n1; k += kThreads) tw1[k] = a.tw1[k];
+  for (int k = tid; k < n2; k += kThreads) tw2[k] = a.tw2[k];
+  __syncthreads();
+
+  const int o0 = a.off[s], o1 = a.off[s + 1];
+  const float2* Up = FWD ? a.U + (a.us_rel ? sl : s) * a.us + i3 * nplane : nullptr;
+  float2* Vp = ADJ ? a.V + (long long)(a.vs_rel ? sl : s) * a.vs + i3 * nplane : nullptr;
+  const int ngroups = (n2 + G - 1) / G;
+  const double cf = a.tfwd ? a.tfwd[s] * a.invN1 : 0.0;
+  const double ca = a.tadj ? a.tadj[s] * a.invN1 : 0.0;
+
+  bool first_out = true;
+  for (int sc0 = o0; sc0 < o1 || (sc0 == o0 && ADJ); sc0 += kMaxJ * kThreads) {
+    const int sc1 = min(o1, sc0 + kMaxJ * kThreads);
+    int jk1[kMaxJ], jk2[kMaxJ];
+    float jw[kMaxJ];
+#pragma unroll
+    for (int i = 0; i < kMaxJ; ++i) {
+      const int j = sc0 + tid + i * kThreads;
+      if (j < sc1) {
+        jk1[i] = a.k1[j];
+        jk2[i] = a.k2[j];
+        jw[i] = a.We ? a.We[j] : 1.0f;
+      } else {
+        jk1[i] = -1;
+        jk2[i] = 0;
+        jw[i] = 0.f;
+      }
+    }
+    float esum[kMaxJ];
+#pragma unroll
+    for (int i = 0; i < kMaxJ; ++i) esum[i] = 0.f;
+
+    for (int c0 = 0; c0 < a.C; c0 += kCC) {
+      const int cc = min(kCC, a.C - c0);
+      float2 W[kMaxJ][kCC];
+#pragma unroll
+      for (int i = 0; i < kMaxJ; ++i)
+#pragma unroll
+        for (int c = 0; c < kCC; ++c) W[i][c] = make_float2(0.f, 0.f);
+
+      if (FWD) {
+        for (int g = 0; g < ngroups; ++g) {
+          const int r0 = g * G;
+          const int Gv = min(G, n2 - r0);
+          const int tot = G * n1;
+          const int totv = Gv * n1;
+          float2 ureg[kTE];
+#pragma unroll
+          for (int i = 0; i < kTE; ++i) {
+            const int e = tid + i * kThreads;
+            if (e < tot) lds[e] = e < totv ? Up[(long long)r0 * n1 + e] : make_float2(0.f, 0.f);
+          }
+          __syncthreads();
+          if (a.tfwd) {  // last translation of T (along dim 1) before the coil multiply
+            fft_tile<true>(lds, tw1, a.plan1, G, tid);
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              if (e < tot) {
+                const int j = e % n1;
+                lds[e] = cswap(cmul(lds[e], phase_cycles(cf, freq_index(j, n1))));
+              }
+            }
+            __syncthreads();
+            fft_tile<true>(lds, tw1, a.plan1, G, tid);
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              ureg[i] = (e < tot) ? cscale(cswap(lds[e]), inv_n1) : make_float2(0.f, 0.f);
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              ureg[i] = (e < tot) ? lds[e] : make_float2(0.f, 0.f);
+            }
+          }
+          __syncthreads();
+          for (int c = 0; c < cc; ++c) {
+            const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + (long long)r0 * n1;
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              if (e < tot) lds[e] = e < totv ? cmul(ureg[i], Sp[e]) : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+            fft_tile<true>(lds, tw1, a.plan1, G, tid);
+#pragma unroll
+            for (int i = 0; i < kMaxJ; ++i) {
+              if (jk1[i] >= 0) {
+                int idx = (int)(((long long)jk2[i] * r0) % n2);
+                float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int cq = 0; cq < kCC; ++cq)
+                  if (cq == c) acc = W[i][cq];
+                for (int p = 0; p < Gv; ++p) {
+                  acc = cadd(acc, cmul(tw2[idx], lds[p * n1 + jk1[i]]));
+                  idx += jk2[i];
+                  if (idx >= n2) idx -= n2;
+                }
+                // W[i][c] with runtime c: write through a small switch to keep registers
+#pragma unroll
+                for (int cq = 0; cq < kCC; ++cq)
+                  if (cq == c) W[i][cq] = acc;
+              }
+            }
+            __syncthreads();
+          }
+        }
+        // samples of this chunk are complete
+#pragma unroll
+        for (int i = 0; i < kMaxJ; ++i) {
+          const int j = sc0 + tid + i * kThreads;
+          if (jk1[i] >= 0) {
+#pragma unroll
+            for (int c = 0; c < kCC; ++c) {
+              if (c < cc) {
+                const float2 Y = cscale(W[i][c], inv_sqrt);
+                const long long yi = j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c));
+                if (MODE == COIL_ENCODE) a.yout[yi] = Y;
+                if (MODE == COIL_ENERGY) {
+                  const float2 r = csub(Y, a.yin[yi]);
+                  esum[i] += r.x * r.x + r.y * r.y;
+                }
+                if (MODE == COIL_EHE) W[i][c] = cscale(Y, jw[i]);
+              }
+            }
+          }
+        }
+      }
+      if (MODE == COIL_DECODE) {
+#pragma unroll
+        for (int i = 0; i < kMaxJ; ++i) {
+          const int j = sc0 + tid + i * kThreads;
+          if (jk1[i] >= 0) {
+#pragma unroll
+            for (int c = 0; c < kCC; ++c)
+              if (c < cc) W[i][c] = cscale(a.yin[j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c))], jw[i]);
+          }
+        }
+      }
+
+      if (ADJ) {
+        for (int g = 0; g < ngroups; ++g) {
+          const int r0 = g * G;
+          const int Gv = min(G, n2 - r0);
+          const int tot = G * n1;
+          const int totv = Gv * n1;
+          float2 o[kTE];
+#pragma unroll
+          for (int i = 0; i < kTE; ++i) o[i] = make_float2(0.f, 0.f);
+          for (int c = 0; c < cc; ++c) {
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              if (e < tot) lds[e] = make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kMaxJ; ++i) {
+              if (jk1[i] >= 0) {
+                float2 w = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int cq = 0; cq < kCC; ++cq)
+                  if (cq == c) w = W[i][cq];
+                int idx = (int)(((long long)jk2[i] * r0) % n2);
+                for (int p = 0; p < Gv; ++p) {
+                  // conj(W2^{k2 i2}) * w, stored real/imag-swapped for the inverse FFT
+                  const float2 v = cmulc(w, tw2[idx]);
+                  float* dst = reinterpret_cast<float*>(&lds[p * n1 + jk1[i]]);
+                  atomicAdd(dst, v.y);
+                  atomicAdd(dst + 1, v.x);
+                  idx += jk2[i];
+                  if (idx >= n2) idx -= n2;
+                }
+              }
+            }
+            __syncthreads();
+            fft_tile<true>(lds, tw1, a.plan1, G, tid);
+            const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + (long long)r0 * n1;
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              if (e < totv) o[i] = cadd(o[i], cscale(cmulc(cswap(lds[e]), Sp[e]), inv_sqrt));
+            }
+            __syncthreads();
+          }
+          if (a.tadj) {  // first translation of T^H (along dim 1) after the coil sum
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              if (e < tot) lds[e] = o[i];
+            }
+            __syncthreads();
+            fft_tile<true>(lds, tw1, a.plan1, G, tid);
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              if (e < tot) {
+                const int j = e % n1;
+                lds[e] = cswap(cmul(lds[e], phase_cycles(ca, freq_index(j, n1))));
+              }
+            }
+            __syncthreads();
+            fft_tile<true>(lds, tw1, a.plan1, G, tid);
+#pragma unroll
+            for (int i = 0; i < kTE; ++i) {
+              const int e = tid + i * kThreads;
+              if (e < tot) o[i] = cscale(cswap(lds[e]), inv_n1);
+            }
+            __syncthreads();
+          }
+          float2* Vt = Vp + (long long)r0 * n1;
+#pragma unroll
+          for (int i = 0; i < kTE; ++i) {
+            const int e = tid + i * kThreads;
+            if (e < totv) Vt[e] = first_out ? o[i] : cadd(Vt[e], o[i]);
+          }
+        }
+        first_out = false;
+      }
+    }
+    if (MODE == COIL_ENERGY) {
+#pragma unroll
+      for (int i = 0; i < kMaxJ; ++i) {
+        const int j = sc0 + tid + i * kThreads;
+        if (jk1[i] >= 0) a.epart[j + (long long)a.Ns * i3] = esum[i];
+      }
+    }
+    if (sc1 >= o1) break;
+  }
+}
+
+hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  dim3 grid(a.n3, nstates);
+  switch (mode) {
+    case COIL_ENCODE: hipLaunchKernelGGL(coil_kernel<COIL_ENCODE>, grid, dim3(kThreads), 0, st, a); break;
+    case COIL_DECODE: hipLaunchKernelGGL(coil_kernel<COIL_DECODE>, grid, dim3(kThreads), 0, st, a); break;
+    case COIL_EHE: hipLaunchKernelGGL(coil_kernel<COIL_EHE>, grid, dim3(kThreads), 0, st, a); break;
+    case COIL_ENERGY: hipLaunchKernelGGL(coil_kernel<COIL_ENERGY>, grid, dim3(kThreads), 0, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace dis

@@ -1,0 +1,90 @@
+// Kernel argument blocks and host-side launchers shared by the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_engine.hpp"
+
+namespace dis {
+
+// One fused 1D pass along `axis` of every pencil of a (n1,n2,n3) column-major
+// volume:  y = F^-1 diag(phi) F x  with
+//   phi(j) = exp(i (A_s r_src + B_s) k_axis(j)),
+//   k_axis(j) = 2 pi m(j) / Nfull_axis   (m = FFT-order frequency index),
+//   r_src(i) = rscale (i + 1 - rcent)    (full-resolution voxel units).
+// A_s, B_s are per motion state.  This is one shear (src >= 0) or one
+// translation (src < 0) of the sinc rigid transform.
+struct PassArgs {
+  const float2* in;
+  float2* out;
+  long long in_sstride;   // state stride of `in` in elements (0 = broadcast)
+  long long out_sstride;  // state stride of `out`
+  int n1, n2, n3;
+  int axis;
+  int src;
+  int P;                  // pencils per tile
+  int tiles_per_line;     // strided axes: tiles along the contiguous axis
+  const float2* tw;       // twiddles of length n (device)
+  FftPlan plan;
+  const double* coefA;    // [S] or null
+  const double* coefB;    // [S] or null
+  double rscale;
+  int rcent;
+  double invN;            // 1 / Nfull_axis
+  int S;                  // states looped inside a workgroup (BCAST / ACCUM)
+  const int* states;      // MAP: optional list of state indices for grid.y
+  int accumulate;         // ACCUM: add into existing `out`
+};
+
+enum PassMode { MODE_MAP = 0, MODE_BCAST = 1, MODE_ACCUM = 2, MODE_FLIP = 3, MODE_DERIV = 4 };
+
+// Launch a pass of the given mode; grid_states = gridDim.y for MAP/FLIP.
+hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t st);
+
+// twiddle table tw[k] = exp(-2 pi i k / n), k < n, computed in double.
+hipError_t launch_twiddles(float2* tw, int n, hipStream_t st);
+
+// ---------------------------------------------------------------- coil stage
+// One workgroup per (motion state, readout plane).  Forward part: coil
+// multiply, FFT along dim 1, and the DFT along dim 2 evaluated at the
+// state's samples; adjoint part: the transpose of it with conj(S) and the
+// coil sum.  Optional fused dim-1 translation on either side.
+struct CoilArgs {
+  const float2* U;        // transformed volumes, state stride us (0 = broadcast)
+  long long us;
+  float2* V;              // adjoint output volumes, state stride vs
+  long long vs;
+  const float2* Smaps;    // [n1,n2,n3,C]
+  int n1, n2, n3, C;
+  const int* k1;          // [Ns] FFT-order row index along dim 1
+  const int* k2;          // [Ns] along dim 2
+  const int* off;         // [S+1] state offsets
+  const float* We;        // [Ns] or null
+  int Ns;
+  const float2* yin;      // [Ns, n3, C] data (DECODE, ENERGY) or null
+  float2* yout;           // [Ns, n3, C] (ENCODE) or null
+  float* epart;           // [Ns, n3] per-(sample, plane) energy (ENERGY) or null
+  const double* tfwd;     // [S] dim-1 translation coefficient before the coil stage, or null
+  const double* tadj;     // [S] dim-1 translation coefficient after the adjoint, or null
+  double invN1;           // 1/Nfull_1 for the translation phase
+  const float2* tw1;      // twiddles n1
+  const float2* tw2;      // twiddles n2
+  FftPlan plan1;
+  int G;                  // pencils (dim-2 positions) per tile
+  int state0;             // first state handled by this launch
+  int vs_rel;             // V indexed relative to state0
+  int us_rel;             // U indexed relative to state0
+};
+
+enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };
+
+hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st);
+
+// ---------------------------------------------------------------- utilities
+hipError_t launch_bcast(const float2* x, float2* out, long long nvox, int S, hipStream_t st);
+hipError_t launch_sum_states(const float2* in, long long sstride, float2* out, long long nvox, int S,
+                             int accumulate, hipStream_t st);
+hipError_t launch_energy_reduce(const float* epart, int Ns, int n3, const float* Mbe, double* out,
+                                hipStream_t st);
+
+}  // namespace dis

@@ -1,0 +1,386 @@
+// Fused pencil passes of the sinc rigid transform on gfx950.
+//
+// Each workgroup owns a tile of P pencils along one axis, stages it in LDS,
+// runs FFT -> per-pencil phase ramp -> inverse FFT, and writes it back.  The
+// three modes share the tile machinery:
+//   MAP   : one state per grid.y, in -> out (in place allowed)
+//   BCAST : one input volume, all states looped inside the workgroup; the
+//           forward FFT of the input is done once and kept in registers
+//   ACCUM : all states looped inside the workgroup and summed in the
+//           frequency domain; one inverse FFT per tile (state sum of T^H)
+//   FLIP  : the >90 degree flip of Methods/flipping.m along one axis
+#include "kernels.hpp"
+
+namespace dis {
+
+constexpr int kE = kMaxTile / kThreads;  // elements per thread per tile
+constexpr int kMaxPencils = 512;          // tile pencils (host caps P)
+
+struct TileInfo {
+  long long base;
+  long long jstride;
+  int Pv;   // valid pencils in this tile
+  int t_a;  // CONTIG: first pencil index q0; strided: first contiguous index i1_0
+  int t_b;  // strided: the other coordinate (i3 for axis 1, i2 for axis 2)
+};
+
+__device__ __forceinline__ TileInfo tile_info(const PassArgs& a, int t) {
+  TileInfo ti;
+  if (a.axis == 0) {
+    const int np = a.n2 * a.n3;
+    const int q0 = t * a.P;
+    ti.Pv = min(a.P, np - q0);
+    ti.base = (long long)q0 * a.n1;
+    ti.jstride = 1;
+    ti.t_a = q0;
+    ti.t_b = 0;
+  } else {
+    const int line = t / a.tiles_per_line;
+    const int i10 = (t - line * a.tiles_per_line) * a.P;
+    ti.Pv = min(a.P, a.n1 - i10);
+    ti.t_a = i10;
+    ti.t_b = line;
+    if (a.axis == 1) {
+      ti.base = (long long)i10 + (long long)a.n1 * a.n2 * line;
+      ti.jstride = a.n1;
+    } else {
+      ti.base = (long long)i10 + (long long)a.n1 * line;
+      ti.jstride = (long long)a.n1 * a.n2;
+    }
+  }
+  return ti;
+}
+
+// coordinate of pencil p of the tile along axis `src`
+__device__ __forceinline__ int src_coord(const PassArgs& a, const TileInfo& ti, int p) {
+  if (a.axis == 0) {
+    const int q = ti.t_a + p;
+    const int i2 = q % a.n2;
+    return a.src == 1 ? i2 : q / a.n2;
+  }
+  if (a.src == 0) return ti.t_a + p;
+  return ti.t_b;  // axis 1 -> src 2 (i3), axis 2 -> src 1 (i2)
+}
+
+template <bool CONTIG>
+__device__ __forceinline__ void elem_pj(int e, int n, int P, int& p, int& j) {
+  if (CONTIG) { p = e / n; j = e - p * n; }
+  else { j = e / P; p = e - j * P; }
+}
+
+template <bool CONTIG>
+__device__ __forceinline__ long long gaddr(const TileInfo& ti, int e, int n, int P) {
+  int p, j;
+  elem_pj<CONTIG>(e, n, P, p, j);
+  return CONTIG ? ti.base + e : ti.base + p + (long long)j * ti.jstride;
+}
+
+template <bool CONTIG>
+__device__ __forceinline__ void load_tile(float2* lds, const float2* __restrict__ g, const TileInfo& ti, int n,
+                                          int P, int tid) {
+  const int tot = P * n;
+#pragma unroll
+  for (int i = 0; i < kE; ++i) {
+    const int e = tid + i * kThreads;
+    if (e < tot) {
+      int p, j;
+      elem_pj<CONTIG>(e, n, P, p, j);
+      lds[e] = (p < ti.Pv) ? g[gaddr<CONTIG>(ti, e, n, P)] : make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// per-pencil phase coefficient in cycles per frequency index
+__device__ __forceinline__ void pencil_coefs(double* cP, const PassArgs& a, const TileInfo& ti, int s, int tid) {
+  const double A = a.coefA ? a.coefA[s] : 0.0;
+  const double B = a.coefB ? a.coefB[s] : 0.0;
+  for (int p = tid; p < a.P; p += kThreads) {
+    double r = 0.0;
+    if (a.src >= 0) r = a.rscale * (double)(src_coord(a, ti, p) + 1 - a.rcent);
+    cP[p] = (A * r + B) * a.invN;
+  }
+}
+
+template <bool CONTIG, int MODE>
+__global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
+  __shared__ float2 lds[kMaxTile];
+  __shared__ float2 tw[1024];
+  __shared__ double cP[kMaxPencils];
+  const int tid = threadIdx.x;
+  const int n = a.plan.n;
+  const int P = a.P;
+  const int tot = P * n;
+  const float scale = 1.0f / (float)n;
+  for (int k = tid; k < n; k += kThreads) tw[k] = a.tw[k];
+  const TileInfo ti = tile_info(a, blockIdx.x);
+
+  if (MODE == MODE_MAP) {
+    const int s = a.states ? a.states[blockIdx.y] : (int)blockIdx.y;
+    pencil_coefs(cP, a, ti, s, tid);
+    load_tile<CONTIG>(lds, a.in + a.in_sstride * s, ti, n, P, tid);
+    __syncthreads();
+    fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) {
+        int p, j;
+        elem_pj<CONTIG>(e, n, P, p, j);
+        lds[e] = cswap(cmul(lds[e], phase_cycles(cP[p], freq_index(j, n))));
+      }
+    }
+    __syncthreads();
+    fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+    float2* o = a.out + a.out_sstride * s;
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) {
+        int p, j;
+        elem_pj<CONTIG>(e, n, P, p, j);
+        if (p < ti.Pv) o[gaddr<CONTIG>(ti, e, n, P)] = cscale(cswap(lds[e]), scale);
+      }
+    }
+  } else if (MODE == MODE_BCAST) {
+    load_tile<CONTIG>(lds, a.in, ti, n, P, tid);
+    __syncthreads();
+    fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+    float2 spec[kE];
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      spec[i] = (e < tot) ? lds[e] : make_float2(0.f, 0.f);
+    }
+    for (int s = 0; s < a.S; ++s) {
+      pencil_coefs(cP, a, ti, s, tid);
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < kE; ++i) {
+        const int e = tid + i * kThreads;
+        if (e < tot) {
+          int p, j;
+          elem_pj<CONTIG>(e, n, P, p, j);
+          lds[e] = cswap(cmul(spec[i], phase_cycles(cP[p], freq_index(j, n))));
+        }
+      }
+      __syncthreads();
+      fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+      float2* o = a.out + a.out_sstride * s;
+#pragma unroll
+      for (int i = 0; i < kE; ++i) {
+        const int e = tid + i * kThreads;
+        if (e < tot) {
+          int p, j;
+          elem_pj<CONTIG>(e, n, P, p, j);
+          if (p < ti.Pv) o[gaddr<CONTIG>(ti, e, n, P)] = cscale(cswap(lds[e]), scale);
+        }
+      }
+      __syncthreads();
+    }
+  } else if (MODE == MODE_ACCUM) {
+    float2 acc[kE];
+#pragma unroll
+    for (int i = 0; i < kE; ++i) acc[i] = make_float2(0.f, 0.f);
+    for (int s = 0; s < a.S; ++s) {
+      pencil_coefs(cP, a, ti, s, tid);
+      load_tile<CONTIG>(lds, a.in + a.in_sstride * s, ti, n, P, tid);
+      __syncthreads();
+      fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+#pragma unroll
+      for (int i = 0; i < kE; ++i) {
+        const int e = tid + i * kThreads;
+        if (e < tot) {
+          int p, j;
+          elem_pj<CONTIG>(e, n, P, p, j);
+          acc[i] = cadd(acc[i], cmul(lds[e], phase_cycles(cP[p], freq_index(j, n))));
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) lds[e] = cswap(acc[i]);
+    }
+    __syncthreads();
+    fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) {
+        int p, j;
+        elem_pj<CONTIG>(e, n, P, p, j);
+        if (p < ti.Pv) {
+          const long long g = gaddr<CONTIG>(ti, e, n, P);
+          float2 v = cscale(cswap(lds[e]), scale);
+          if (a.accumulate) v = cadd(v, a.out[g]);
+          a.out[g] = v;
+        }
+      }
+    }
+  } else if (MODE == MODE_FLIP) {
+    // Methods/flipping.m:11-22 along this axis: shift by h1=-c+1/2 with
+    // c = n/2+1, reverse, shift by -h1.  Odd n gives integer shifts, which
+    // Methods/shifting.m:29-33 applies as circular index shifts.
+    const int s = a.states ? a.states[blockIdx.y] : (int)blockIdx.y;
+    const float2* in = a.in + a.in_sstride * s;
+    float2* o = a.out + a.out_sstride * s;
+    load_tile<CONTIG>(lds, in, ti, n, P, tid);
+    __syncthreads();
+    float2 v[kE];
+    if (n & 1) {
+      const int h1 = -(n + 1) / 2;  // -c + 1/2 with c = n/2 + 1
+#pragma unroll
+      for (int i = 0; i < kE; ++i) {
+        const int e = tid + i * kThreads;
+        if (e < tot) {
+          int p, j;
+          elem_pj<CONTIG>(e, n, P, p, j);
+          // x3[j] = x2[(j+h1) mod n], x2[i] = x1[n-1-i], x1[i] = x[(i-h1) mod n]
+          const int i2 = ((j + h1) % n + n) % n;
+          const int i1 = n - 1 - i2;
+          const int i0 = ((i1 - h1) % n + n) % n;
+          v[i] = CONTIG ? lds[p * n + i0] : lds[p + P * i0];
+        }
+      }
+    } else {
+      const double h1 = -(double)(n / 2 + 1) + 0.5;
+      for (int pass = 0; pass < 2; ++pass) {
+        const double c = (pass == 0 ? -h1 : h1) / (double)n;  // exp(-i h k), k = 2 pi m / n
+        fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+#pragma unroll
+        for (int i = 0; i < kE; ++i) {
+          const int e = tid + i * kThreads;
+          if (e < tot) {
+            int p, j;
+            elem_pj<CONTIG>(e, n, P, p, j);
+            lds[e] = cswap(cmul(lds[e], phase_cycles(c, freq_index(j, n))));
+          }
+        }
+        __syncthreads();
+        fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+        if (pass == 0) {
+          // back to image space (swap + scale) and reverse along the pencil
+#pragma unroll
+          for (int i = 0; i < kE; ++i) {
+            const int e = tid + i * kThreads;
+            if (e < tot) {
+              int p, j;
+              elem_pj<CONTIG>(e, n, P, p, j);
+              const int jr = n - 1 - j;
+              v[i] = cscale(cswap(CONTIG ? lds[p * n + jr] : lds[p + P * jr]), scale);
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < kE; ++i) {
+            const int e = tid + i * kThreads;
+            if (e < tot) lds[e] = v[i];
+          }
+          __syncthreads();
+        } else {
+#pragma unroll
+          for (int i = 0; i < kE; ++i) {
+            const int e = tid + i * kThreads;
+            if (e < tot) v[i] = cscale(cswap(lds[e]), scale);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) {
+        int p, j;
+        elem_pj<CONTIG>(e, n, P, p, j);
+        if (p < ti.Pv) o[gaddr<CONTIG>(ti, e, n, P)] = v[i];
+      }
+    }
+  }
+}
+
+hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t st) {
+  const int n = a.plan.n;
+  int ntiles;
+  if (a.axis == 0) ntiles = (a.n2 * a.n3 + a.P - 1) / a.P;
+  else ntiles = a.tiles_per_line * (a.axis == 1 ? a.n3 : a.n2);
+  (void)n;
+  dim3 grid(ntiles, (mode == MODE_MAP || mode == MODE_FLIP) ? grid_states : 1);
+  const bool contig = a.axis == 0;
+#define DIS_LAUNCH(C, M) hipLaunchKernelGGL((pass_kernel<C, M>), grid, dim3(kThreads), 0, st, a)
+  switch (mode) {
+    case MODE_MAP: if (contig) DIS_LAUNCH(true, MODE_MAP); else DIS_LAUNCH(false, MODE_MAP); break;
+    case MODE_BCAST: if (contig) DIS_LAUNCH(true, MODE_BCAST); else DIS_LAUNCH(false, MODE_BCAST); break;
+    case MODE_ACCUM: if (contig) DIS_LAUNCH(true, MODE_ACCUM); else DIS_LAUNCH(false, MODE_ACCUM); break;
+    case MODE_FLIP: if (contig) DIS_LAUNCH(true, MODE_FLIP); else DIS_LAUNCH(false, MODE_FLIP); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef DIS_LAUNCH
+  return hipGetLastError();
+}
+
+__global__ void twiddle_kernel(float2* tw, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)n, &s, &c);
+    tw[k] = make_float2((float)c, (float)s);
+  }
+}
+
+hipError_t launch_twiddles(float2* tw, int n, hipStream_t st) {
+  hipLaunchKernelGGL(twiddle_kernel, dim3((n + 255) / 256), dim3(256), 0, st, tw, n);
+  return hipGetLastError();
+}
+
+__global__ void bcast_kernel(const float2* __restrict__ x, float2* __restrict__ out, long long nvox, int S) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nvox) {
+    const float2 v = x[i];
+    for (int s = 0; s < S; ++s) out[i + nvox * s] = v;
+  }
+}
+
+hipError_t launch_bcast(const float2* x, float2* out, long long nvox, int S, hipStream_t st) {
+  hipLaunchKernelGGL(bcast_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st, x, out, nvox, S);
+  return hipGetLastError();
+}
+
+__global__ void sum_states_kernel(const float2* __restrict__ in, long long sstride, float2* __restrict__ out,
+                                  long long nvox, int S, int accumulate) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nvox) {
+    float2 acc = accumulate ? out[i] : make_float2(0.f, 0.f);
+    for (int s = 0; s < S; ++s) acc = cadd(acc, in[i + sstride * s]);
+    out[i] = acc;
+  }
+}
+
+hipError_t launch_sum_states(const float2* in, long long sstride, float2* out, long long nvox, int S, int accumulate,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(sum_states_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st, in, sstride, out,
+                     nvox, S, accumulate);
+  return hipGetLastError();
+}
+
+// per-sample energy: sum over planes (in fixed order) of the per-(sample,
+// plane) coil sums, weighted by the readout mask Mbe (or 1).
+__global__ void energy_reduce_kernel(const float* __restrict__ epart, int Ns, int n3, const float* __restrict__ Mbe,
+                                     double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < Ns) {
+    double acc = 0.0;
+    for (int i3 = 0; i3 < n3; ++i3) {
+      const double w = Mbe ? (double)Mbe[i3] : 1.0;
+      acc += w * (double)epart[j + (long long)Ns * i3];
+    }
+    out[j] = acc;
+  }
+}
+
+hipError_t launch_energy_reduce(const float* epart, int Ns, int n3, const float* Mbe, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(energy_reduce_kernel, dim3((Ns + 255) / 256), dim3(256), 0, st, epart, Ns, n3, Mbe, out);
+  return hipGetLastError();
+}
+
+}  // namespace dis

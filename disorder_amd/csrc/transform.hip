@@ -1,0 +1,446 @@
+// Host orchestration: the pass sequence of the sinc rigid transform
+// (Methods/sincRigidTransform.m:38-114) and the batching of motion states
+// through the coil stage (Reconstruction/encode.m, decode.m, encodedecode.m).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "transform.hpp"
+
+namespace dis {
+
+namespace {
+
+constexpr int kPer0[3] = {0, 2, 1};  // per(1,:) - 1
+constexpr int kPer1[3] = {1, 0, 2};  // per(2,:) - 1
+
+#define DCHK(c, expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) return ctx_fail((c), DIS_E_HIP, hipGetErrorString(e_)); \
+  } while (0)
+
+double wrap_to_pi(double a) {  // Mapping Toolbox wrapToPi
+  if (a < -M_PI || a > M_PI) {
+    double b = a + M_PI;
+    const bool pos = b > 0;
+    b = b - std::floor(b / (2 * M_PI)) * (2 * M_PI);
+    if (b == 0 && pos) b = 2 * M_PI;
+    a = b - M_PI;
+  }
+  return a;
+}
+
+double wrap_to_pi_half(double a) {  // Methods/wrapToPiHalf.m:9-12
+  a = a - std::floor(a / M_PI) * M_PI;
+  if (a > M_PI / 2) a -= M_PI;
+  return a;
+}
+
+// Per-state factors of Methods/precomputeFactorsSincRigidTransform.m:45-60.
+struct StateParam {
+  double tan2[3], sn[3], t[3];
+  bool flip[3];
+};
+
+StateParam state_param(const double* T, int Sld, int s) {
+  StateParam p;
+  for (int d = 0; d < 3; ++d) p.t[d] = T[s + (size_t)Sld * d];
+  for (int m = 0; m < 3; ++m) {
+    double th = wrap_to_pi(T[s + (size_t)Sld * (3 + m)]);
+    p.flip[m] = std::fabs(th) > M_PI / 2;
+    th = wrap_to_pi_half(th);
+    p.tan2[m] = std::tan(th / 2);
+    p.sn[m] = std::sin(th);
+  }
+  return p;
+}
+
+// One step of the pass sequence: a fused shear/translation pass (flip_m < 0)
+// or the flip of rotation flip_m along `axis`.
+struct Step {
+  int axis, src;
+  int flip_m;
+  size_t offA, offB;  // staging offsets of the [S] coefficient arrays
+  bool hasA, hasB;
+  size_t offList;     // flip: staging offset of the flagged state list
+  int nlist;
+};
+
+// Build the pass sequence of the forward (di=1) or inverse (di=0) transform
+// for states [0,S) of T (rows s0..s0+S-1 of a [Sld x 6] array).
+std::vector<Step> build_steps(const double* T, int S, int Sld, int di, bool skip_t1, Staging& st) {
+  std::vector<StateParam> sp(S);
+  for (int s = 0; s < S; ++s) sp[s] = state_param(T, Sld, s);
+  const double sgn_tan = di == 1 ? 1.0 : -1.0;  // (-1)^(di-1)
+  const double sgn_sin = di == 1 ? -1.0 : 1.0;  // (-1)^di
+  const double sgn_t = di == 1 ? -1.0 : 1.0;    // (-1)^di
+  std::vector<double> buf(S);
+  auto shear = [&](int axis, int src, int m, bool tan_kind, int tdim) {
+    Step stp{};
+    stp.axis = axis;
+    stp.src = src;
+    stp.flip_m = -1;
+    stp.hasA = m >= 0;
+    stp.hasB = tdim >= 0;
+    if (stp.hasA) {
+      for (int s = 0; s < S; ++s) buf[s] = tan_kind ? sgn_tan * sp[s].tan2[m] : sgn_sin * sp[s].sn[m];
+      stp.offA = st.add(buf.data(), sizeof(double) * S);
+    }
+    if (stp.hasB) {
+      for (int s = 0; s < S; ++s) buf[s] = sgn_t * sp[s].t[tdim];
+      stp.offB = st.add(buf.data(), sizeof(double) * S);
+    }
+    return stp;
+  };
+  auto flips = [&](int m, std::vector<Step>& out) {
+    std::vector<int> lst;
+    for (int s = 0; s < S; ++s)
+      if (sp[s].flip[m]) lst.push_back(s);
+    if (lst.empty()) return;
+    const size_t o = st.add(lst.data(), sizeof(int) * lst.size());
+    for (int ax : {kPer0[m], kPer1[m]}) {
+      Step f{};
+      f.axis = ax;
+      f.src = -1;
+      f.flip_m = m;
+      f.offList = o;
+      f.nlist = (int)lst.size();
+      out.push_back(f);
+    }
+  };
+  std::vector<Step> steps;
+  if (di == 1) {
+    for (int m = 0; m < 3; ++m) {
+      const int a = kPer0[m], b = kPer1[m];
+      flips(m, steps);
+      steps.push_back(shear(a, b, m, true, -1));
+      steps.push_back(shear(b, a, m, false, -1));
+      // the dim-2 translation joins the last shear along dim 2 (:53, :66-71)
+      steps.push_back(shear(a, b, m, true, m == 2 ? 1 : -1));
+    }
+    steps.push_back(shear(2, -1, -1, false, 2));
+    if (!skip_t1) steps.push_back(shear(0, -1, -1, false, 0));
+  } else {
+    if (!skip_t1) steps.push_back(shear(0, -1, -1, false, 0));
+    steps.push_back(shear(2, -1, -1, false, 2));
+    for (int m = 2; m >= 0; --m) {
+      const int a = kPer0[m], b = kPer1[m];
+      steps.push_back(shear(a, b, m, true, m == 2 ? 1 : -1));
+      steps.push_back(shear(b, a, m, false, -1));
+      steps.push_back(shear(a, b, m, true, -1));
+      flips(m, steps);
+    }
+  }
+  return steps;
+}
+
+PassArgs pass_args(const Grid& gr, const Step& stp, const unsigned char* dbase, int S) {
+  PassArgs a{};
+  a.n1 = gr.n[0];
+  a.n2 = gr.n[1];
+  a.n3 = gr.n[2];
+  a.axis = stp.axis;
+  a.src = stp.src;
+  const int n = gr.n[stp.axis];
+  int P = 1;
+  while (P * 2 * n <= kMaxTile && P * 2 <= 512) P *= 2;
+  if (stp.axis != 0 && P > gr.n[0]) P = gr.n[0];
+  a.P = P;
+  a.tiles_per_line = (gr.n[0] + P - 1) / P;
+  a.tw = gr.tw[stp.axis];
+  a.plan = gr.plan[stp.axis];
+  a.coefA = stp.hasA ? (const double*)(dbase + stp.offA) : nullptr;
+  a.coefB = stp.hasB ? (const double*)(dbase + stp.offB) : nullptr;
+  if (stp.src >= 0) {
+    a.rscale = (double)gr.Nfull[stp.src] / (double)gr.n[stp.src];
+    a.rcent = (gr.n[stp.src] + 2) / 2;  // ceil((n+1)/2)
+  }
+  a.invN = 1.0 / (double)gr.Nfull[stp.axis];
+  a.S = S;
+  a.states = stp.flip_m >= 0 ? (const int*)(dbase + stp.offList) : nullptr;
+  return a;
+}
+
+int run_steps(dis_ctx* c, const Grid& gr, const std::vector<Step>& steps, const unsigned char* dbase, int S,
+              const float2* in, long long in_ss, float2* W, bool bcast_first, float2* final_out, int final_mode,
+              bool accumulate) {
+  hipStream_t stream = ctx_stream(c);
+  const long long nv = gr.nvox;
+  for (size_t i = 0; i < steps.size(); ++i) {
+    const Step& stp = steps[i];
+    PassArgs a = pass_args(gr, stp, dbase, S);
+    const bool first = i == 0, last = i + 1 == steps.size();
+    a.in = first ? in : W;
+    a.in_sstride = first ? in_ss : nv;
+    a.out = W;
+    a.out_sstride = nv;
+    const double V = 8.0 * (double)nv;  // bytes of one complex64 volume
+    if (stp.flip_m >= 0) {
+      a.in = W;
+      a.in_sstride = nv;
+      const int ps = prof_begin(c, DIS_K_FLIP);
+      DCHK(c, launch_pass(a, MODE_FLIP, stp.nlist, stream));
+      prof_end(c, ps, 2.0 * V * stp.nlist);
+      continue;
+    }
+    if (first && bcast_first) {
+      const int ps = prof_begin(c, DIS_K_BCAST);
+      DCHK(c, launch_pass(a, MODE_BCAST, 1, stream));
+      prof_end(c, ps, V * (1.0 + S));
+    } else if (last && final_mode == MODE_ACCUM) {
+      a.out = final_out;
+      a.out_sstride = 0;
+      a.accumulate = accumulate ? 1 : 0;
+      const int ps = prof_begin(c, DIS_K_ACCUM);
+      DCHK(c, launch_pass(a, MODE_ACCUM, 1, stream));
+      prof_end(c, ps, V * (S + (accumulate ? 2.0 : 1.0)));
+    } else {
+      if (last && final_out) {
+        a.out = final_out;
+      }
+      const int ps = prof_begin(c, DIS_K_PASS);
+      DCHK(c, launch_pass(a, MODE_MAP, S, stream));
+      prof_end(c, ps, 2.0 * V * S);
+    }
+  }
+  return DIS_OK;
+}
+
+}  // namespace
+
+int make_plan(int n, FftPlan* pl) {
+  if (n <= 0 || n > 1024) return DIS_E_UNSUPPORTED;
+  pl->n = n;
+  pl->nst = 0;
+  int m = n;
+  for (int r : {7, 5, 3}) {
+    while (m % r == 0) {
+      if (pl->nst >= kMaxStages) return DIS_E_UNSUPPORTED;
+      pl->radix[pl->nst++] = r;
+      m /= r;
+    }
+  }
+  int a = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    ++a;
+  }
+  if (m != 1) return DIS_E_UNSUPPORTED;
+  std::vector<int> two;
+  while (a >= 4) {
+    two.push_back(16);
+    a -= 4;
+  }
+  if (a == 3) two.push_back(8);
+  if (a == 2) two.push_back(4);
+  if (a == 1) two.push_back(2);
+  // powers of two first (the largest radix leads)
+  std::vector<int> all(two.begin(), two.end());
+  for (int s = 0; s < pl->nst; ++s) all.push_back(pl->radix[s]);
+  if ((int)all.size() > kMaxStages) return DIS_E_UNSUPPORTED;
+  pl->nst = (int)all.size();
+  for (int s = 0; s < pl->nst; ++s) pl->radix[s] = all[s];
+  return DIS_OK;
+}
+
+int make_grid(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], Grid* g) {
+  for (int d = 0; d < 3; ++d) {
+    if (n[d] <= 0 || Nfull[d] <= 0) return ctx_fail(c, DIS_E_INVALID, "grid sizes must be positive");
+    g->n[d] = n[d];
+    g->Nfull[d] = Nfull[d];
+    if (make_plan(n[d], &g->plan[d]) != DIS_OK)
+      return ctx_fail(c, DIS_E_UNSUPPORTED, "grid size must be <= 1024 with prime factors 2, 3, 5, 7");
+    int st = ctx_twiddles(c, n[d], &g->tw[d]);
+    if (st) return st;
+  }
+  g->nvox = (long long)n[0] * n[1] * n[2];
+  return DIS_OK;
+}
+
+int rigid_forward(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, const float2* x, long long x_sstride,
+                  float2* U, bool skip_t1) {
+  Staging st;
+  std::vector<Step> steps = build_steps(T, S, Sld, 1, skip_t1, st);
+  const unsigned char* d = nullptr;
+  int rc = ctx_upload_bytes(c, st, &d);
+  if (rc) return rc;
+  hipStream_t stream = ctx_stream(c);
+  const bool flip0 = !steps.empty() && steps[0].flip_m == 0;
+  bool bcast = x_sstride == 0 && !flip0;
+  const float2* in = x;
+  long long in_ss = x_sstride;
+  if (flip0) {  // materialise the states before flipping some of them
+    if (x_sstride == 0) {
+      DCHK(c, launch_bcast(x, U, gr.nvox, S, stream));
+    } else if (x != U) {
+      DCHK(c, hipMemcpyAsync(U, x, sizeof(float2) * gr.nvox * S, hipMemcpyDeviceToDevice, stream));
+    }
+    in = U;
+    in_ss = gr.nvox;
+  }
+  return run_steps(c, gr, steps, d, S, in, in_ss, U, bcast, nullptr, MODE_MAP, false);
+}
+
+int rigid_inverse(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, const float2* V, long long v_sstride,
+                  float2* work, float2* out, bool sumR, bool skip_t1, bool accumulate) {
+  Staging st;
+  std::vector<Step> steps = build_steps(T, S, Sld, 0, skip_t1, st);
+  const unsigned char* d = nullptr;
+  int rc = ctx_upload_bytes(c, st, &d);
+  if (rc) return rc;
+  hipStream_t stream = ctx_stream(c);
+  const bool flip_last = steps.back().flip_m >= 0;
+  float2* W = sumR ? work : out;
+  if (sumR && !flip_last) return run_steps(c, gr, steps, d, S, V, v_sstride, W, false, out, MODE_ACCUM, accumulate);
+  rc = run_steps(c, gr, steps, d, S, V, v_sstride, W, false, nullptr, MODE_MAP, false);
+  if (rc) return rc;
+  if (sumR) DCHK(c, launch_sum_states(W, gr.nvox, out, gr.nvox, S, accumulate ? 1 : 0, stream));
+  return DIS_OK;
+}
+
+int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* yin, float2* yout,
+             float2* xout, int mode, EnergyOut* eo) {
+  const dis_geom& g = v.g;
+  Grid gr;
+  int rc = make_grid(c, g.n, g.Nfull, &gr);
+  if (rc) return rc;
+  hipStream_t stream = ctx_stream(c);
+  const int S = g.nstates;
+  bool anyT = false;
+  for (int i = 0; i < 6 * g.nmotion; ++i)
+    if (T[i] != 0.0) anyT = true;
+  const int NM = anyT ? g.nmotion : 0;  // encode.m:61 -- no transform at all when T == 0
+  const bool fwd = mode != COIL_DECODE;
+  const bool adj = mode == COIL_DECODE || mode == COIL_EHE;
+  const long long nv = gr.nvox;
+  const size_t vol = sizeof(float2) * nv;
+  int maxnec = 0;
+  for (int s = 0; s < S; ++s) maxnec = std::max(maxnec, v.hoff[s + 1] - v.hoff[s]);
+  // E^H E runs in place (U -> U) only when one workgroup pass covers all coils and samples
+  const bool inplace = !(mode == COIL_EHE && (g.ncoils > 8 || maxnec > 4 * kThreads));
+  const int vols_per_state = inplace ? 1 : 2;
+
+  // workspace: [Sb (x2) volumes][energy partials Ns*n3 floats][per-sample doubles Ns]
+  const size_t ns = (size_t)std::max(1, g.nsamples);
+  const size_t epbytes = mode == COIL_ENERGY ? ((sizeof(float) * ns * g.n[2] + 255) & ~size_t(255)) : 0;
+  const size_t esbytes = mode == COIL_ENERGY ? sizeof(double) * ns : 0;
+  size_t lim = ctx_workspace_limit(c);
+  int Sb = std::max(1, S);
+  if (lim) {
+    if (lim < vol * vols_per_state + epbytes + esbytes) return ctx_fail(c, DIS_E_NOMEM, "workspace limit below one state");
+    Sb = (int)std::min<size_t>((size_t)Sb, (lim - epbytes - esbytes) / (vol * vols_per_state));
+  }
+  void* ws = nullptr;
+  rc = ctx_workspace(c, vol * Sb * vols_per_state + epbytes + esbytes, &ws);
+  if (rc) return rc;
+  float2* U = (float2*)ws;
+  float2* Vb = inplace ? U : U + nv * Sb;
+  float* epart = (float*)((char*)ws + vol * Sb * vols_per_state);
+  double* esamp = (double*)((char*)epart + epbytes);
+
+  const int G = std::max(1, std::min(g.n[1], kMaxTile / g.n[0]));
+  bool first_out = true;
+  auto coil_args = [&](int s0, const float2* Usrc, long long us) {
+    CoilArgs a{};
+    a.U = Usrc;
+    a.us = us;
+    a.us_rel = 1;
+    a.V = Vb;
+    a.vs = nv;
+    a.vs_rel = 1;
+    a.Smaps = v.S;
+    a.n1 = g.n[0];
+    a.n2 = g.n[1];
+    a.n3 = g.n[2];
+    a.C = g.ncoils;
+    a.k1 = v.k1;
+    a.k2 = v.k2;
+    a.off = v.off;
+    a.We = v.We;
+    a.Ns = g.nsamples;
+    a.yin = yin;
+    a.yout = yout;
+    a.epart = epart;
+    a.invN1 = 1.0 / (double)g.Nfull[0];
+    a.tw1 = gr.tw[0];
+    a.tw2 = gr.tw[1];
+    a.plan1 = gr.plan[0];
+    a.G = G;
+    a.state0 = s0;
+    return a;
+  };
+
+  // motion states, in batches of Sb
+  for (int s0 = 0; s0 < NM; s0 += Sb) {
+    const int sb = std::min(Sb, NM - s0);
+    if (fwd) {
+      rc = rigid_forward(c, gr, T + s0, sb, g.nmotion, x, 0, U, true);
+      if (rc) return rc;
+    }
+    Staging st;
+    std::vector<double> tf(sb), ta(sb);
+    for (int s = 0; s < sb; ++s) {
+      tf[s] = -T[s0 + s];  // forward e1 along dim 1: (-1)^1 * t1
+      ta[s] = T[s0 + s];   // inverse: (-1)^0 * t1
+    }
+    const size_t otf = st.add(tf.data(), sizeof(double) * sb);
+    const size_t ota = st.add(ta.data(), sizeof(double) * sb);
+    const unsigned char* d = nullptr;
+    rc = ctx_upload_bytes(c, st, &d);  // stream-ordered after the transform kernels
+    if (rc) return rc;
+    CoilArgs a = coil_args(s0, U, nv);
+    a.tfwd = fwd ? (const double*)(d + otf) - s0 : nullptr;
+    a.tadj = adj ? (const double*)(d + ota) - s0 : nullptr;
+    {
+      const int ps = prof_begin(c, DIS_K_COIL);
+      DCHK(c, launch_coil(a, mode, sb, stream));
+      // pass model of the coil stage per state: (6C+2) volumes (E^H E), (3C+1) forward or adjoint only
+      const double cv = mode == COIL_EHE ? 6.0 * g.ncoils + 2 : 3.0 * g.ncoils + 1;
+      prof_end(c, ps, cv * 8.0 * (double)nv * sb);
+    }
+    if (adj) {
+      rc = rigid_inverse(c, gr, T + s0, sb, g.nmotion, Vb, nv, Vb, xout, true, true, !first_out);
+      if (rc) return rc;
+      first_out = false;
+    }
+  }
+  // segments without a transform: the coil stage reads x directly
+  for (int s0 = NM; s0 < S; s0 += Sb) {
+    const int sb = std::min(Sb, S - s0);
+    CoilArgs a = coil_args(s0, x, 0);
+    DCHK(c, launch_coil(a, mode, sb, stream));
+    if (adj) {
+      DCHK(c, launch_sum_states(Vb, nv, xout, nv, sb, first_out ? 0 : 1, stream));
+      first_out = false;
+    }
+  }
+  if (mode == COIL_ENERGY && eo) {
+    const int Ns = g.nsamples;
+    const unsigned char* d = nullptr;
+    if (eo->Mbe) {
+      Staging st;
+      st.add(eo->Mbe, sizeof(float) * g.n[2]);
+      rc = ctx_upload_bytes(c, st, &d);
+      if (rc) return rc;
+    }
+    std::vector<double> h(ns, 0.0);
+    if (Ns > 0) {
+      DCHK(c, launch_energy_reduce(epart, Ns, g.n[2], eo->Mbe ? (const float*)d : nullptr, esamp, stream));
+      DCHK(c, hipMemcpyAsync(h.data(), esamp, sizeof(double) * Ns, hipMemcpyDeviceToHost, stream));
+    }
+    DCHK(c, hipStreamSynchronize(stream));
+    if (eo->per_sample) std::copy(h.begin(), h.begin() + Ns, eo->per_sample);
+    if (eo->total) {
+      double t = 0;
+      for (int j = 0; j < Ns; ++j) t += h[j];
+      *eo->total = t;
+    }
+  }
+  return DIS_OK;
+}
+
+}  // namespace dis

@@ -1,0 +1,82 @@
+// Host orchestration of the sinc rigid transform and of the encoding operator.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstring>
+#include <vector>
+
+#include "../../include/disorder.h"
+#include "kernels.hpp"
+
+namespace dis {
+
+// Level grid with its FFT plans and twiddle tables.
+struct Grid {
+  int n[3];
+  int Nfull[3];
+  long long nvox;
+  FftPlan plan[3];
+  const float2* tw[3];
+};
+
+// Device-side view of a plan (sampling + coil maps).
+struct OpView {
+  dis_geom g;
+  const float2* S;
+  const int* k1;
+  const int* k2;
+  const int* off;
+  const float* We;
+  const int* hoff;  // host copy of the offsets
+};
+
+struct EnergyOut {
+  const float* Mbe = nullptr;  // host [n3] or null
+  double* per_sample = nullptr;  // host [Ns] or null
+  double* total = nullptr;       // host or null
+};
+
+// Collects small host arrays for one upload; offsets are 16-byte aligned.
+struct Staging {
+  std::vector<unsigned char> bytes;
+  size_t add(const void* p, size_t nbytes) {
+    size_t o = (bytes.size() + 15) & ~size_t(15);
+    bytes.resize(o + nbytes);
+    if (nbytes) std::memcpy(bytes.data() + o, p, nbytes);
+    return o;
+  }
+};
+
+// context helpers implemented in api.hip
+int ctx_fail(dis_ctx* c, int code, const char* msg);
+hipStream_t ctx_stream(dis_ctx* c);
+int ctx_twiddles(dis_ctx* c, int n, const float2** out);
+int ctx_workspace(dis_ctx* c, size_t bytes, void** out);
+size_t ctx_workspace_limit(dis_ctx* c);
+int ctx_upload_bytes(dis_ctx* c, const Staging& st, const unsigned char** dptr);
+
+// Event timing around a launch when profiling is enabled (no-ops otherwise).
+int prof_begin(dis_ctx* c, int kclass);  // returns a slot or -1
+void prof_end(dis_ctx* c, int slot, double bytes);
+
+int make_plan(int n, FftPlan* pl);
+int make_grid(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], Grid* g);
+
+// T: host [Sld x 6] column-major; the states transformed are [0, S).
+// Forward: U[s] = T_s x for s < S.  x_sstride = 0 broadcasts one volume.
+// skip_t1 leaves out the final dim-1 translation (fused into the coil stage).
+int rigid_forward(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, const float2* x,
+                  long long x_sstride, float2* U, bool skip_t1);
+// Inverse: out = sum_s T_s^H V[s] (sumR) or out[s] = T_s^H V[s].  `work`
+// (S writable volumes, may equal V) holds the intermediate states when sumR.  skip_t1 leaves out the
+// first dim-1 translation (fused into the coil stage).  accumulate adds into
+// out (sumR only).
+int rigid_inverse(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, const float2* V,
+                  long long v_sstride, float2* work, float2* out, bool sumR, bool skip_t1, bool accumulate);
+
+// E, E^H, E^H E and the residual energy through one coil-stage mode.
+int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* yin, float2* yout,
+             float2* xout, int mode, EnergyOut* eo);
+
+}  // namespace dis

@@ -1,0 +1,264 @@
+"""Host-side mirror of DISORDER's hot-path interface over the C-ABI.
+
+Arrays use MATLAB's memory layout: a volume is a complex64 torch tensor of
+shape (n1, n2, n3) with column-major strides (see ``colmajor``); per-state
+stacks are (n1, n2, n3, S); compact k-space data is (Ns, n3, C).  Device
+tensors are passed to the library as raw pointers; all work runs in
+libdisorder_hip.so on the context's stream.
+
+Function names follow the reference: ``sincRigidTransform``
+(Methods/sincRigidTransform.m:1), ``encode`` (Reconstruction/encode.m:1),
+``decode`` (Reconstruction/decode.m:1), ``encodedecode``
+(Reconstruction/encodedecode.m:1), ``computeEnergy``
+(Reconstruction/computeEnergy.m:1).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import DisGeom, check
+
+
+def colmajor(t: torch.Tensor) -> torch.Tensor:
+    """Same values and shape, column-major (MATLAB) strides."""
+    nd = t.dim()
+    perm = list(range(nd))[::-1]
+    return t.permute(*perm).contiguous().permute(*perm)
+
+
+def is_colmajor(t: torch.Tensor) -> bool:
+    st = 1
+    for d in range(t.dim()):
+        if t.shape[d] != 1 and t.stride(d) != st:
+            return False
+        st *= t.shape[d]
+    return True
+
+
+def empty_colmajor(shape, device, dtype=torch.complex64):
+    return torch.empty(tuple(shape)[::-1], dtype=dtype, device=device).permute(*range(len(shape) - 1, -1, -1))
+
+
+def _ptr(t: torch.Tensor):
+    if t.dtype != torch.complex64:
+        raise TypeError("complex64 tensors are required")
+    if not t.is_cuda:
+        raise TypeError("device tensors are required")
+    if not is_colmajor(t):
+        raise ValueError("tensors must have column-major (MATLAB) strides; use colmajor()")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _tbuf(T, rows=None):
+    """Motion [S x 6] -> column-major float64 host buffer."""
+    T = np.atleast_2d(np.asarray(T, dtype=np.float64))
+    if T.shape[1] != 6:
+        raise ValueError("T must be [S x 6]")
+    flat = np.ascontiguousarray(T.T).ravel()  # T[s + S*g]
+    return flat, flat.ctypes.data_as(_lib.c_double_p)
+
+
+class Context:
+    """One GPU, one stream (dis_ctx)."""
+
+    def __init__(self, device: int = 0, workspace_limit: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.lib.dis_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = torch.device("cuda", device)
+        if workspace_limit:
+            check(self.lib.dis_set_workspace_limit(self.h, workspace_limit), self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dis_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(self.lib.dis_sync(self.h), self.h)
+
+    def profile(self, on=True):
+        check(self.lib.dis_profile_enable(self.h, 1 if on else 0), self.h)
+
+    def profile_read(self):
+        """{kernel class: (launches, total ms, algorithmic bytes)}"""
+        out = {}
+        for k, name in _lib.K_NAMES.items():
+            n = ctypes.c_int64(0)
+            ms = ctypes.c_double(0)
+            by = ctypes.c_double(0)
+            check(self.lib.dis_profile_read(self.h, k, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)), self.h)
+            out[name] = (n.value, ms.value, by.value)
+        return out
+
+    def _prep(self):
+        # order the library's stream after torch's producers of the inputs
+        torch.cuda.current_stream(self.device).synchronize()
+
+
+_default_ctx = {}
+
+
+def default_context(device: int = 0) -> Context:
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+def sincRigidTransform(x: torch.Tensor, T, di: int = 1, sumR: bool = True, Nfull=None, ctx: Context = None):
+    """Methods/sincRigidTransform.m:1 with the factors of
+    precomputeFactorsSincRigidTransform(kG, rkG, T, di, 0, 1, 1).
+
+    di=1: x (n1,n2,n3) or (n1,n2,n3,S) -> (n1,n2,n3,S).
+    di=0: x (n1,n2,n3,S) -> (n1,n2,n3) if sumR else (n1,n2,n3,S)."""
+    ctx = ctx or default_context(x.device.index or 0)
+    Tf, Tp = _tbuf(T)
+    S = Tf.size // 6
+    n = (ctypes.c_int32 * 3)(*x.shape[:3])
+    N = (ctypes.c_int32 * 3)(*(Nfull if Nfull is not None else x.shape[:3]))
+    if di == 1:
+        per_state = 1 if x.dim() == 4 and x.shape[3] > 1 else 0
+        if x.dim() == 4 and x.shape[3] not in (1, S):
+            raise ValueError("state count of x does not match T")
+        out = empty_colmajor(tuple(x.shape[:3]) + (S,), x.device)
+    else:
+        if x.dim() != 4 or x.shape[3] != S:
+            raise ValueError("x must be (n1,n2,n3,S) for the inverse")
+        per_state = 1
+        out = empty_colmajor(tuple(x.shape[:3]) + ((S,) if not sumR else ()), x.device)
+    ctx._prep()
+    check(ctx.lib.dis_rigid_transform(ctx.h, n, N, Tp, S, di, int(bool(sumR)), per_state, _ptr(x), _ptr(out)),
+          ctx.h)
+    return out
+
+
+class Encoding:
+    """The E / EH structures of the reconstruction path as one plan
+    (dis_plan): coil maps Sf, per-segment sampling (E.Fs rows as FFT-order
+    indices k1/k2, offsets E.nSt), weights EH.We and NMs motion states."""
+
+    def __init__(self, Smaps: torch.Tensor, k1, k2, state_off, Nfull=None, We=None, nmotion=None,
+                 ctx: Context = None):
+        self.ctx = ctx or default_context(Smaps.device.index or 0)
+        self.S = Smaps  # kept alive: the plan borrows the pointer
+        n = tuple(Smaps.shape[:3])
+        C = Smaps.shape[3]
+        self.n = n
+        self.C = C
+        self.k1 = np.ascontiguousarray(k1, dtype=np.int32)
+        self.k2 = np.ascontiguousarray(k2, dtype=np.int32)
+        self.off = np.ascontiguousarray(state_off, dtype=np.int32)
+        nst = len(self.off) - 1
+        self.nstates = nst
+        self.nmotion = nst if nmotion is None else int(nmotion)
+        self.Ns = int(self.off[-1])
+        g = DisGeom()
+        g.n[:] = n
+        g.Nfull[:] = tuple(Nfull) if Nfull is not None else n
+        g.ncoils = C
+        g.nstates = nst
+        g.nmotion = self.nmotion
+        g.nsamples = self.Ns
+        self.geom = g
+        wp = None
+        if We is not None:
+            self.We = np.ascontiguousarray(We, dtype=np.float32)
+            wp = self.We.ctypes.data_as(_lib.c_float_p)
+        h = ctypes.c_void_p()
+        check(self.ctx.lib.dis_plan_create(self.ctx.h, ctypes.byref(g), _ptr(Smaps),
+                                           self.k1.ctypes.data_as(_lib.c_int32_p),
+                                           self.k2.ctypes.data_as(_lib.c_int32_p),
+                                           self.off.ctypes.data_as(_lib.c_int32_p), wp, ctypes.byref(h)),
+              self.ctx.h)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.dis_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_weights(self, We):
+        if We is None:
+            check(self.ctx.lib.dis_plan_set_weights(self.h, None), self.ctx.h)
+        else:
+            self.We = np.ascontiguousarray(We, dtype=np.float32)
+            check(self.ctx.lib.dis_plan_set_weights(self.h, self.We.ctypes.data_as(_lib.c_float_p)), self.ctx.h)
+
+    def _T(self, T):
+        if self.nmotion == 0:
+            return np.zeros(6), None
+        Tf, Tp = _tbuf(T)
+        if Tf.size != 6 * self.nmotion:
+            raise ValueError("T must be [nmotion x 6]")
+        return Tf, Tp
+
+    def encode(self, x, T, out=None):
+        Tf, Tp = self._T(T)
+        y = out if out is not None else empty_colmajor((self.Ns, self.n[2], self.C), x.device)
+        self.ctx._prep()
+        check(self.ctx.lib.dis_encode(self.h, Tp, _ptr(x), _ptr(y)), self.ctx.h)
+        return y
+
+    def decode(self, y, T, out=None):
+        Tf, Tp = self._T(T)
+        x = out if out is not None else empty_colmajor(self.n, y.device)
+        self.ctx._prep()
+        check(self.ctx.lib.dis_decode(self.h, Tp, _ptr(y), _ptr(x)), self.ctx.h)
+        return x
+
+    def encodedecode(self, x, T, out=None):
+        Tf, Tp = self._T(T)
+        o = out if out is not None else empty_colmajor(self.n, x.device)
+        self.ctx._prep()
+        check(self.ctx.lib.dis_encodedecode(self.h, Tp, _ptr(x), _ptr(o)), self.ctx.h)
+        return o
+
+    def energy(self, x, y, T, Mbe=None, per_sample=False):
+        Tf, Tp = self._T(T)
+        ps = np.zeros(max(1, self.Ns), np.float64)
+        tot = ctypes.c_double(0.0)
+        mb = None
+        if Mbe is not None:
+            self._Mbe = np.ascontiguousarray(Mbe, dtype=np.float32)
+            mb = self._Mbe.ctypes.data_as(_lib.c_float_p)
+        self.ctx._prep()
+        check(self.ctx.lib.dis_energy(self.h, Tp, _ptr(x), _ptr(y), mb, ps.ctypes.data_as(_lib.c_double_p),
+                                      ctypes.byref(tot)), self.ctx.h)
+        return ps[:self.Ns] if per_sample else tot.value
+
+
+def encode(x, E: Encoding, T):
+    """Reconstruction/encode.m:1 -- y = E x."""
+    return E.encode(x, T)
+
+
+def decode(y, E: Encoding, T):
+    """Reconstruction/decode.m:1 -- x = E^H (We .* y)."""
+    return E.decode(y, T)
+
+
+def encodedecode(x, E: Encoding, T):
+    """Reconstruction/encodedecode.m:1 -- E^H We E x."""
+    return E.encodedecode(x, T)
+
+
+def computeEnergy(y, x, E: Encoding, T, Mbe=None, per_sample=False):
+    """Reconstruction/computeEnergy.m:34-57 (no=2, FP64 accumulation)."""
+    return E.energy(x, y, T, Mbe=Mbe, per_sample=per_sample)

@@ -1,0 +1,131 @@
+/* C-ABI of the MI355X-native DISORDER aligned-reconstruction hot path.
+ *
+ * Every entry point returns an int status (DIS_OK = 0, negative on error);
+ * the message of the last error of a context is dis_last_error(ctx).
+ * Nothing unwinds across this boundary.
+ *
+ * Data conventions (match MATLAB's mxGPUArray / mxGetComplexSingles layout,
+ * so a MEX gateway passes pointers through without copies or transposes):
+ *   - complex data is interleaved single precision (re, im) pairs;
+ *   - arrays are column-major; a volume is [n1, n2, n3] with dims 1-2 the
+ *     phase-encode plane and dim 3 the readout;
+ *   - a stack of per-state volumes is [n1, n2, n3, S] (MATLAB keeps the state
+ *     on dim 5 with singleton dim 4: same memory);
+ *   - compact k-space data y is [Ns, n3, C] (MATLAB [Ns, 1, n3, C]); samples
+ *     of all states are concatenated in state order;
+ *   - motion T is a HOST array [S x 6] column-major (T[s + S*g]), g = 0..2
+ *     translations in full-resolution voxels, g = 3..5 rotations in radians
+ *     (MATLAB E.Tr [1,1,1,1,S,6]);
+ *   - every array argument named x, y, out, S_maps, G is a DEVICE pointer on
+ *     the context's GPU; every other pointer is a host pointer.
+ * A context is not reentrant: use one thread per context.  All calls are
+ * synchronous on return.
+ */
+#ifndef DISORDER_H
+#define DISORDER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DIS_OK 0
+#define DIS_E_INVALID (-1)     /* bad argument or shape */
+#define DIS_E_UNSUPPORTED (-2) /* feature of the reference not supported here */
+#define DIS_E_NOMEM (-3)       /* device allocation failed or workspace limit too small */
+#define DIS_E_HIP (-4)         /* HIP runtime error */
+#define DIS_E_NODEVICE (-5)    /* no usable GPU */
+
+typedef struct dis_ctx dis_ctx;
+typedef struct dis_plan dis_plan;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dis_version(void);
+/* Static text of a status code. */
+const char* dis_status_string(int status);
+
+/* One HIP device, one stream, a growable workspace. */
+int dis_ctx_create(int device, dis_ctx** out);
+int dis_ctx_destroy(dis_ctx* ctx);
+const char* dis_last_error(const dis_ctx* ctx);
+int dis_sync(dis_ctx* ctx);
+/* Upper bound on the workspace in bytes (0 = no bound).  Larger problems are
+ * processed in batches of motion states that fit. */
+int dis_set_workspace_limit(dis_ctx* ctx, size_t bytes);
+
+/* sincRigidTransform with precomputeFactorsSincRigidTransform(kG,rkG,T,di,0,1,1)
+ * (replaces Methods/sincRigidTransform.m:1 and
+ * Methods/precomputeFactorsSincRigidTransform.m:1 as called from
+ * Reconstruction/encode.m:67-69 and decode.m:104-106).
+ *   n      : level grid [n1,n2,n3] of x;  Nfull : full-resolution grid
+ *            (generateTransformGrids(Nfull,gpu,n,ceil((n+1)/2),1),
+ *            Methods/downsampleOperators.m:63)
+ *   di = 1 : x is [n] (x_per_state = 0) or [n,S] (= 1); out is [n,S]
+ *   di = 0 : x is [n,S]; out is [n] when sumR, else [n,S]
+ * Rotations beyond 90 degrees take the flip path of Methods/flipping.m. */
+int dis_rigid_transform(dis_ctx* ctx, const int32_t n[3], const int32_t Nfull[3], const double* T, int32_t S,
+                        int32_t di, int32_t sumR, int32_t x_per_state, const void* x, void* out);
+
+/* Encoding operator description (the E/EH structs of Reconstruction/solveXT.m
+ * after Methods/buildHarmonicSampling.m):
+ *   n        : level grid of x, also the phase-encode grid of y (E.Uf, SENSE
+ *              folding, is not supported: NX must equal NY)
+ *   Nfull    : full-resolution grid of the motion parameters
+ *   ncoils   : C (E.Sf dim 4)
+ *   nstates  : segments (E.NSe); the first nmotion (E.NMs) are rigidly
+ *              transformed, the rest (outer segments, encode.m:76-79) are not
+ *   nsamples : Ns = sum of segment sizes */
+typedef struct {
+  int32_t n[3];
+  int32_t Nfull[3];
+  int32_t ncoils;
+  int32_t nstates;
+  int32_t nmotion;
+  int32_t nsamples;
+} dis_geom;
+
+/* S_maps: DEVICE [n1,n2,n3,C] coil sensitivities (E.Sf), borrowed: must
+ * outlive the plan.  k1, k2: 0-based FFT-order phase-encode indices of every
+ * sample (rows of E.Fs{1}, E.Fs{2}); state_off: [nstates+1] segment offsets
+ * (E.nSt/E.nEc); We: [Ns] sample weights (EH.We) or NULL. */
+int dis_plan_create(dis_ctx* ctx, const dis_geom* g, const void* S_maps, const int32_t* k1, const int32_t* k2,
+                    const int32_t* state_off, const float* We, dis_plan** out);
+int dis_plan_destroy(dis_plan* plan);
+/* Replace EH.We ([Ns] host, or NULL for none). */
+int dis_plan_set_weights(dis_plan* plan, const float* We);
+
+/* y = E x  (Reconstruction/encode.m:1).  T: host [nmotion x 6]. */
+int dis_encode(dis_plan* plan, const double* T, const void* x, void* y);
+/* x = E^H (We .* y)  (Reconstruction/decode.m:1). */
+int dis_decode(dis_plan* plan, const double* T, const void* y, void* x);
+/* out = E^H We E x  (Reconstruction/encodedecode.m:1). */
+int dis_encodedecode(dis_plan* plan, const double* T, const void* x, void* out);
+/* Reconstruction/computeEnergy.m:34-57 (no = 2): residual E x - y with
+ * readout weights Mbe ([n3] host or NULL, EH.Mbe).  per_sample ([Ns] host or
+ * NULL) receives sum over readout and coils per sample (avdi); total (host or
+ * NULL) the sum over all samples.  Accumulated in FP64. */
+int dis_energy(dis_plan* plan, const double* T, const void* x, const void* y, const float* Mbe,
+               double* per_sample, double* total);
+
+/* Kernel classes for dis_profile_read. */
+#define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */
+#define DIS_K_BCAST 1  /* first forward pass, one input volume to all states */
+#define DIS_K_ACCUM 2  /* last inverse pass with the state sum */
+#define DIS_K_COIL 3   /* coil stage (coil multiply, dim-1 FFT, sampled dim-2 DFT, adjoint) */
+#define DIS_K_FLIP 4   /* >90 degree flip pass */
+#define DIS_K_COUNT 5
+
+/* Per-kernel-class timing with HIP events recorded around every launch on the
+ * context's stream (no counterpart in the reference, which only prints
+ * tic/toc totals, Reconstruction/solveXT.m:184,263).  Enabling resets the
+ * counters.  bytes_total is the algorithmic HBM traffic of the launches
+ * (each pass reads and writes every state volume once). */
+int dis_profile_enable(dis_ctx* ctx, int32_t on);
+int dis_profile_read(dis_ctx* ctx, int32_t kernel_class, int64_t* launches, double* ms_total, double* bytes_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DISORDER_H */

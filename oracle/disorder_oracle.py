@@ -1,0 +1,791 @@
+"""FP64 CPU oracle for DISORDER's aligned-reconstruction hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker (or, in ``bench.py``, as the timed CPU baseline).  The product path in
+``disorder_amd`` never imports it.
+
+This is a NumPy restatement, written from the text of the reference MATLAB
+sources, of the functions on the hot path.  Every function cites the reference
+file:line it follows.  All arithmetic is float64 / complex128.
+
+Array conventions (MATLAB column-major semantics, 0-based here):
+  * a volume is an ndarray of shape (N1, N2, N3); dims 1-2 are phase encode,
+    dim 3 is the readout (``Reconstruction/solveXT.m:51-56``);
+  * a stack of per-state volumes has shape (N1, N2, N3, S): the state axis is
+    the LAST axis here (MATLAB keeps it on dim 5);
+  * coil maps have shape (N1, N2, N3, C);
+  * compact k-space data has shape (Ns, N3, C): samples of all states are
+    concatenated in state order (``Reconstruction/encode.m:146``,
+    ``Methods/buildHarmonicSampling.m:36-49``);
+  * motion parameters T have shape (S, 6): T[:, 0:3] translations in
+    full-resolution voxels, T[:, 3:6] rotations in radians
+    (``Methods/sincRigidTransform.m:33-55``).
+
+Pinning (see tests/test_oracle_*.py): closed forms (numpy.fft against the
+literal dftmtx, integer shifts against numpy.roll, pi/2 rotations against
+index permutations), invariants (unitarity, inverse(forward) = identity,
+adjoint dot tests, row-DFT form == FFT+mask form), central finite differences
+of the forward transform for the gradient, a dense-matrix least-squares
+solution for CG, and recovery of a known motion for LM.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+
+import numpy as np
+
+_FFT_WORKERS = None  # None: numpy.fft; an int: scipy.fft with that many workers
+
+
+def set_fft_workers(n):
+    """Use scipy.fft with ``n`` threads (None restores numpy.fft).  Only the
+    CPU-baseline timing uses this; results are identical to rounding."""
+    global _FFT_WORKERS
+    _FFT_WORKERS = n
+
+
+def _fft(x, axis):
+    if _FFT_WORKERS:
+        import scipy.fft as sf
+        return sf.fft(x, axis=axis, workers=_FFT_WORKERS)
+    return np.fft.fft(x, axis=axis)
+
+
+def _ifft(x, axis):
+    if _FFT_WORKERS:
+        import scipy.fft as sf
+        return sf.ifft(x, axis=axis, workers=_FFT_WORKERS)
+    return np.fft.ifft(x, axis=axis)
+
+
+def _fft2o(x):
+    if _FFT_WORKERS:
+        import scipy.fft as sf
+        return sf.fft2(x, axes=(0, 1), norm="ortho", workers=_FFT_WORKERS)
+    return np.fft.fft2(x, axes=(0, 1), norm="ortho")
+
+
+def _ifft2o(x):
+    if _FFT_WORKERS:
+        import scipy.fft as sf
+        return sf.ifft2(x, axes=(0, 1), norm="ortho", workers=_FFT_WORKERS)
+    return np.fft.ifft2(x, axes=(0, 1), norm="ortho")
+
+# Rotation m (0-based) acts in the plane of axes (PER[0][m], PER[1][m]):
+# per=[1 3 2;2 1 3] in Methods/generateTransformGrids.m:55-56 and
+# Methods/sincRigidTransform.m:33-34.
+PER = ((0, 2, 1), (1, 0, 2))
+# Hessian pairs ha=[diag(1..6), nchoosek(1:6,2)] (Reconstruction/LMsolver.m:26)
+HA = [(g, g) for g in range(6)] + list(itertools.combinations(range(6), 2))
+
+
+# ----------------------------------------------------------------------------
+# Angle helpers
+# ----------------------------------------------------------------------------
+def wrap_to_pi(a):
+    """MATLAB Mapping Toolbox ``wrapToPi`` (call site
+    Methods/precomputeFactorsSincRigidTransform.m:46): angles already in
+    [-pi, pi] are unchanged; others are wrapped, positive multiples of pi map
+    to +pi."""
+    a = np.array(a, dtype=np.float64, copy=True)
+    q = (a < -np.pi) | (a > np.pi)
+    if np.any(q):
+        b = a[q] + np.pi
+        pos = b > 0
+        b = np.mod(b, 2 * np.pi)
+        b[(b == 0) & pos] = 2 * np.pi
+        a[q] = b - np.pi
+    return a
+
+
+def wrap_to_pi_half(a):
+    """Methods/wrapToPiHalf.m:9-12: mod(x,pi), then subtract pi above pi/2."""
+    a = np.mod(np.asarray(a, dtype=np.float64), np.pi)
+    return np.where(a > np.pi / 2, a - np.pi, a)
+
+
+def restrict_transform(T):
+    """Methods/restrictTransform.m:28-29 (no limits): rotations into [-pi, pi]
+    by repeated +-2pi."""
+    T = np.array(T, dtype=np.float64, copy=True)
+    ang = T[:, 3:6]
+    while np.any(ang > np.pi):
+        ang[ang > np.pi] -= 2 * np.pi
+    while np.any(ang < -np.pi):
+        ang[ang < -np.pi] += 2 * np.pi
+    T[:, 3:6] = ang
+    return T
+
+
+# ----------------------------------------------------------------------------
+# Grids (Methods/generateGrid.m:34, Methods/generateTransformGrids.m:48-61)
+# ----------------------------------------------------------------------------
+def generate_grid(N, nor, cent):
+    """Methods/generateGrid.m:33-39: p_m[j] = (nor_m/N_m) * (j - cent_m),
+    j = 1..N_m."""
+    return [(float(nor[m]) / N[m]) * (np.arange(1, N[m] + 1, dtype=np.float64) - cent[m])
+            for m in range(len(N))]
+
+
+def transform_grids(Nfull, n):
+    """Spatial and (ifftshifted) spectral grids of a pyramid level.
+
+    Methods/generateTransformGrids.m:48-52 as called from
+    Methods/downsampleOperators.m:62-63 (cent=ceil((n+1)/2), sh=1):
+      r_d[j] = (N_d/n_d)(j - ceil((n_d+1)/2)),
+      k_d    = ifftshift((2 pi/N_d)(j - ceil((n_d+1)/2))).
+    """
+    n = [int(v) for v in n]
+    Nfull = [float(v) for v in Nfull]
+    cent = [math.ceil((v + 1) / 2) for v in n]
+    r = generate_grid(n, Nfull, cent)
+    k = generate_grid(n, [2 * np.pi * n[d] / Nfull[d] for d in range(3)], cent)
+    k = [np.fft.ifftshift(km) for km in k]
+    return r, k
+
+
+def _along(v, axis, ndim=4):
+    shp = [1] * ndim
+    shp[axis] = v.shape[0]
+    return v.reshape(shp)
+
+
+def _plane(coef, ra, axis_r, kb, axis_k):
+    """exp-argument plane coef[s] * r(axis_r) * k(axis_k), shape (.., S)."""
+    return (_along(ra, axis_r) * _along(kb, axis_k)) * coef.reshape(1, 1, 1, -1)
+
+
+# ----------------------------------------------------------------------------
+# Factors (Methods/precomputeFactorsSincRigidTransform.m:45-131, gr=1, sh=1)
+# ----------------------------------------------------------------------------
+class Factors:
+    """Per-state transform factors ``et``/``etg`` for the grouped (gr=1),
+    pre-shifted (sh=1) case used by the reconstruction path.
+
+    e1[d]  : (.., S) arrays along axis d        exp(t_d k_d)          (:115)
+    e2[m]  : plane over (a=PER0[m], b=PER1[m])  exp(tan2j_m rk1_m)    (:79)
+    e3[m]  : same plane                          exp(sinj_m  rk2_m)    (:80)
+    flip   : (S,3) bool, |wrapToPi(theta)| > pi/2                      (:48-50)
+    cgrid  : flip centres n/2+1                                        (:31,52)
+    etg1[d]: -i k_d                                                    (:123)
+    etg2[m]: i (1+tan^2(th/2))/2 rk1_m e2_m                            (:89,99)
+    etg3[m]: -i cos(th) rk2_m e3_m                                     (:90,99)
+    """
+
+    def __init__(self, T, n, Nfull, di, cg=False):
+        T = np.atleast_2d(np.asarray(T, dtype=np.float64))
+        self.S = T.shape[0]
+        self.n = [int(v) for v in n]
+        self.di = di
+        r, k = transform_grids(Nfull, n)
+        self.r, self.k = r, k
+        sgn = (-1.0) ** di
+        t = sgn * 1j * T[:, 0:3]
+        th = wrap_to_pi(T[:, 3:6])
+        self.flip = np.abs(th) > np.pi / 2
+        self.cgrid = [nv / 2 + 1 for nv in self.n]
+        th = wrap_to_pi_half(th)
+        tan2 = np.tan(th / 2)
+        tan2j = ((-1.0) ** (di - 1)) * 1j * tan2
+        sinj = sgn * 1j * np.sin(th)
+        self.e2, self.e3 = [], []
+        for m in range(3):
+            a, b = PER[0][m], PER[1][m]
+            # rk{1}{m} = r_{per(2,m)} .* k_{per(1,m)};  rk{2}{m} = r_{per(1,m)} .* k_{per(2,m)}
+            self.e2.append(np.exp(_plane(tan2j[:, m], r[b], b, k[a], a)))
+            self.e3.append(np.exp(_plane(sinj[:, m], r[a], a, k[b], b)))
+        self.e1 = [np.exp(_along(k[d], d) * t[:, d].reshape(1, 1, 1, -1)) for d in range(3)]
+        if cg:
+            tc = (1 + tan2 * tan2) / 2
+            co = np.cos(th)
+            self.etg1 = [-1j * _along(k[d], d) for d in range(3)]
+            self.etg2, self.etg3 = [], []
+            for m in range(3):
+                a, b = PER[0][m], PER[1][m]
+                self.etg2.append(1j * _plane(tc[:, m], r[b], b, k[a], a) * self.e2[m])
+                self.etg3.append(-1j * _plane(co[:, m], r[a], a, k[b], b) * self.e3[m])
+
+
+# ----------------------------------------------------------------------------
+# DFT helpers (Methods/fftGPU.m:37-42, ifftGPU.m:31-39, build1DFTM.m:21-52)
+# ----------------------------------------------------------------------------
+def dftmtx(n):
+    """Methods/build1DFTM.m:21 -- dftmtx(n)(j,k) = exp(-2 pi i (j-1)(k-1)/n)."""
+    j = np.arange(n)
+    return np.exp(-2j * np.pi * np.outer(j, j) / n)
+
+
+def fft_dim(x, m):
+    """fftGPU(x,m): non-unitary forward DFT along axis m."""
+    return _fft(x, m)
+
+
+def ifft_dim(x, m):
+    """ifftGPU(x,m): conj(F)/n along axis m."""
+    return _ifft(x, m)
+
+
+# ----------------------------------------------------------------------------
+# Shifting / flipping (Methods/shifting.m:25-48, Methods/flipping.m:11-22)
+# ----------------------------------------------------------------------------
+def shifting(x, H):
+    """Methods/shifting.m:25-48 with f=0, zp=0.  H maps axis -> scalar shift.
+    Integer shifts are circular index shifts (:29-33); others are Fourier
+    phase ramps exp(-i h k) with k = ifftshift((2pi/n)(j-ceil((n+1)/2)))
+    (:39-45)."""
+    for m, h in H.items():
+        n = x.shape[m]
+        if float(h) == math.floor(h):
+            if n != 1:
+                x = np.roll(x, int(h), axis=m)
+        else:
+            kg = generate_grid([n], [2 * np.pi], [math.ceil((n + 1) / 2)])[0]
+            kg = -1j * np.fft.ifftshift(kg)
+            ph = np.exp(h * kg)
+            x = ifft_dim(fft_dim(x, m) * _along(ph, m, x.ndim), m)
+    return x
+
+
+def flipping(x, centers):
+    """Methods/flipping.m:11-22: shift by -c+1/2, flip, shift by c-1/2 on the
+    axes listed in ``centers`` (axis -> centre)."""
+    H = {a: -c + 0.5 for a, c in centers.items()}
+    x = shifting(x, H)
+    for a in centers:
+        x = np.flip(x, axis=a)
+    H = {a: -h for a, h in H.items()}
+    return shifting(x, H)
+
+
+def _flip_states(x, f, m):
+    """x.*(1-et5)+flipping(x,et4).*et5 (Methods/sincRigidTransform.m:43,109)."""
+    fl = f.flip[:, m]
+    if not np.any(fl):
+        return x
+    a, b = PER[0][m], PER[1][m]
+    if x.shape[3] != f.S:
+        x = np.broadcast_to(x, x.shape[:3] + (f.S,)).copy()
+    xf = flipping(x, {a: f.cgrid[a], b: f.cgrid[b]})
+    w = fl.astype(np.float64).reshape(1, 1, 1, -1)
+    return x * (1 - w) + xf * w
+
+
+# ----------------------------------------------------------------------------
+# Sinc rigid transform (Methods/sincRigidTransform.m:38-114)
+# ----------------------------------------------------------------------------
+def sinc_rigid_transform(x, f: Factors, di, sumR=True, keep=False):
+    """Methods/sincRigidTransform.m:38-114 (useFull=1, gr=1 cell factors).
+
+    di=1: x is (N1,N2,N3) or (N1,N2,N3,1|S); returns (N1,N2,N3,S) and, with
+    keep, the intermediates xB[0..4] (``xB{1..5}``).
+    di=0: x is (N1,N2,N3,S); returns the state sum (N1,N2,N3) when sumR.
+    """
+    x = np.asarray(x, dtype=np.complex128)
+    if x.ndim == 3:
+        x = x[..., None]
+    xB = [None] * 5
+    if di == 1:
+        xB[4] = x
+        for m in range(3):
+            a, b = PER[0][m], PER[1][m]
+            x = _flip_states(x, f, m)
+            x = fft_dim(x, a)
+            xB[m] = x
+            x = ifft_dim(x * f.e2[m], a)
+            x = ifft_dim(fft_dim(x, b) * f.e3[m], b)
+            x = fft_dim(x, a) * f.e2[m]
+            if m != 2:
+                x = ifft_dim(x, a)
+        for d in (0, 2):  # dim 2 (axis 1) is already in k-space (:66-68)
+            x = fft_dim(x, d)
+        xB[3] = x
+        for d in range(3):
+            x = x * f.e1[d]
+        for d in (2, 1, 0):
+            x = ifft_dim(x, d)
+        if x.shape[3] != f.S:
+            x = np.broadcast_to(x, x.shape[:3] + (f.S,)).copy()
+    else:
+        for d in range(3):  # (:86-90)
+            x = fft_dim(x, d) * f.e1[d]
+            if d != 1:
+                x = ifft_dim(x, d)
+        for m in (2, 1, 0):  # (:99-110)
+            a, b = PER[0][m], PER[1][m]
+            if m != 2:
+                x = fft_dim(x, a)
+            x = ifft_dim(x * f.e2[m], a)
+            x = ifft_dim(fft_dim(x, b) * f.e3[m], b)
+            x = ifft_dim(fft_dim(x, a) * f.e2[m], a)
+            x = _flip_states(x, f, m)
+        if sumR:
+            x = x.sum(axis=3)
+    return (x, xB) if keep else x
+
+
+def rigid_forward(x, T, n, Nfull):
+    return sinc_rigid_transform(x, Factors(T, n, Nfull, 1), 1)
+
+
+def rigid_inverse(x, T, n, Nfull, sumR=True):
+    return sinc_rigid_transform(x, Factors(T, n, Nfull, 0), 0, sumR=sumR)
+
+
+# ----------------------------------------------------------------------------
+# Gradient (Methods/sincRigidTransformGradient.m:28-147)
+# ----------------------------------------------------------------------------
+def sinc_rigid_transform_gradient(xB, f: Factors):
+    """d(T x)/dT_g, g = 0..5 (Methods/sincRigidTransformGradient.m:28-147).
+    ``f`` must be built with cg=True, di=1; ``xB`` from sinc_rigid_transform
+    (keep=True).  Returns a list of 6 arrays (N1,N2,N3,S).  The resampling
+    quirk at :149 is a no-op because Npad == N here."""
+    G = [None] * 6
+    S = f.S
+
+    def bc(v):
+        return v if v.shape[3] == S else np.broadcast_to(v, v.shape[:3] + (S,)).copy()
+
+    for m in range(3):  # translations (:29-36)
+        y = xB[3] * f.etg1[m]
+        for d in range(3):
+            y = y * f.e1[d]
+        for d in range(3):
+            y = ifft_dim(y, d)
+        G[m] = bc(y)
+
+    def rot_chain(m):
+        """Product rule through the three shears of rotation m (:39-52)."""
+        a, b = PER[0][m], PER[1][m]
+        x1 = xB[m] * f.e2[m]
+        x2 = xB[m] * f.etg2[m]
+        x1 = fft_dim(ifft_dim(x1, a), b)
+        x2 = fft_dim(ifft_dim(x2, a), b)
+        x2 = x1 * f.etg3[m] + x2 * f.e3[m]
+        x1 = x1 * f.e3[m]
+        x1 = fft_dim(ifft_dim(x1, b), a)
+        x2 = fft_dim(ifft_dim(x2, b), a)
+        return x1 * f.etg2[m] + x2 * f.e2[m]
+
+    def later_rotation(x, m):
+        """Rotation m applied to an image-space volume (:54-64, :66-75)."""
+        a, b = PER[0][m], PER[1][m]
+        x = _flip_states(x, f, m)
+        x = fft_dim(x, a) * f.e2[m]
+        x = ifft_dim(x, a)
+        x = ifft_dim(fft_dim(x, b) * f.e3[m], b)
+        x = fft_dim(x, a) * f.e2[m]
+        if m != 2:
+            x = ifft_dim(x, a)
+        return x
+
+    def finish(x):
+        """Translation stage (:77-85)."""
+        for d in (0, 2):
+            x = fft_dim(x, d)
+        for d in range(3):
+            x = x * f.e1[d]
+        for d in (2, 1, 0):
+            x = ifft_dim(x, d)
+        return x
+
+    for m in range(3):
+        x = rot_chain(m)
+        a = PER[0][m]
+        if m != 2:
+            x = ifft_dim(x, a)
+            for mm in range(m + 1, 3):
+                x = later_rotation(x, mm)
+        G[3 + m] = bc(finish(x))
+    return G
+
+
+# ----------------------------------------------------------------------------
+# Sampling (Methods/buildHarmonicSampling.m:36-49)
+# ----------------------------------------------------------------------------
+class Sampling:
+    """Per-state sample lists.  k1/k2 are 0-based FFT-order indices into the
+    (n1, n2) phase-encode grid; samples are concatenated in state order;
+    ``off`` holds the state offsets (``E.nSt``)."""
+
+    def __init__(self, k1, k2, off, n1, n2, We=None):
+        self.k1 = np.asarray(k1, dtype=np.int64)
+        self.k2 = np.asarray(k2, dtype=np.int64)
+        self.off = np.asarray(off, dtype=np.int64)
+        self.n1, self.n2 = int(n1), int(n2)
+        self.We = None if We is None else np.asarray(We, dtype=np.float64)
+
+    @property
+    def S(self):
+        return len(self.off) - 1
+
+    @property
+    def Ns(self):
+        return int(self.off[-1])
+
+    def state(self, s):
+        sl = slice(int(self.off[s]), int(self.off[s + 1]))
+        return sl, self.k1[sl], self.k2[sl]
+
+    def state_of_sample(self):
+        """``E.mSt`` (0-based)."""
+        return np.repeat(np.arange(self.S), np.diff(self.off))
+
+
+def build_harmonic_sampling(H, nstates):
+    """Methods/buildHarmonicSampling.m:36-49 (integer part).
+
+    H: (n1, n2) int array; H[k1,k2] = state (1..nstates) that sampled that
+    phase-encode position, 0 if unsampled.  For s = 1..nstates+1 the samples
+    of state mod(s, nstates+1) are taken in MATLAB linear (column-major)
+    order, so the last group is the unsampled positions (:37).
+    Returns (k1, k2, off) with off of length nstates+2 (``E.nSt`` with the
+    outer group appended, :49-50) and the column-major linear indices.
+    """
+    n1, n2 = H.shape
+    Hl = H.reshape(-1, order="F")
+    k1s, k2s, idx, off = [], [], [], [0]
+    for s in range(1, nstates + 2):
+        ind = np.nonzero(Hl == (s % (nstates + 1)))[0]
+        idx.append(ind)
+        k1s.append(ind % n1)
+        k2s.append(ind // n1)
+        off.append(off[-1] + len(ind))
+    return (np.concatenate(k1s), np.concatenate(k2s), np.asarray(off, np.int64),
+            np.concatenate(idx))
+
+
+# ----------------------------------------------------------------------------
+# Operators (Reconstruction/encode.m, decode.m, encodedecode.m)
+# ----------------------------------------------------------------------------
+def _transform_states(x, T, n, Nfull, nmotion):
+    S = T.shape[0]
+    if np.any(T[:nmotion] != 0):  # encode.m:61
+        xt = rigid_forward(x, T[:nmotion], n, Nfull)
+    else:
+        xt = np.broadcast_to(x[..., None], x.shape + (nmotion,))
+    if S > nmotion:  # outer segments use the untransformed x (encode.m:76-79)
+        xt = np.concatenate([xt, np.broadcast_to(x[..., None], x.shape + (S - nmotion,))], axis=3)
+    return xt
+
+
+def _inverse_states(v, T, n, Nfull, nmotion):
+    if np.any(T[:nmotion] != 0):  # decode.m:97-111
+        out = rigid_inverse(v[..., :nmotion], T[:nmotion], n, Nfull, sumR=True)
+    else:
+        out = v[..., :nmotion].sum(axis=3)
+    return out + v[..., nmotion:].sum(axis=3)
+
+
+def encode(x, Smaps, samp: Sampling, T, Nfull, nmotion=None):
+    """Reconstruction/encode.m:55-146 (Tr, Sf, Fs fields): y = E x, shape
+    (Ns, N3, C).  Per state: T_s x (:59-80), coil multiply (:93-96), then the
+    unitary 2D DFT over dims 1-2 evaluated at the state's samples
+    (:120-136, Fs rows from buildStandardDFTM(NY,1))."""
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    n = x.shape
+    nmotion = T.shape[0] if nmotion is None else nmotion
+    xt = _transform_states(x, T, n, Nfull, nmotion)
+    C = Smaps.shape[3]
+    y = np.zeros((samp.Ns, n[2], C), np.complex128)
+    for s in range(samp.S):
+        sl, k1, k2 = samp.state(s)
+        if len(k1) == 0:
+            continue
+        for c in range(C):
+            X = _fft2o(Smaps[..., c] * xt[..., s])
+            y[sl, :, c] = X[k1, k2, :]
+    return y
+
+
+def encode_rowdft(x, Smaps, samp: Sampling, T, Nfull, nmotion=None):
+    """Literal row-DFT form of Reconstruction/encode.m:120-136: aplGPU with
+    the selected rows of F/sqrt(n1), then sum(xQ .* F2rows, 2)."""
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    n = x.shape
+    nmotion = T.shape[0] if nmotion is None else nmotion
+    xt = _transform_states(x, T, n, Nfull, nmotion)
+    C = Smaps.shape[3]
+    F1 = dftmtx(n[0]) / np.sqrt(n[0])
+    F2 = dftmtx(n[1]) / np.sqrt(n[1])
+    y = np.zeros((samp.Ns, n[2], C), np.complex128)
+    for s in range(samp.S):
+        sl, k1, k2 = samp.state(s)
+        for c in range(C):
+            xs = Smaps[..., c] * xt[..., s]
+            xq = np.tensordot(F1[k1, :], xs, axes=(1, 0))          # (Nec, N2, N3)
+            y[sl, :, c] = np.sum(xq * F2[k2, :][:, :, None], axis=1)
+    return y
+
+
+def decode(y, Smaps, samp: Sampling, T, Nfull, nmotion=None):
+    """Reconstruction/decode.m:25-120: x = E^H (We .* y).  Per state: weights
+    (:46), conj(F2 rows) and F1 rows' adjoint (:51-52), conj(S) and coil sum
+    (:82-84), inverse transform (di=0) and state sum (:96-112)."""
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    n1, n2 = samp.n1, samp.n2
+    n3 = y.shape[1]
+    C = Smaps.shape[3]
+    S = samp.S
+    nmotion = S if nmotion is None else nmotion
+    v = np.zeros((n1, n2, n3, S), np.complex128)
+    for s in range(S):
+        sl, k1, k2 = samp.state(s)
+        ys = y[sl]
+        if samp.We is not None:
+            ys = ys * samp.We[sl][:, None, None]
+        for c in range(C):
+            K = np.zeros((n1, n2, n3), np.complex128)
+            np.add.at(K, (k1, k2), ys[:, :, c])
+            v[..., s] += np.conj(Smaps[..., c]) * _ifft2o(K)
+    return _inverse_states(v, T, (n1, n2, n3), Nfull, nmotion)
+
+
+def encodedecode(x, Smaps, samp: Sampling, T, Nfull, nmotion=None):
+    """Reconstruction/encodedecode.m:56-211: E^H W E x (weights applied once,
+    :133)."""
+    return decode(encode(x, Smaps, samp, T, Nfull, nmotion), Smaps, samp, T, Nfull, nmotion)
+
+
+# ----------------------------------------------------------------------------
+# Energy, CG (Reconstruction/computeEnergy.m, CGsolver.m)
+# ----------------------------------------------------------------------------
+def compute_energy(y, x, Smaps, samp, T, Nfull, nmotion=None, per_sample=False, Mbe=None):
+    """Reconstruction/computeEnergy.m:34-57 (no=2): residual E x - y, optional
+    readout weights sqrt(Mbe) (:39); per-sample sums over readout and coils
+    with avdi (:42), else the total energy."""
+    res = encode(x, Smaps, samp, T, Nfull, nmotion) - y
+    a2 = np.abs(res) ** 2
+    if Mbe is not None:
+        a2 = a2 * np.asarray(Mbe, np.float64)[None, :, None]
+    if per_sample:
+        return a2.sum(axis=(1, 2))
+    return float(a2.sum())
+
+
+def cg_solver(y, Smaps, samp, T, Nfull, x0=None, nIt=300, tolType="Energy", tol=1e-5,
+              Se=None, Ma=None, lam=None, nmotion=None):
+    """Reconstruction/CGsolver.m:45-109 with A.Se (precondition.m:19-23),
+    C.Ma (constrain.m:12) and an optional Tikhonov term R.Ti.la (regularize.m:24).
+
+    Returns (x, En, nn) where En holds [fidelity; regulariser] energies of the
+    odd iterations for tolType 'Energy' (computeEnergy.m:72-80)."""
+    def C_(v):
+        return v if Ma is None else v * Ma
+
+    def P_(v):
+        return v if Se is None else v * Se
+
+    def R_(v):
+        return 0 if lam is None else lam * v
+
+    def EHE(v):
+        return C_(encodedecode(v, Smaps, samp, T, Nfull, nmotion) + R_(v))
+
+    def energy(v):
+        fid = compute_energy(y, v, Smaps, samp, T, Nfull, nmotion)
+        reg = 0.0 if lam is None else float(abs(np.sum(lam * v * np.conj(v))))
+        return np.array([fid, reg])
+
+    tolTyp = {"RelativeResidual2Error": 1, "Energy": 6, "None": 7}[tolType]
+    NitTest = 2
+    b = C_(decode(y, Smaps, samp, T, Nfull, nmotion))
+    if x0 is None:
+        x = np.zeros_like(b)
+        ehe = np.zeros_like(b)
+    else:
+        x = np.array(x0, np.complex128, copy=True)
+        ehe = EHE(x)
+    bnorm2 = None
+    if tolTyp not in (6, 7):
+        zb = np.conj(P_(b)) * b
+        bnorm2 = float(np.sum(np.abs(zb)))
+    r = b - ehe
+    z = P_(r)
+    p = z.copy()
+    rsold = np.sum(np.conj(z) * r)
+    if abs(rsold) < 1e-6:  # :70-75
+        return x, np.zeros((2, 0)), 0
+    err = np.inf
+    En = {}
+    n = 0
+    for n in range(1, nIt + 1):
+        ehe = EHE(p)
+        g = rsold / np.sum(np.conj(p) * ehe)
+        x = x + g * p
+        if n % NitTest == 1 and tolTyp == 6:
+            En[n] = energy(x)
+        r = r - g * ehe
+        z = P_(r)
+        zr = np.conj(z) * r
+        rs = np.sum(zr)
+        d = rs / (rsold + np.finfo(np.float64).eps)
+        p = z + d * p
+        rsold = rs
+        if tolTyp == 1:
+            err = math.sqrt(float(np.sum(np.abs(zr))) / bnorm2)
+        elif tolTyp == 6 and n % NitTest == 1 and n > 1:
+            e0, e1 = En[n - NitTest].sum(), En[n].sum()
+            err = (e0 - e1) / e0
+        if err < tol or abs(rs) < 1e-6:
+            break
+    Ek = sorted(En)
+    return x, (np.stack([En[k] for k in Ek], axis=1) if Ek else np.zeros((2, 0))), n
+
+
+# ----------------------------------------------------------------------------
+# LM (Reconstruction/LMsolver.m:43-225)
+# ----------------------------------------------------------------------------
+def lm_partials(y, x, Smaps, samp, T, Nfull, states=None):
+    """Per-state residual energy, gradient dG and Gauss-Newton Hessian dH
+    (Reconstruction/LMsolver.m:52-122), all states transformed (NMs = S).
+    Returns EnPrev (S,), dG (S,6), dH (S,21); energies are divided by numel(y)."""
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    S = T.shape[0]
+    states = range(S) if states is None else states
+    NElY = y.size
+    n = x.shape
+    En = np.zeros(S)
+    dG = np.zeros((S, 6))
+    dH = np.zeros((S, 21))
+    for s in states:
+        f = Factors(T[s:s + 1], n, Nfull, 1, cg=True)
+        xt, xB = sinc_rigid_transform(x, f, 1, keep=True)
+        G = sinc_rigid_transform_gradient(xB, f)
+        sub = _one_state(samp, s)
+        Tz = np.zeros((1, 6))
+        r = encode(xt[..., 0], Smaps, sub, Tz, Nfull) - y[samp.state(s)[0]]
+        En[s] = float(np.sum(np.abs(r) ** 2)) / NElY
+        J = [encode(G[g][..., 0], Smaps, sub, Tz, Nfull) for g in range(6)]
+        for g in range(6):
+            dG[s, g] = float(np.sum(np.real(J[g] * np.conj(r))))
+        for h, (i, j) in enumerate(HA):
+            dH[s, h] = float(np.sum(np.real(J[i] * np.conj(J[j]))))
+    return En, dG, dH
+
+
+def _one_state(samp, s):
+    sl, k1, k2 = samp.state(s)
+    return Sampling(k1, k2, [0, len(k1)], samp.n1, samp.n2,
+                    None if samp.We is None else samp.We[sl])
+
+
+def lm_state_energy(y, x, Smaps, samp, T, Nfull, states):
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    out = {}
+    for s in states:
+        sub = _one_state(samp, s)
+        r = encode(x, Smaps, sub, T[s:s + 1], Nfull) - y[samp.state(s)[0]]
+        out[s] = float(np.sum(np.abs(r) ** 2)) / y.size
+    return out
+
+
+def lm_step(dH, dG, w, winit):
+    """Damped 6x6 FP64 solve (Reconstruction/LMsolver.m:132-141):
+    MH_ii=(1+w)H_ii, MH_ij=H_ij, step = -(winit/w) MH\\g."""
+    MH = np.zeros((6, 6))
+    for h, (i, j) in enumerate(HA):
+        if i == j:
+            MH[i, i] = (1 + w) * dH[h]
+        else:
+            MH[i, j] = MH[j, i] = dH[h]
+    return -(winit / w) * np.linalg.solve(MH, dG)
+
+
+def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None):
+    """Reconstruction/LMsolver.m:43-225 for one outer call.  Returns the new
+    (T, w, cT, Enmin)."""
+    T = np.array(np.atleast_2d(T), np.float64, copy=True)
+    w = np.array(w, np.float64, copy=True)
+    cT = np.array(cT, bool, copy=True)
+    S = T.shape[0]
+    est = [s for s in range(S) if not cT[s]]
+    EnPrev, dG, dH = lm_partials(y, x, Smaps, samp, T, Nfull, est)
+    En = EnPrev.copy()
+    dGEff = np.zeros((S, 6))
+    flagw = np.zeros(S, bool)
+    while True:
+        for s in range(S):
+            if not cT[s] and w[s] < 1e10:
+                dGEff[s] = lm_step(dH[s], dG[s], w[s], winit)
+        Tup = restrict_transform(T + dGEff)
+        cfull = cT | flagw
+        todo = [s for s in range(S) if not cfull[s]]
+        for s, e in lm_state_energy(y, x, Smaps, samp, Tup, Nfull, todo).items():
+            En[s] = e
+        big = w > 1e10
+        En[big] = EnPrev[big]
+        flagw[En <= EnPrev] = True
+        if log is not None:
+            log.append((EnPrev.sum(), En.sum()))
+        if not np.all(flagw):
+            rej = (En > EnPrev) & ~cT
+            w[rej] *= 2.0
+        else:
+            w[~cT] /= 1.2
+            w[w < 1e-8] *= 1.2
+            T = Tup
+            tra = np.max(np.abs(dGEff[:, 0:3]), axis=1)
+            rot = np.max(np.abs(dGEff[:, 3:6] * 180 / np.pi), axis=1)
+            cT[(tra < tL[0]) & (rot < tL[1])] = True
+            break
+    return T, w, cT, np.minimum(EnPrev, En)
+
+
+# ----------------------------------------------------------------------------
+# Coil compression (Methods/compressCoils.m:75-118)
+# ----------------------------------------------------------------------------
+def compress_coils(Smaps, perc, y=None, ncomp=None):
+    """Methods/compressCoils.m:75-118: Shat = S/(sum|S|^2+1e-9); P = sum_vox
+    Shat S^H (C x C); U from svd(P); keep Cc columns by energy fraction perc
+    (or ncomp); S <- U(:,1:Cc)^H S, y <- U(:,1:Cc)^H y along coils."""
+    C = Smaps.shape[-1]
+    Sv = Smaps.reshape(-1, C)
+    nrm = np.sum(np.abs(Sv) ** 2, axis=1, keepdims=True) + 1e-9
+    Sh = Sv / nrm
+    P = Sh.T @ np.conj(Sv)          # P(i,j) = sum_v Shat_i conj(S_j)
+    U, D, _ = np.linalg.svd(P)
+    if ncomp is None:
+        cs = np.cumsum(D) / np.sum(D)
+        ncomp = int(np.argmax(cs >= perc) + 1)
+    A = np.conj(U[:, :ncomp]).T     # U(:,1:Cc)'
+    Sc = (Sv @ A.T).reshape(Smaps.shape[:-1] + (ncomp,))
+    yc = None if y is None else (y.reshape(-1, C) @ A.T).reshape(y.shape[:-1] + (ncomp,))
+    return Sc, yc, P, A
+
+
+# ----------------------------------------------------------------------------
+# Pyramid helpers (Methods/resampling.m:52-89, Methods/downsampleOperators.m:34)
+# ----------------------------------------------------------------------------
+def resample_fourier(x, nres):
+    """Methods/resampling.m:45-80 (fo=0, mirror=0): per axis, fft/N, fftshift,
+    crop/zero-pad centred at ceil((Nmax+1)/2), ifftshift, ifft*Nres."""
+    x = np.asarray(x, np.complex128)
+    for m, nr in enumerate(nres):
+        no = x.shape[m]
+        if no == nr:
+            continue
+        nmin, nmax = min(no, nr), max(no, nr)
+        zero = math.ceil((nmax + 1) / 2)
+        orig = zero - math.ceil((nmin - 1) / 2)
+        fina = zero + math.floor((nmin - 1) / 2)
+        X = np.fft.fftshift(np.fft.fft(x, axis=m) / no, axes=m)
+        shp = list(x.shape)
+        shp[m] = nr
+        if no < nr:
+            Y = np.zeros(shp, np.complex128)
+            idx = [slice(None)] * x.ndim
+            idx[m] = slice(orig - 1, fina)
+            Y[tuple(idx)] = X
+        else:
+            idx = [slice(None)] * x.ndim
+            idx[m] = slice(orig - 1, fina)
+            Y = X[tuple(idx)]
+        x = np.fft.ifft(np.fft.ifftshift(Y, axes=m), axis=m) * nr
+    return x
+
+
+def level_size(N, l):
+    """Methods/downsampleOperators.m:34: ceil(N/2^l)."""
+    return [int(math.ceil(v / 2 ** l)) for v in N]

@@ -1,0 +1,168 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 oracle on the
+same seeded inputs.  Tolerance: relative L2 <= 1e-5 for the transform and the
+encode/decode operators (FP32 complex arithmetic vs FP64)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import disorder_oracle as O
+from tests.helpers import checkered_sampling, crandn, random_motion, rel_l2, smooth_volume
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def to_dev(a, dev):
+    import disorder_amd as D
+    t = torch.from_numpy(np.asfortranarray(np.asarray(a, np.complex64)))
+    return D.colmajor(t.to(dev))
+
+
+def to_np(t):
+    return t.cpu().numpy().astype(np.complex128)
+
+
+@pytest.mark.parametrize("n,Nfull", [((16, 12, 10), None), ((16, 12, 10), (32, 24, 20)), ((15, 14, 9), None),
+                                     ((28, 20, 21), (56, 40, 42)), ((64, 48, 64), None)])
+def test_rigid_forward_inverse(dev, n, Nfull):
+    import disorder_amd as D
+    rng = np.random.default_rng(sum(n))
+    Nf = Nfull or n
+    x = crandn(rng, n)
+    S = 4
+    T = random_motion(rng, S)
+    T[1] = 0.0
+    xt = D.sincRigidTransform(to_dev(x, dev), T, di=1, Nfull=Nf)
+    ref = O.rigid_forward(x, T, n, Nf)
+    for s in range(S):
+        assert rel_l2(to_np(xt)[..., s], ref[..., s]) < TOL, (n, s)
+    # inverse with the state sum and without
+    v = crandn(rng, n + (S,))
+    out = D.sincRigidTransform(to_dev(v, dev), T, di=0, sumR=True, Nfull=Nf)
+    assert rel_l2(to_np(out), O.rigid_inverse(v, T, n, Nf, sumR=True)) < TOL
+    outs = D.sincRigidTransform(to_dev(v, dev), T, di=0, sumR=False, Nfull=Nf)
+    refs = O.rigid_inverse(v, T, n, Nf, sumR=False)
+    for s in range(S):
+        assert rel_l2(to_np(outs)[..., s], refs[..., s]) < TOL
+
+
+@pytest.mark.parametrize("n", [(16, 12, 10), (15, 9, 14)])
+def test_rigid_flip_path(dev, n):
+    """Rotations beyond 90 degrees (Methods/flipping.m via sincRigidTransform.m:43,109)."""
+    import disorder_amd as D
+    rng = np.random.default_rng(7)
+    x = crandn(rng, n)
+    T = np.array([[0.3, -0.2, 0.5, 2.2, 0.1, -0.05],
+                  [0.0, 0.4, 0.0, 0.1, -2.5, 0.2],
+                  [0.2, 0.0, -0.3, -0.1, 0.05, 1.9],
+                  [0.1, 0.1, 0.1, 0.05, 0.02, 0.03]])
+    xt = to_np(D.sincRigidTransform(to_dev(x, dev), T, di=1))
+    ref = O.rigid_forward(x, T, n, n)
+    for s in range(T.shape[0]):
+        assert rel_l2(xt[..., s], ref[..., s]) < TOL, s
+    v = crandn(rng, n + (T.shape[0],))
+    assert rel_l2(to_np(D.sincRigidTransform(to_dev(v, dev), T, di=0)), O.rigid_inverse(v, T, n, n)) < TOL
+
+
+def test_rigid_properties_full_size(dev):
+    """Size-independent properties at the config grid 256x192x256: T=0 is the
+    identity, an integer translation is a circular shift, inverse(forward) = I,
+    unitarity."""
+    import disorder_amd as D
+    n = (256, 192, 256)
+    g = torch.Generator(device=dev).manual_seed(3)
+    x = D.colmajor(torch.randn(n, dtype=torch.complex64, device=dev, generator=g))
+    T = np.array([[0, 0, 0, 0, 0, 0], [3, -2, 5, 0, 0, 0], [1.3, -0.7, 2.1, 0.05, -0.08, 0.06]], np.float64)
+    xt = D.sincRigidTransform(x, T, di=1)
+    nx = torch.linalg.vector_norm(x).item()
+    e0 = (torch.linalg.vector_norm(xt[..., 0] - x) / nx).item()
+    assert e0 < 1e-6
+    rolled = torch.roll(x, shifts=(3, -2, 5), dims=(0, 1, 2))
+    assert (torch.linalg.vector_norm(xt[..., 1] - rolled) / nx).item() < 1e-5
+    for s in range(3):
+        assert abs(torch.linalg.vector_norm(xt[..., s]).item() / nx - 1) < 1e-5
+    back = D.sincRigidTransform(D.colmajor(xt[..., 2:3].contiguous()), T[2:3], di=0)
+    assert (torch.linalg.vector_norm(back - x) / nx).item() < 1e-5
+
+
+def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
+    x = smooth_volume(rng, n) + 0.1 * crandn(rng, n)
+    Sm = crandn(rng, n + (C,)) * 0.5
+    if S == 4:
+        H = checkered_sampling(rng, n[0], n[1], 4, 2, 2)
+    else:
+        H = rng.integers(1, S + 1, size=(n[0], n[1]))
+    k1, k2, off, _ = O.build_harmonic_sampling(H, S)
+    We = rng.uniform(0.5, 1.5, off[S]) if weights else None
+    samp = O.Sampling(k1, k2, off[:S + 1], n[0], n[1], We)
+    nm = S if nmotion is None else nmotion
+    T = random_motion(rng, nm)
+    return x, Sm, samp, T, nm, (Nfull or n)
+
+
+@pytest.mark.parametrize("n,C,S,nmotion,weights", [((16, 12, 8), 3, 4, None, False),
+                                                    ((16, 12, 8), 3, 5, 4, True),
+                                                    ((20, 15, 12), 9, 6, None, True)])
+def test_encode_decode_ehe(dev, n, C, S, nmotion, weights):
+    import disorder_amd as D
+    rng = np.random.default_rng(11)
+    x, Sm, samp, T, nm, Nf = _problem(rng, n, C, S, nmotion, weights)
+    E = D.Encoding(to_dev(Sm, dev), samp.k1, samp.k2, samp.off, Nfull=Nf, We=samp.We, nmotion=nm)
+    Tall = np.vstack([T, np.zeros((S - nm, 6))])
+    yr = O.encode(x, Sm, samp, T, Nf, nmotion=nm)
+    xd = to_dev(x, dev)
+    y = to_np(E.encode(xd, T))
+    assert rel_l2(y, yr) < TOL
+    yin = crandn(rng, yr.shape)
+    xr = O.decode(yin, Sm, samp, T, Nf, nmotion=nm)
+    assert rel_l2(to_np(E.decode(to_dev(yin, dev), T)), xr) < TOL
+    er = O.encodedecode(x, Sm, samp, T, Nf, nmotion=nm)
+    assert rel_l2(to_np(E.encodedecode(xd, T)), er) < TOL
+    # residual energy per sample and total (FP64 accumulation)
+    ps = E.energy(xd, to_dev(yin, dev), T, per_sample=True)
+    pr = O.compute_energy(yin, x, Sm, samp, T, Nf, nmotion=nm, per_sample=True)
+    assert rel_l2(ps, pr) < TOL
+    del Tall
+
+
+def test_encode_zero_motion_and_level_grid(dev):
+    """T == 0 skips the transform (encode.m:61); level grids (Nfull = 2n)."""
+    import disorder_amd as D
+    rng = np.random.default_rng(5)
+    n = (16, 12, 10)
+    x, Sm, samp, T, nm, _ = _problem(rng, n, 2, 4)
+    Nf = (32, 24, 20)
+    E = D.Encoding(to_dev(Sm, dev), samp.k1, samp.k2, samp.off, Nfull=Nf)
+    xd = to_dev(x, dev)
+    for TT in (np.zeros((4, 6)), T):
+        assert rel_l2(to_np(E.encode(xd, TT)), O.encode(x, Sm, samp, TT, Nf)) < TOL
+        assert rel_l2(to_np(E.encodedecode(xd, TT)), O.encodedecode(x, Sm, samp, TT, Nf)) < TOL
+
+
+def test_ehe_properties_config_grid(dev):
+    """At 256x192x256, 8 coils, 64 DISORDER shots: the adjoint identity
+    <E x, y> = <x, E^H y>, E^H E = decode(encode(x)), Hermitian E^H E."""
+    import disorder_amd as D
+    n = (256, 192, 256)
+    C, S = 8, 64
+    rng = np.random.default_rng(0)
+    H = checkered_sampling(rng, n[0], n[1], S, 8, 8)
+    k1, k2, off, _ = O.build_harmonic_sampling(H, S)
+    off = off[:S + 1]
+    g = torch.Generator(device=dev).manual_seed(1)
+    Sm = D.colmajor(torch.randn(n + (C,), dtype=torch.complex64, device=dev, generator=g) * 0.3)
+    E = D.Encoding(Sm, k1, k2, off)
+    T = random_motion(rng, S, rot_deg=3, tra=2)
+    x = D.colmajor(torch.randn(n, dtype=torch.complex64, device=dev, generator=g))
+    z = D.colmajor(torch.randn(n, dtype=torch.complex64, device=dev, generator=g))
+    y = E.encode(x, T)
+    yr = D.colmajor(torch.randn(tuple(y.shape), dtype=torch.complex64, device=dev, generator=g))
+    a = torch.vdot(yr.flatten(), y.flatten()).item()
+    b = torch.vdot(E.decode(yr, T).flatten(), x.flatten()).item()
+    assert abs(a - b) / abs(a) < 1e-5
+    ehe = E.encodedecode(x, T)
+    dec = E.decode(y, T)
+    assert (torch.linalg.vector_norm(ehe - dec) / torch.linalg.vector_norm(dec)).item() < 1e-5
+    p = torch.vdot(z.flatten(), ehe.flatten()).item()
+    q = torch.vdot(E.encodedecode(z, T).flatten(), x.flatten()).item()
+    assert abs(p - q) / abs(p) < 1e-5
