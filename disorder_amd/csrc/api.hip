@@ -27,6 +27,7 @@ struct dis_ctx {
   size_t ws_limit = 0;
   void* dcoef = nullptr;
   size_t dcoef_cap = 0;
+  std::map<int, std::pair<void*, size_t>> bufs;
   // profiling
   bool prof = false;
   struct Rec {
@@ -106,6 +107,26 @@ int ctx_workspace(dis_ctx* c, size_t bytes, void** out) {
 }
 
 size_t ctx_workspace_limit(dis_ctx* c) { return c->ws_limit; }
+
+int ctx_buffer(dis_ctx* c, int slot, size_t bytes, void** out) {
+  auto& b = c->bufs[slot];
+  if (bytes > b.second) {
+    if (b.first) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      HIPCHK(c, hipFree(b.first));
+      b.first = nullptr;
+      b.second = 0;
+    }
+    if (hipMalloc(&b.first, bytes) != hipSuccess) {
+      b.first = nullptr;
+      (void)hipGetLastError();
+      return fail(c, DIS_E_NOMEM, "hipMalloc of a solver buffer failed");
+    }
+    b.second = bytes;
+  }
+  *out = b.first;
+  return DIS_OK;
+}
 
 // Upload the staged bytes into the device coefficient buffer (grown on
 // demand).  Stream-ordered after all earlier work; synchronous on return, so
@@ -208,6 +229,8 @@ int dis_ctx_destroy(dis_ctx* c) {
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
   if (c->ws) (void)hipFree(c->ws);
+  for (auto& kv : c->bufs)
+    if (kv.second.first) (void)hipFree(kv.second.first);
   if (c->dcoef) (void)hipFree(c->dcoef);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -429,3 +452,20 @@ int dis_energy(dis_plan* p, const double* T, const void* x, const void* y, const
 }
 
 }  // extern "C"
+
+extern "C" int dis_cg(dis_plan* p, const double* T, const void* y, const float* Se, const float* Ma, double lambda,
+                      void* x, int32_t x_init, int32_t nIt, int32_t tolType, double tol, double* En, int32_t* nEn,
+                      int32_t* nn) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!x || !y || nIt < 0 || (p->g.nmotion > 0 && !T)) return fail(c, DIS_E_INVALID, "dis_cg: invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  int ne = 0, it = 0;
+  int st = run_cg(c, plan_view(p), T, (const float2*)y, Se, Ma, lambda, (float2*)x, x_init != 0, nIt, tolType, tol,
+                  En, &ne, &it);
+  if (st) return st;
+  if (nEn) *nEn = ne;
+  if (nn) *nn = it;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}

@@ -54,6 +54,8 @@ hipStream_t ctx_stream(dis_ctx* c);
 int ctx_twiddles(dis_ctx* c, int n, const float2** out);
 int ctx_workspace(dis_ctx* c, size_t bytes, void** out);
 size_t ctx_workspace_limit(dis_ctx* c);
+// Named device buffers that outlive one call (slot 0 is the op workspace).
+int ctx_buffer(dis_ctx* c, int slot, size_t bytes, void** out);
 int ctx_upload_bytes(dis_ctx* c, const Staging& st, const unsigned char** dptr);
 
 // Event timing around a launch when profiling is enabled (no-ops otherwise).
@@ -78,5 +80,8 @@ int rigid_inverse(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, c
 // E, E^H, E^H E and the residual energy through one coil-stage mode.
 int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* yin, float2* yout,
              float2* xout, int mode, EnergyOut* eo);
+
+int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const float* Se, const float* Ma,
+           double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn);
 
 }  // namespace dis

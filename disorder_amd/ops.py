@@ -53,6 +53,15 @@ def _ptr(t: torch.Tensor):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _fptr(t):
+    """Real float32 column-major device tensor (or None) -> float*."""
+    if t is None:
+        return None
+    if t.dtype != torch.float32 or not t.is_cuda or not is_colmajor(t):
+        raise TypeError("float32 column-major device tensors are required")
+    return ctypes.cast(ctypes.c_void_p(t.data_ptr()), _lib.c_float_p)
+
+
 def _tbuf(T, rows=None):
     """Motion [S x 6] -> column-major float64 host buffer."""
     T = np.atleast_2d(np.asarray(T, dtype=np.float64))
@@ -242,6 +251,28 @@ class Encoding:
         check(self.ctx.lib.dis_energy(self.h, Tp, _ptr(x), _ptr(y), mb, ps.ctypes.data_as(_lib.c_double_p),
                                       ctypes.byref(tot)), self.ctx.h)
         return ps[:self.Ns] if per_sample else tot.value
+
+
+TOLTYPES = {"RelativeResidual2Error": 1, "Energy": 6, "None": 7}
+
+
+def CGsolver(y, E: "Encoding", T, x=None, nIt=300, tolType="Energy", tol=1e-5, Se=None, Ma=None, lam=0.0):
+    """Reconstruction/CGsolver.m:1 -- preconditioned CG on E^H We E x + lam x =
+    E^H We y with preconditioner Se (A.Se) and mask Ma (C.Ma); FP64 scalars.
+    Returns (x, En, nn): En is (2, k) [fidelity; regulariser] of the odd
+    iterations for tolType 'Energy'."""
+    tt = TOLTYPES[tolType]
+    init = x is not None
+    xo = colmajor(x.clone()) if init else empty_colmajor(E.n, y.device)
+    En = np.zeros(2 * (nIt // 2 + 2), np.float64)
+    nEn = ctypes.c_int32(0)
+    nn = ctypes.c_int32(0)
+    Tf, Tp = E._T(T)
+    E.ctx._prep()
+    check(E.ctx.lib.dis_cg(E.h, Tp, _ptr(y), _fptr(Se), _fptr(Ma), float(lam), _ptr(xo), 1 if init else 0, int(nIt),
+                           tt, float(tol), En.ctypes.data_as(_lib.c_double_p), ctypes.byref(nEn), ctypes.byref(nn)),
+          E.ctx.h)
+    return xo, En[:2 * nEn.value].reshape(-1, 2).T.copy(), nn.value
 
 
 def encode(x, E: Encoding, T):

@@ -109,6 +109,20 @@ int dis_encodedecode(dis_plan* plan, const double* T, const void* x, void* out);
 int dis_energy(dis_plan* plan, const double* T, const void* x, const void* y, const float* Mbe,
                double* per_sample, double* total);
 
+/* Preconditioned CG on the normal equations (replaces
+ * Reconstruction/CGsolver.m:1 with A.Se, C.Ma and a Tikhonov R.Ti.la):
+ *   b = Ma.*decode(y);  r = b - Ma.*(E^H E x + lambda x);  z = Se.*r; ...
+ * Se, Ma: DEVICE [n1,n2,n3] real preconditioner (A.Se) and mask (C.Ma), or
+ * NULL.  x: DEVICE [n1,n2,n3], initial guess when x_init (else zeroed), the
+ * solution on return.  tolType: 1 'RelativeResidual2Error', 6 'Energy',
+ * 7 'None' (Reconstruction/stopCondition.m).  Inner products, the step
+ * scalars and the residual norm are FP64.  En (host, 2 doubles per odd
+ * iteration: fidelity, regulariser) receives the 'Energy' history, nEn its
+ * column count, nn the iterations run (0 on the early exit of
+ * CGsolver.m:70-75). */
+int dis_cg(dis_plan* plan, const double* T, const void* y, const float* Se, const float* Ma, double lambda, void* x,
+           int32_t x_init, int32_t nIt, int32_t tolType, double tol, double* En, int32_t* nEn, int32_t* nn);
+
 /* Kernel classes for dis_profile_read. */
 #define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */
 #define DIS_K_BCAST 1  /* first forward pass, one input volume to all states */

@@ -166,3 +166,36 @@ def test_ehe_properties_config_grid(dev):
     p = torch.vdot(z.flatten(), ehe.flatten()).item()
     q = torch.vdot(E.encodedecode(z, T).flatten(), x.flatten()).item()
     assert abs(p - q) / abs(p) < 1e-5
+
+
+@pytest.mark.parametrize("tolType,nIt,lam,use_pre", [("RelativeResidual2Error", 6, 0.0, True),
+                                                     ("Energy", 7, 0.05, True), ("None", 4, 0.0, False)])
+def test_cg_matches_oracle(dev, tolType, nIt, lam, use_pre):
+    """CGsolver: iterates, energies and iteration count against the FP64 oracle
+    (FP32 vectors, FP64 scalars: tolerance 1e-4 after a few iterations)."""
+    import disorder_amd as D
+    rng = np.random.default_rng(21)
+    n, C, S = (16, 12, 8), 3, 4
+    x, Sm, samp, T, nm, Nf = _problem(rng, n, C, S, weights=True)
+    y = O.encode(x, Sm, samp, T, Nf) + 0.01 * crandn(rng, (samp.Ns, n[2], C))
+    Se = 1.0 / (np.sum(np.abs(Sm) ** 2, axis=3) + 1e-9) if use_pre else None
+    Ma = (rng.uniform(size=n) > 0.1).astype(np.float64)
+    xr, Er, nr = O.cg_solver(y, Sm, samp, T, Nf, nIt=nIt, tolType=tolType, tol=0.0, Se=Se, Ma=Ma,
+                             lam=(lam or None))
+    E = D.Encoding(to_dev(Sm, dev), samp.k1, samp.k2, samp.off, Nfull=Nf, We=samp.We)
+
+    def fdev(a):
+        return None if a is None else D.colmajor(torch.from_numpy(np.asfortranarray(a.astype(np.float32))).to(dev))
+
+    xg, Eg, ng = D.CGsolver(to_dev(y, dev), E, T, nIt=nIt, tolType=tolType, tol=0.0, Se=fdev(Se), Ma=fdev(Ma),
+                            lam=lam)
+    assert ng == nr
+    assert rel_l2(to_np(xg), xr) < 1e-4
+    if tolType == "Energy":
+        assert Eg.shape == Er.shape
+        assert np.allclose(Eg, Er, rtol=1e-4)
+    # warm start from the oracle's solution continues identically
+    xg2, _, _ = D.CGsolver(to_dev(y, dev), E, T, x=to_dev(xr, dev), nIt=2, tolType="None", Se=fdev(Se), Ma=fdev(Ma),
+                           lam=lam)
+    xr2, _, _ = O.cg_solver(y, Sm, samp, T, Nf, x0=xr, nIt=2, tolType="None", Se=Se, Ma=Ma, lam=(lam or None))
+    assert rel_l2(to_np(xg2), xr2) < 1e-4
