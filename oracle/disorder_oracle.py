@@ -459,8 +459,7 @@ def build_harmonic_sampling(H, nstates):
 # ----------------------------------------------------------------------------
 # Operators (Reconstruction/encode.m, decode.m, encodedecode.m)
 # ----------------------------------------------------------------------------
-def _transform_states(x, T, n, Nfull, nmotion):
-    S = T.shape[0]
+def _transform_states(x, T, n, Nfull, nmotion, S):
     if np.any(T[:nmotion] != 0):  # encode.m:61
         xt = rigid_forward(x, T[:nmotion], n, Nfull)
     else:
@@ -486,7 +485,7 @@ def encode(x, Smaps, samp: Sampling, T, Nfull, nmotion=None):
     T = np.atleast_2d(np.asarray(T, np.float64))
     n = x.shape
     nmotion = T.shape[0] if nmotion is None else nmotion
-    xt = _transform_states(x, T, n, Nfull, nmotion)
+    xt = _transform_states(x, T, n, Nfull, nmotion, samp.S)
     C = Smaps.shape[3]
     y = np.zeros((samp.Ns, n[2], C), np.complex128)
     for s in range(samp.S):
@@ -505,7 +504,7 @@ def encode_rowdft(x, Smaps, samp: Sampling, T, Nfull, nmotion=None):
     T = np.atleast_2d(np.asarray(T, np.float64))
     n = x.shape
     nmotion = T.shape[0] if nmotion is None else nmotion
-    xt = _transform_states(x, T, n, Nfull, nmotion)
+    xt = _transform_states(x, T, n, Nfull, nmotion, samp.S)
     C = Smaps.shape[3]
     F1 = dftmtx(n[0]) / np.sqrt(n[0])
     F2 = dftmtx(n[1]) / np.sqrt(n[1])

@@ -324,3 +324,20 @@ def test_resample_fourier_roundtrip():
     assert np.abs(down - x).max() < 1e-12
     assert O.level_size((256, 192, 256), 3) == [32, 24, 32]
     assert O.level_size((15, 9, 7), 1) == [8, 5, 4]
+
+
+def test_outer_segments_use_untransformed_x():
+    """Segments beyond NMs are encoded from the untransformed x (encode.m:76-79)."""
+    rng = np.random.default_rng(16)
+    n, C, S = (8, 6, 4), 2, 3
+    x = crandn(rng, n)
+    Sm = crandn(rng, n + (C,))
+    H = rng.integers(1, S + 1, size=(8, 6))
+    k1, k2, off, _ = O.build_harmonic_sampling(H, S)
+    samp = O.Sampling(k1, k2, off[:S + 1], 8, 6)
+    T = random_motion(rng, 2, 3, 1)
+    y = O.encode(x, Sm, samp, T, n, nmotion=2)
+    yfull = O.encode(x, Sm, samp, np.vstack([T, np.zeros((1, 6))]), n)
+    assert np.abs(y - yfull).max() < 1e-12
+    yr = crandn(rng, y.shape)
+    assert abs(np.vdot(yr, y) - np.vdot(O.decode(yr, Sm, samp, T, n, nmotion=2), x)) < 1e-10 * abs(np.vdot(yr, y))
