@@ -12,6 +12,7 @@
 // sums the coils in registers.  For E^H E both halves run in the same
 // workgroup, so the state's k-space never leaves the chip.
 #include "kernels.hpp"
+#include "regfft.hpp"
 
 namespace dis {
 
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
   }
 }
 
-hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
   dim3 grid(a.n3, nstates);
   switch (mode) {
     case COIL_ENCODE: hipLaunchKernelGGL(coil_kernel<COIL_ENCODE>, grid, dim3(kThreads), 0, st, a); break;
@@ -280,6 +281,264 @@ hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st)
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Register path for n1 = 16 * R1.  Sixteen dim-2 rows per tile, sixteen
+// lanes per row (regfft.hpp).  Workgroups are numbered so that the S states
+// of one readout plane run back to back on one XCD: they share that plane's
+// coil maps through the XCD's L2.
+constexpr int kRJ = 3;  // samples per thread per chunk on the register path
+
+template <int R1, int MODE>
+__global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nstates) {
+  constexpr bool FWD = MODE == COIL_ENCODE || MODE == COIL_EHE || MODE == COIL_ENERGY;
+  constexpr bool ADJ = MODE == COIL_DECODE || MODE == COIL_EHE;
+  constexpr int N = R1 * 16;
+  constexpr int XB = RegFftSize<R1>::XB;
+  static_assert(16 * XB >= 16 * N, "exchange region doubles as the row tile");
+  __shared__ fvec2 xraw[16 * XB];
+  __shared__ fvec2 t1raw[N];
+  __shared__ fvec2 t2raw[1024];
+  const int tid = threadIdx.x;
+  const int p = tid >> 4, t = tid & 15;
+  lds_float2* tw1 = (lds_float2*)t1raw;
+  lds_float2* tw2 = (lds_float2*)t2raw;
+  lds_float2* xb = (lds_float2*)xraw + p * XB;
+  lds_float2* Zt = (lds_float2*)xraw;  // [16 rows][N] natural order
+  const int n2 = a.n2;
+  for (int k = tid; k < N; k += kThreads) st2(tw1 + k, a.tw1[k]);
+  for (int k = tid; k < n2; k += kThreads) st2(tw2 + k, a.tw2[k]);
+  __syncthreads();
+
+  // XCD-aware decode of the 1D grid: XCD x takes planes x, x+8, ... with all
+  // states of a plane consecutive.
+  const int id = blockIdx.x;
+  const int xcd = id & 7, local = id >> 3;
+  const int i3 = (local / nstates) * 8 + xcd;
+  const int sl = local - (local / nstates) * nstates;
+  if (i3 >= a.n3) return;
+  const int s = a.state0 + sl;
+  const long long nplane = (long long)N * n2;
+  const long long nvox = nplane * a.n3;
+  const float inv_sqrt = rsqrtf((float)N * (float)n2);
+  const float inv_n = 1.0f / (float)N;
+  const int o0 = a.off[s], o1 = a.off[s + 1];
+  const float2* Up = FWD ? a.U + (a.us_rel ? sl : s) * a.us + i3 * nplane : nullptr;
+  float2* Vp = ADJ ? a.V + (long long)(a.vs_rel ? sl : s) * a.vs + i3 * nplane : nullptr;
+  const int ngroups = (n2 + 15) / 16;
+  const double cf = a.tfwd ? a.tfwd[s] * a.invN1 : 0.0;
+  const double ca = a.tadj ? a.tadj[s] * a.invN1 : 0.0;
+
+  bool first_out = true;
+  for (int sc0 = o0; sc0 < o1 || (sc0 == o0 && ADJ); sc0 += kRJ * kThreads) {
+    const int sc1 = min(o1, sc0 + kRJ * kThreads);
+    int jk1[kRJ], jk2[kRJ];
+    float jw[kRJ];
+#pragma unroll
+    for (int i = 0; i < kRJ; ++i) {
+      const int j = sc0 + tid + i * kThreads;
+      const bool ok = j < sc1;
+      jk1[i] = ok ? a.k1[j] : -1;
+      jk2[i] = ok ? a.k2[j] : 0;
+      jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
+    }
+    float esum[kRJ];
+#pragma unroll
+    for (int i = 0; i < kRJ; ++i) esum[i] = 0.f;
+
+    for (int c0 = 0; c0 < a.C; c0 += kCC) {
+      const int cc = min(kCC, a.C - c0);
+      float2 W[kRJ][kCC];
+#pragma unroll
+      for (int i = 0; i < kRJ; ++i)
+#pragma unroll
+        for (int c = 0; c < kCC; ++c) W[i][c] = make_float2(0.f, 0.f);
+
+      if (FWD) {
+        for (int g = 0; g < ngroups; ++g) {
+          const int r0 = g * 16;
+          const int Gv = min(16, n2 - r0);
+          const bool row_ok = p < Gv;
+          const long long rowoff = (long long)(r0 + p) * N;
+          float2 u[16];
+#pragma unroll
+          for (int r = 0; r < R1; ++r) u[r] = row_ok ? Up[rowoff + t + 16 * r] : make_float2(0.f, 0.f);
+          if (a.tfwd) {  // last translation of T along dim 1
+            reg_fft<R1, 16>(u, t, xb, tw1);
+            if (t < R1) {
+              float2 ph[16];
+              ramp16<R1>(ph, cf, t);
+#pragma unroll
+              for (int k = 0; k < 16; ++k) u[k] = cswap(cmul(u[k], ph[k]));
+            }
+            reg_fft<16, R1>(u, t, xb, tw1);
+#pragma unroll
+            for (int r = 0; r < R1; ++r) u[r] = cscale(cswap(u[r]), inv_n);
+          }
+          for (int c = 0; c < cc; ++c) {
+            const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
+            float2 v[16];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) v[r] = row_ok ? cmul(u[r], Sp[t + 16 * r]) : make_float2(0.f, 0.f);
+            reg_fft<R1, 16>(v, t, xb, tw1);
+            if (t < R1) {
+#pragma unroll
+              for (int k = 0; k < 16; ++k) st2(Zt + p * N + t + R1 * k, v[k]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kRJ; ++i) {
+              if (jk1[i] >= 0) {
+                int idx = (int)(((long long)jk2[i] * r0) % n2);
+                float2 acc = make_float2(0.f, 0.f);
+                for (int q = 0; q < Gv; ++q) {
+                  acc = cadd(acc, cmul(ld2(tw2 + idx), ld2(Zt + q * N + jk1[i])));
+                  idx += jk2[i];
+                  if (idx >= n2) idx -= n2;
+                }
+#pragma unroll
+                for (int cq = 0; cq < kCC; ++cq)
+                  if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
+              }
+            }
+            __syncthreads();
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < kRJ; ++i) {
+          const int j = sc0 + tid + i * kThreads;
+          if (jk1[i] >= 0) {
+#pragma unroll
+            for (int c = 0; c < kCC; ++c) {
+              if (c < cc) {
+                const float2 Y = cscale(W[i][c], inv_sqrt);
+                const long long yi = j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c));
+                if (MODE == COIL_ENCODE) a.yout[yi] = Y;
+                if (MODE == COIL_ENERGY) {
+                  const float2 r = csub(Y, a.yin[yi]);
+                  esum[i] += r.x * r.x + r.y * r.y;
+                }
+                if (MODE == COIL_EHE) W[i][c] = cscale(Y, jw[i]);
+              }
+            }
+          }
+        }
+      }
+      if (MODE == COIL_DECODE) {
+#pragma unroll
+        for (int i = 0; i < kRJ; ++i) {
+          const int j = sc0 + tid + i * kThreads;
+          if (jk1[i] >= 0) {
+#pragma unroll
+            for (int c = 0; c < kCC; ++c)
+              if (c < cc) W[i][c] = cscale(a.yin[j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c))], jw[i]);
+          }
+        }
+      }
+
+      if (ADJ) {
+        for (int g = 0; g < ngroups; ++g) {
+          const int r0 = g * 16;
+          const int Gv = min(16, n2 - r0);
+          const bool row_ok = p < Gv;
+          const long long rowoff = (long long)(r0 + p) * N;
+          float2 o[16];
+#pragma unroll
+          for (int k = 0; k < 16; ++k) o[k] = make_float2(0.f, 0.f);
+          for (int c = 0; c < cc; ++c) {
+#pragma unroll
+            for (int r = 0; r < R1; ++r) st2(Zt + p * N + t + 16 * r, make_float2(0.f, 0.f));
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < kRJ; ++i) {
+              if (jk1[i] >= 0) {
+                float2 w = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int cq = 0; cq < kCC; ++cq)
+                  if (cq == c) w = W[i][cq];
+                int idx = (int)(((long long)jk2[i] * r0) % n2);
+                for (int q = 0; q < Gv; ++q) {
+                  // conj(W2^{k2 i2}) * w, real/imag swapped for the inverse FFT
+                  const float2 vv = cmulc(w, ld2(tw2 + idx));
+                  __attribute__((address_space(3))) float* dst =
+                      (__attribute__((address_space(3))) float*)(Zt + q * N + jk1[i]);
+                  atomicAdd((float*)dst, vv.y);
+                  atomicAdd((float*)dst + 1, vv.x);
+                  idx += jk2[i];
+                  if (idx >= n2) idx -= n2;
+                }
+              }
+            }
+            __syncthreads();
+            float2 v[16];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * N + t + 16 * r);
+            reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
+            if (t < R1 && row_ok) {
+              const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
+#pragma unroll
+              for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), Sp[t + R1 * k]), inv_sqrt));
+            }
+          }
+          if (a.tadj) {  // first translation of T^H along dim 1 (input layout of reg_fft<16,R1>)
+            reg_fft<16, R1>(o, t, xb, tw1);
+            if (t < 16) {
+              float2 ph[16];
+              ramp_in<R1>(ph, ca, t);
+#pragma unroll
+              for (int r = 0; r < R1; ++r) o[r] = cswap(cmul(o[r], ph[r]));
+            }
+            reg_fft<R1, 16>(o, t, xb, tw1);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) o[k] = cscale(cswap(o[k]), inv_n);
+          }
+          if (t < R1 && row_ok) {
+            float2* Vt = Vp + rowoff;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) Vt[t + R1 * k] = first_out ? o[k] : cadd(Vt[t + R1 * k], o[k]);
+          }
+        }
+        first_out = false;
+      }
+    }
+    if (MODE == COIL_ENERGY) {
+#pragma unroll
+      for (int i = 0; i < kRJ; ++i) {
+        const int j = sc0 + tid + i * kThreads;
+        if (jk1[i] >= 0) a.epart[j + (long long)a.Ns * i3] = esum[i];
+      }
+    }
+    if (sc1 >= o1) break;
+  }
+}
+
+template <int R1>
+hipError_t launch_rcoil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  const int nblk = ((a.n3 + 7) / 8) * 8 * nstates;
+  dim3 grid(nblk);
+  switch (mode) {
+    case COIL_ENCODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_ENCODE>), grid, dim3(kThreads), 0, st, a, nstates); break;
+    case COIL_DECODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_DECODE>), grid, dim3(kThreads), 0, st, a, nstates); break;
+    case COIL_EHE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_EHE>), grid, dim3(kThreads), 0, st, a, nstates); break;
+    case COIL_ENERGY: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_ENERGY>), grid, dim3(kThreads), 0, st, a, nstates); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  if (a.n2 <= 1024 && !a.force_generic) {
+    switch (reg_radix(a.n1)) {
+      case 16: return launch_rcoil<16>(a, mode, nstates, st);
+      case 12: return launch_rcoil<12>(a, mode, nstates, st);
+      case 8: return launch_rcoil<8>(a, mode, nstates, st);
+      case 4: return launch_rcoil<4>(a, mode, nstates, st);
+      case 2: return launch_rcoil<2>(a, mode, nstates, st);
+      default: break;
+    }
+  }
+  return launch_coil_generic(a, mode, nstates, st);
 }
 
 }  // namespace dis

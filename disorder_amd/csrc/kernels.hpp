@@ -34,12 +34,15 @@ struct PassArgs {
   int S;                  // states looped inside a workgroup (BCAST / ACCUM)
   const int* states;      // MAP: optional list of state indices for grid.y
   int accumulate;         // ACCUM: add into existing `out`
+  int force_generic;      // use the LDS-staged engine even where the register path exists
 };
 
 enum PassMode { MODE_MAP = 0, MODE_BCAST = 1, MODE_ACCUM = 2, MODE_FLIP = 3, MODE_DERIV = 4 };
 
 // Launch a pass of the given mode; grid_states = gridDim.y for MAP/FLIP.
 hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t st);
+hipError_t launch_pass_generic(const PassArgs& a, int mode, int grid_states, hipStream_t st);
+int reg_radix(int n);
 
 // twiddle table tw[k] = exp(-2 pi i k / n), k < n, computed in double.
 hipError_t launch_twiddles(float2* tw, int n, hipStream_t st);
@@ -74,11 +77,13 @@ struct CoilArgs {
   int state0;             // first state handled by this launch
   int vs_rel;             // V indexed relative to state0
   int us_rel;             // U indexed relative to state0
+  int force_generic;      // use the LDS-staged engine even where the register path exists
 };
 
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st);
+hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStream_t st);
 
 // ---------------------------------------------------------------- utilities
 hipError_t launch_bcast(const float2* x, float2* out, long long nvox, int S, hipStream_t st);

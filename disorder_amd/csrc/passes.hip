@@ -10,6 +10,7 @@
 //           frequency domain; one inverse FFT per tile (state sum of T^H)
 //   FLIP  : the >90 degree flip of Methods/flipping.m along one axis
 #include "kernels.hpp"
+#include "regfft.hpp"
 
 namespace dis {
 
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
   }
 }
 
-hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t st) {
+hipError_t launch_pass_generic(const PassArgs& a, int mode, int grid_states, hipStream_t st) {
   const int n = a.plan.n;
   int ntiles;
   if (a.axis == 0) ntiles = (a.n2 * a.n3 + a.P - 1) / a.P;
@@ -317,6 +318,160 @@ hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t
   }
 #undef DIS_LAUNCH
   return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Register path for N = 16 * R1: sixteen lanes per pencil, sixteen pencils
+// per workgroup, data in registers from the global load to the global store,
+// one LDS exchange per transform (regfft.hpp).
+template <int R1, bool CONTIG>
+__device__ __forceinline__ void rload(float2 (&v)[16], const float2* __restrict__ g, const TileInfo& ti, int p,
+                                      int t) {
+  constexpr int N = R1 * 16;
+#pragma unroll
+  for (int r = 0; r < R1; ++r) {
+    const int j = t + 16 * r;
+    v[r] = (p < ti.Pv) ? g[CONTIG ? ti.base + (long long)p * N + j : ti.base + p + (long long)j * ti.jstride]
+                       : make_float2(0.f, 0.f);
+  }
+}
+
+template <int R1, bool CONTIG>
+__device__ __forceinline__ void rstore(const float2 (&v)[16], float2* __restrict__ g, const TileInfo& ti, int p,
+                                       int t, float scale, bool acc) {
+  constexpr int N = R1 * 16;
+  if (p >= ti.Pv) return;
+#pragma unroll
+  for (int r = 0; r < R1; ++r) {
+    const int j = t + 16 * r;
+    const long long ad = CONTIG ? ti.base + (long long)p * N + j : ti.base + p + (long long)j * ti.jstride;
+    float2 o = cscale(cswap(v[r]), scale);
+    if (acc) o = cadd(o, g[ad]);
+    g[ad] = o;
+  }
+}
+
+__device__ __forceinline__ double lane_coef(const PassArgs& a, const TileInfo& ti, int s, int p) {
+  const double A = a.coefA ? a.coefA[s] : 0.0;
+  const double B = a.coefB ? a.coefB[s] : 0.0;
+  double r = 0.0;
+  if (a.src >= 0) r = a.rscale * (double)(src_coord(a, ti, p) + 1 - a.rcent);
+  return (A * r + B) * a.invN;
+}
+
+template <int R1, bool CONTIG, int MODE>
+__global__ __launch_bounds__(kThreads) void rpass_kernel(PassArgs a) {
+  constexpr int N = R1 * 16;
+  constexpr int XB = RegFftSize<R1>::XB;
+  __shared__ fvec2 xraw[16 * XB];
+  __shared__ fvec2 traw[N];
+  const int tid = threadIdx.x;
+  const int p = CONTIG ? (tid >> 4) : (tid & 15);
+  const int t = CONTIG ? (tid & 15) : (tid >> 4);
+  lds_float2* tw = (lds_float2*)traw;
+  lds_float2* xb = (lds_float2*)xraw + p * XB;
+  for (int k = tid; k < N; k += kThreads) st2(tw + k, a.tw[k]);
+  __syncthreads();
+  const TileInfo ti = tile_info(a, blockIdx.x);
+  const float scale = 1.0f / (float)N;
+  float2 v[16];
+  if (MODE == MODE_MAP) {
+    const int s = a.states ? a.states[blockIdx.y] : (int)blockIdx.y;
+    const double c = lane_coef(a, ti, s, p);
+    rload<R1, CONTIG>(v, a.in + a.in_sstride * s, ti, p, t);
+    reg_fft<R1, 16>(v, t, xb, tw);
+    if (t < R1) {
+      float2 ph[16];
+      ramp16<R1>(ph, c, t);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = cswap(cmul(v[k], ph[k]));
+    }
+    reg_fft<16, R1>(v, t, xb, tw);
+    rstore<R1, CONTIG>(v, a.out + a.out_sstride * s, ti, p, t, scale, false);
+  } else if (MODE == MODE_BCAST) {
+    rload<R1, CONTIG>(v, a.in, ti, p, t);
+    reg_fft<R1, 16>(v, t, xb, tw);
+    float2 spec[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) spec[k] = v[k];
+    for (int s = 0; s < a.S; ++s) {
+      const double c = lane_coef(a, ti, s, p);
+      if (t < R1) {
+        float2 ph[16];
+        ramp16<R1>(ph, c, t);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = cswap(cmul(spec[k], ph[k]));
+      }
+      reg_fft<16, R1>(v, t, xb, tw);
+      rstore<R1, CONTIG>(v, a.out + a.out_sstride * s, ti, p, t, scale, false);
+    }
+  } else if (MODE == MODE_ACCUM) {
+    float2 acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = make_float2(0.f, 0.f);
+    for (int s = 0; s < a.S; ++s) {
+      const double c = lane_coef(a, ti, s, p);
+      rload<R1, CONTIG>(v, a.in + a.in_sstride * s, ti, p, t);
+      reg_fft<R1, 16>(v, t, xb, tw);
+      if (t < R1) {
+        float2 ph[16];
+        ramp16<R1>(ph, c, t);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc[k] = cadd(acc[k], cmul(v[k], ph[k]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = cswap(acc[k]);
+    reg_fft<16, R1>(v, t, xb, tw);
+    rstore<R1, CONTIG>(v, a.out, ti, p, t, scale, a.accumulate != 0);
+  }
+}
+
+// R1 of the register path for length n (0 when not available)
+int reg_radix(int n) {
+  switch (n) {
+    case 256: return 16;
+    case 192: return 12;
+    case 128: return 8;
+    case 64: return 4;
+    case 32: return 2;
+    default: return 0;
+  }
+}
+
+template <int R1>
+hipError_t launch_rpass(PassArgs a, int mode, int grid_states, hipStream_t st) {
+  a.P = 16;
+  a.tiles_per_line = (a.n1 + 15) / 16;
+  int ntiles;
+  if (a.axis == 0) ntiles = (a.n2 * a.n3 + 15) / 16;
+  else ntiles = a.tiles_per_line * (a.axis == 1 ? a.n3 : a.n2);
+  dim3 grid(ntiles, mode == MODE_MAP ? grid_states : 1);
+  const bool contig = a.axis == 0;
+#define DIS_RL(C, M) hipLaunchKernelGGL((rpass_kernel<R1, C, M>), grid, dim3(kThreads), 0, st, a)
+  switch (mode) {
+    case MODE_MAP: if (contig) DIS_RL(true, MODE_MAP); else DIS_RL(false, MODE_MAP); break;
+    case MODE_BCAST: if (contig) DIS_RL(true, MODE_BCAST); else DIS_RL(false, MODE_BCAST); break;
+    case MODE_ACCUM: if (contig) DIS_RL(true, MODE_ACCUM); else DIS_RL(false, MODE_ACCUM); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef DIS_RL
+  return hipGetLastError();
+}
+
+hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t st) {
+  const int n = a.plan.n;
+  const int R1 = reg_radix(n);
+  if (R1 && mode != MODE_FLIP && (a.axis == 0 || a.n1 >= 16) && !a.force_generic) {
+    switch (R1) {
+      case 16: return launch_rpass<16>(a, mode, grid_states, st);
+      case 12: return launch_rpass<12>(a, mode, grid_states, st);
+      case 8: return launch_rpass<8>(a, mode, grid_states, st);
+      case 4: return launch_rpass<4>(a, mode, grid_states, st);
+      case 2: return launch_rpass<2>(a, mode, grid_states, st);
+    }
+  }
+  return launch_pass_generic(a, mode, grid_states, st);
 }
 
 __global__ void twiddle_kernel(float2* tw, int n) {

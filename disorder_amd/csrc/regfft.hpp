@@ -1,0 +1,116 @@
+// Register-resident two-level FFT for pencils of length N = R1 * 16.
+//
+// Sixteen lanes own one pencil.  Decimation in time with n = t + 16 r and
+// k = k1 + R1 k2:
+//   stage 1 (lane t < 16):  Y_t[k1] = DFT_R1_r( x[t + 16 r] ) * W_N^{t k1}
+//   exchange through LDS (one write, one read per element)
+//   stage 2 (lane k1 < R1): X[k1 + R1 k2] = DFT_16_t( Y_t[k1] )
+// The forward output (lane k1 holds X[k1 + R1 k2]) is exactly the stage-1
+// input pattern of the swapped factorisation (16, R1), so the inverse of a
+// fused pass runs reg_fft<16, R1> on the same registers and returns the data
+// to the original input layout (lane t holds y[t + 16 r]) with no extra
+// exchange.  For R1 = 16 both factorisations are the same.
+#pragma once
+#include "fft_engine.hpp"
+
+namespace dis {
+
+#define DIS_SQRT3_2 0.86602540378443864676f
+
+template <> struct Dft<12> {
+  // r = 4a + b, k = c + 3d:  W12^{rk} = W3^{ac} W12^{bc} W4^{bd}
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 y[4][3];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      float2 t3[3] = {v[b], v[b + 4], v[b + 8]};
+      DftOdd<3>::run(t3);
+      y[b][0] = t3[0];
+      y[b][1] = t3[1];
+      y[b][2] = t3[2];
+    }
+    y[1][1] = wconst(y[1][1], DIS_SQRT3_2, 0.5f);   // W12^1
+    y[1][2] = wconst(y[1][2], 0.5f, DIS_SQRT3_2);   // W12^2
+    y[2][1] = wconst(y[2][1], 0.5f, DIS_SQRT3_2);   // W12^2
+    y[2][2] = wconst(y[2][2], -0.5f, DIS_SQRT3_2);  // W12^4
+    y[3][1] = cmul_mi(y[3][1]);                     // W12^3 = -i
+    y[3][2] = make_float2(-y[3][2].x, -y[3][2].y);  // W12^6 = -1
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float2 a0 = y[0][c], a1 = y[1][c], a2 = y[2][c], a3 = y[3][c];
+      dft4s(a0, a1, a2, a3);
+      v[c] = a0;
+      v[c + 3] = a1;
+      v[c + 6] = a2;
+      v[c + 9] = a3;
+    }
+  }
+};
+
+// Exchange-buffer size (elements) per pencil for both factorisations.
+template <int R1> struct RegFftSize {
+  static constexpr int N = R1 * 16;
+  static constexpr int XB = (R1 * 17 > 16 * (R1 + 1)) ? R1 * 17 : 16 * (R1 + 1);
+};
+
+// One two-level FFT of a pencil: RA x RB = N, lanes t < RB enter with
+// v[r] = x[t + RB r] (r < RA); lanes k1 < RA leave with v[k2] = X[k1 + RA k2]
+// (k2 < RB).  xb: this pencil's exchange region; tw: W_N^i, i < N.  Contains
+// two workgroup barriers (every lane of the workgroup must call it).
+template <int RA, int RB>
+__device__ __forceinline__ void reg_fft(float2 (&v)[16], int t, lds_float2* xb, const lds_float2* tw) {
+  constexpr int LD = RB + 1;  // padded exchange row
+  if (t < RB) {
+    Dft<RA>::run(v);
+#pragma unroll
+    for (int k1 = 1; k1 < RA; ++k1) v[k1] = cmul(v[k1], ld2(tw + t * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < RA; ++k1) st2(xb + k1 * LD + t, v[k1]);
+  }
+  __syncthreads();
+  if (t < RA) {
+#pragma unroll
+    for (int tt = 0; tt < RB; ++tt) v[tt] = ld2(xb + t * LD + tt);
+    Dft<RB>::run(v);
+  }
+  __syncthreads();
+}
+
+// Phase ramp exp(2 pi i c m) over the forward output of lane k1 (< R1):
+// element k2 has frequency index m = k1 + R1 k2 (k2 < 8) or k1 + R1 k2 - N
+// (k2 >= 8, FFT order with the Nyquist bin negative).  Two anchors from
+// double-reduced sincos and a 7-step complex recurrence each.
+template <int R1>
+__device__ __forceinline__ void ramp16(float2 (&ph)[16], double c, int k1) {
+  constexpr int N = R1 * 16;
+  const float2 a0 = phase_cycles(c, k1);
+  const float2 a1 = phase_cycles(c, k1 - N / 2);
+  const float2 w = phase_cycles(c, R1);
+  ph[0] = a0;
+  ph[8] = a1;
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    ph[k] = cmul(ph[k - 1], w);
+    ph[k + 8] = cmul(ph[k + 7], w);
+  }
+}
+
+// Phase ramp over the stage-1 input layout of reg_fft<R1,16> (lane t holds
+// element t + 16 r, r < R1): m = t + 16 r (r < R1/2) or t + 16 r - N.
+template <int R1>
+__device__ __forceinline__ void ramp_in(float2 (&ph)[16], double c, int t) {
+  constexpr int N = R1 * 16;
+  constexpr int H = R1 / 2;
+  const float2 a0 = phase_cycles(c, t);
+  const float2 a1 = phase_cycles(c, t - N / 2);
+  const float2 w = phase_cycles(c, 16);
+  ph[0] = a0;
+  ph[H] = a1;
+#pragma unroll
+  for (int k = 1; k < H; ++k) {
+    ph[k] = cmul(ph[k - 1], w);
+    ph[k + H] = cmul(ph[k + H - 1], w);
+  }
+}
+
+}  // namespace dis

@@ -23,7 +23,10 @@ def to_np(t):
 
 
 @pytest.mark.parametrize("n,Nfull", [((16, 12, 10), None), ((16, 12, 10), (32, 24, 20)), ((15, 14, 9), None),
-                                     ((28, 20, 21), (56, 40, 42)), ((64, 48, 64), None)])
+                                     ((28, 20, 21), (56, 40, 42)), ((64, 48, 64), None),
+                                     # register-path lengths 32, 64, 128, 192, 256 on every axis
+                                     ((32, 64, 128), (64, 128, 256)), ((192, 16, 32), None), ((256, 32, 20), None),
+                                     ((24, 256, 16), None), ((16, 18, 192), None)])
 def test_rigid_forward_inverse(dev, n, Nfull):
     import disorder_amd as D
     rng = np.random.default_rng(sum(n))
@@ -102,7 +105,12 @@ def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
 
 @pytest.mark.parametrize("n,C,S,nmotion,weights", [((16, 12, 8), 3, 4, None, False),
                                                     ((16, 12, 8), 3, 5, 4, True),
-                                                    ((20, 15, 12), 9, 6, None, True)])
+                                                    ((20, 15, 12), 9, 6, None, True),
+                                                    # register-path coil stage (n1 = 32, 64, 192, 256)
+                                                    ((32, 24, 16), 3, 4, None, True),
+                                                    ((64, 20, 8), 8, 5, 4, True),
+                                                    ((192, 16, 6), 2, 4, None, False),
+                                                    ((256, 12, 4), 10, 4, None, True)])
 def test_encode_decode_ehe(dev, n, C, S, nmotion, weights):
     import disorder_amd as D
     rng = np.random.default_rng(11)
