@@ -47,6 +47,8 @@ struct dis_plan {
   int* k2 = nullptr;
   int* off = nullptr;
   float* We = nullptr;
+  int* rowptr = nullptr;  // [S][n1+1] per-state CSR by k1
+  int* csr = nullptr;     // [Ns]
   std::vector<int> hoff;
 };
 
@@ -347,6 +349,32 @@ int dis_plan_create(dis_ctx* c, const dis_geom* g, const void* S_maps, const int
          hipMemcpy(p->k2, k2, g->nsamples * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   }
   if (ok) ok = hipMemcpy(p->off, state_off, (g->nstates + 1) * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  // per-state CSR of the samples by phase-encode row k1 (stable in sample
+  // order): rowptr[s][k1], entries (k2 << 16) | (j - off[s])
+  if (ok) {
+    const int n1 = g->n[0];
+    std::vector<int> rp((size_t)g->nstates * (n1 + 1), 0), ent(std::max(1, g->nsamples));
+    for (int s = 0; s < g->nstates; ++s) {
+      const int o0 = state_off[s], o1 = state_off[s + 1];
+      if (o1 - o0 > 65535) {
+        ok = false;
+        break;
+      }
+      int* r = rp.data() + (size_t)s * (n1 + 1);
+      for (int j = o0; j < o1; ++j) ++r[k1[j] + 1];
+      for (int k = 0; k < n1; ++k) r[k + 1] += r[k];
+      std::vector<int> fill(r, r + n1);
+      for (int j = o0; j < o1; ++j) ent[o0 + fill[k1[j]]++] = (k2[j] << 16) | (j - o0);
+    }
+    if (!ok) {
+      dis_plan_destroy(p);
+      return fail(c, DIS_E_UNSUPPORTED, "dis_plan_create: more than 65535 samples in one state");
+    }
+    ok = hipMalloc(&p->rowptr, rp.size() * sizeof(int)) == hipSuccess &&
+         hipMalloc(&p->csr, ent.size() * sizeof(int)) == hipSuccess &&
+         hipMemcpy(p->rowptr, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(p->csr, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  }
   if (!ok) {
     dis_plan_destroy(p);
     return fail(c, DIS_E_NOMEM, "dis_plan_create: device allocation failed");
@@ -363,6 +391,8 @@ int dis_plan_destroy(dis_plan* p) {
   if (p->k2) (void)hipFree(p->k2);
   if (p->off) (void)hipFree(p->off);
   if (p->We) (void)hipFree(p->We);
+  if (p->rowptr) (void)hipFree(p->rowptr);
+  if (p->csr) (void)hipFree(p->csr);
   delete p;
   return DIS_OK;
 }
@@ -393,6 +423,8 @@ OpView plan_view(dis_plan* p) {
   v.k2 = p->k2;
   v.off = p->off;
   v.We = p->We;
+  v.rowptr = p->rowptr;
+  v.csr = p->csr;
   v.g = p->g;
   v.hoff = p->hoff.data();
   return v;

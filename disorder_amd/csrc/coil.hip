@@ -296,16 +296,19 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   constexpr bool ADJ = MODE == COIL_DECODE || MODE == COIL_EHE;
   constexpr int N = R1 * 16;
   constexpr int XB = RegFftSize<R1>::XB;
-  static_assert(16 * XB >= 16 * N, "exchange region doubles as the row tile");
-  __shared__ fvec2 xraw[16 * XB];
+  constexpr int XZ = (16 * XB > 16 * N) ? 16 * XB : 16 * N;
+  constexpr int kEntCap = 2048;
+  __shared__ fvec2 xzraw[XZ];                 // FFT exchange, aliased with the 16-row tile
+  __shared__ fvec2 wcraw[kRJ * kThreads];     // adjoint: sample values of the current coil
+  __shared__ int csrraw[N + 1 + kEntCap];     // adjoint: the state's CSR by k1
   __shared__ fvec2 t1raw[N];
   __shared__ fvec2 t2raw[1024];
   const int tid = threadIdx.x;
   const int p = tid >> 4, t = tid & 15;
   lds_float2* tw1 = (lds_float2*)t1raw;
   lds_float2* tw2 = (lds_float2*)t2raw;
-  lds_float2* xb = (lds_float2*)xraw + p * XB;
-  lds_float2* Zt = (lds_float2*)xraw;  // [16 rows][N] natural order
+  lds_float2* xb = (lds_float2*)xzraw + p * XB;
+  lds_float2* Zt = (lds_float2*)xzraw;  // [16 rows][N] natural order
   const int n2 = a.n2;
   for (int k = tid; k < N; k += kThreads) st2(tw1 + k, a.tw1[k]);
   for (int k = tid; k < n2; k += kThreads) st2(tw2 + k, a.tw2[k]);
@@ -333,7 +336,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   bool first_out = true;
   for (int sc0 = o0; sc0 < o1 || (sc0 == o0 && ADJ); sc0 += kRJ * kThreads) {
     const int sc1 = min(o1, sc0 + kRJ * kThreads);
-    int jk1[kRJ], jk2[kRJ];
+    int jk1[kRJ], jk2[kRJ], jstep[kRJ];
     float jw[kRJ];
 #pragma unroll
     for (int i = 0; i < kRJ; ++i) {
@@ -342,6 +345,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
       jk1[i] = ok ? a.k1[j] : -1;
       jk2[i] = ok ? a.k2[j] : 0;
       jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
+      jstep[i] = (16 * jk2[i]) % n2;  // twiddle-index advance per 16-row group
     }
     float esum[kRJ];
 #pragma unroll
@@ -356,14 +360,20 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         for (int c = 0; c < kCC; ++c) W[i][c] = make_float2(0.f, 0.f);
 
       if (FWD) {
+        int ib[kRJ];  // W2 index of row r0 for each sample: (k2 * r0) mod n2
+#pragma unroll
+        for (int i = 0; i < kRJ; ++i) ib[i] = 0;
         for (int g = 0; g < ngroups; ++g) {
           const int r0 = g * 16;
           const int Gv = min(16, n2 - r0);
           const bool row_ok = p < Gv;
-          const long long rowoff = (long long)(r0 + p) * N;
+          const long long rowoff = (long long)(r0 + (row_ok ? p : 0)) * N;
           float2 u[16];
 #pragma unroll
-          for (int r = 0; r < R1; ++r) u[r] = row_ok ? Up[rowoff + t + 16 * r] : make_float2(0.f, 0.f);
+          for (int r = 0; r < R1; ++r) {
+            const float2 ld = Up[rowoff + t + 16 * r];
+            u[r] = row_ok ? ld : make_float2(0.f, 0.f);
+          }
           if (a.tfwd) {  // last translation of T along dim 1
             reg_fft<R1, 16>(u, t, xb, tw1);
             if (t < R1) {
@@ -380,7 +390,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
             const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
             float2 v[16];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) v[r] = row_ok ? cmul(u[r], Sp[t + 16 * r]) : make_float2(0.f, 0.f);
+            for (int r = 0; r < R1; ++r) v[r] = cmul(u[r], Sp[t + 16 * r]);
             reg_fft<R1, 16>(v, t, xb, tw1);
             if (t < R1) {
 #pragma unroll
@@ -390,10 +400,11 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
             for (int i = 0; i < kRJ; ++i) {
               if (jk1[i] >= 0) {
-                int idx = (int)(((long long)jk2[i] * r0) % n2);
+                int idx = ib[i];
                 float2 acc = make_float2(0.f, 0.f);
-                for (int q = 0; q < Gv; ++q) {
-                  acc = cadd(acc, cmul(ld2(tw2 + idx), ld2(Zt + q * N + jk1[i])));
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                  if (q < Gv) acc = cadd(acc, cmul(ld2(tw2 + idx), ld2(Zt + q * N + jk1[i])));
                   idx += jk2[i];
                   if (idx >= n2) idx -= n2;
                 }
@@ -402,7 +413,12 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                   if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
               }
             }
-            __syncthreads();
+            __syncthreads();  // the row tile aliases the next FFT's exchange buffer
+          }
+#pragma unroll
+          for (int i = 0; i < kRJ; ++i) {
+            ib[i] += jstep[i];
+            if (ib[i] >= n2) ib[i] -= n2;
           }
         }
 #pragma unroll
@@ -424,6 +440,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
             }
           }
         }
+        __syncthreads();  // last sample gathers done before the adjoint reuses Zt
       }
       if (MODE == COIL_DECODE) {
 #pragma unroll
@@ -438,44 +455,76 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
       }
 
       if (ADJ) {
+        // Atomic-free adjoint: the weighted samples of coil c are published
+        // in LDS; lane k owns phase-encode row k1 = k and adds each of its
+        // samples into all 16 dim-2 rows of the group (exclusive writes),
+        // then every row is inverse-FFT'd along dim 1.
+        lds_float2* wc = (lds_float2*)wcraw;
+        typedef __attribute__((address_space(3))) int lds_int;
+        lds_int* rpl = (lds_int*)csrraw;
+        lds_int* entl = rpl + (N + 1);
+        const int* rp = a.rowptr + (long long)s * (N + 1);
+        const int jl0 = sc0 - o0, jl1 = sc1 - o0;
+        const float fn2 = 1.0f / (float)n2;
+        const int nent = o1 - o0;
+        const bool csr_lds = nent <= kEntCap;
+        if (csr_lds) {
+          for (int k = tid; k <= N; k += kThreads) rpl[k] = rp[k];
+          for (int e = tid; e < nent; e += kThreads) entl[e] = a.csr[o0 + e];
+        }
         for (int g = 0; g < ngroups; ++g) {
           const int r0 = g * 16;
           const int Gv = min(16, n2 - r0);
           const bool row_ok = p < Gv;
-          const long long rowoff = (long long)(r0 + p) * N;
+          const int i2 = r0 + (row_ok ? p : 0);
+          const long long rowoff = (long long)i2 * N;
           float2 o[16];
 #pragma unroll
           for (int k = 0; k < 16; ++k) o[k] = make_float2(0.f, 0.f);
           for (int c = 0; c < cc; ++c) {
 #pragma unroll
-            for (int r = 0; r < R1; ++r) st2(Zt + p * N + t + 16 * r, make_float2(0.f, 0.f));
-            __syncthreads();
-#pragma unroll
             for (int i = 0; i < kRJ; ++i) {
-              if (jk1[i] >= 0) {
-                float2 w = make_float2(0.f, 0.f);
+              float2 w = make_float2(0.f, 0.f);
 #pragma unroll
-                for (int cq = 0; cq < kCC; ++cq)
-                  if (cq == c) w = W[i][cq];
-                int idx = (int)(((long long)jk2[i] * r0) % n2);
-                for (int q = 0; q < Gv; ++q) {
-                  // conj(W2^{k2 i2}) * w, real/imag swapped for the inverse FFT
-                  const float2 vv = cmulc(w, ld2(tw2 + idx));
-                  __attribute__((address_space(3))) float* dst =
-                      (__attribute__((address_space(3))) float*)(Zt + q * N + jk1[i]);
-                  atomicAdd((float*)dst, vv.y);
-                  atomicAdd((float*)dst + 1, vv.x);
-                  idx += jk2[i];
-                  if (idx >= n2) idx -= n2;
+              for (int cq = 0; cq < kCC; ++cq)
+                if (cq == c) w = W[i][cq];
+              st2(wc + tid + i * kThreads, w);
+            }
+            __syncthreads();
+            for (int k1 = tid; k1 < N; k1 += kThreads) {
+              float2 acc[16];
+#pragma unroll
+              for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
+              const int e0 = csr_lds ? rpl[k1] : rp[k1];
+              const int e1 = csr_lds ? rpl[k1 + 1] : rp[k1 + 1];
+              for (int e = e0; e < e1; ++e) {
+                const int ent = csr_lds ? entl[e] : a.csr[o0 + e];
+                const int jl = ent & 0xffff;
+                if (jl >= jl0 && jl < jl1) {
+                  const int k2 = ent >> 16;
+                  const float2 w = ld2(wc + (jl - jl0));
+                  const int prod = k2 * r0;
+                  int idx = prod - n2 * (int)((float)prod * fn2);
+                  idx += idx < 0 ? n2 : 0;
+                  idx -= idx >= n2 ? n2 : 0;
+#pragma unroll
+                  for (int q = 0; q < 16; ++q) {
+                    acc[q] = cadd(acc[q], cmulc(w, ld2(tw2 + idx)));  // conj(W2^{k2 (r0+q)}) w
+                    idx += k2;
+                    idx -= idx >= n2 ? n2 : 0;
+                  }
                 }
               }
+#pragma unroll
+              for (int q = 0; q < 16; ++q) st2(Zt + q * N + k1, cswap(acc[q]));  // swapped for the inverse FFT
             }
             __syncthreads();
             float2 v[16];
 #pragma unroll
             for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * N + t + 16 * r);
+            __syncthreads();  // the row tile aliases the exchange buffer
             reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
-            if (t < R1 && row_ok) {
+            if (t < R1) {
               const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
 #pragma unroll
               for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), Sp[t + R1 * k]), inv_sqrt));
@@ -483,7 +532,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
           }
           if (a.tadj) {  // first translation of T^H along dim 1 (input layout of reg_fft<16,R1>)
             reg_fft<16, R1>(o, t, xb, tw1);
-            if (t < 16) {
+            {
               float2 ph[16];
               ramp_in<R1>(ph, ca, t);
 #pragma unroll

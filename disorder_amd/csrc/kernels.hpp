@@ -78,6 +78,8 @@ struct CoilArgs {
   int vs_rel;             // V indexed relative to state0
   int us_rel;             // U indexed relative to state0
   int force_generic;      // use the LDS-staged engine even where the register path exists
+  const int* rowptr;      // [S][n1+1] per-state row pointers of the samples sorted by k1
+  const int* csr;         // [Ns] (k2 << 16) | (sample index within its state), sorted by k1 per state
 };
 
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };

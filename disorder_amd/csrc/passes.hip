@@ -362,7 +362,7 @@ __device__ __forceinline__ double lane_coef(const PassArgs& a, const TileInfo& t
 template <int R1, bool CONTIG, int MODE>
 __global__ __launch_bounds__(kThreads) void rpass_kernel(PassArgs a) {
   constexpr int N = R1 * 16;
-  constexpr int XB = RegFftSize<R1>::XB;
+  constexpr int XB = CONTIG ? RegFftSize<R1>::XB : RegFftSize<R1>::XB_STRIDED;
   __shared__ fvec2 xraw[16 * XB];
   __shared__ fvec2 traw[N];
   const int tid = threadIdx.x;

@@ -47,10 +47,17 @@ template <> struct Dft<12> {
   }
 };
 
-// Exchange-buffer size (elements) per pencil for both factorisations.
+// Exchange-buffer size (elements) per pencil for both factorisations,
+// padded so that the lanes of a half-wavefront (32 lanes, LDS b64 accesses
+// are served per half) hit distinct banks:
+//   CONTIG (lane = 16 p + t): two pencils of 16 consecutive t -> stride = 16 mod 32
+//   strided (lane = p + 16 t): sixteen pencils, two t values -> stride = 2 mod 32 (x odd)
 template <int R1> struct RegFftSize {
   static constexpr int N = R1 * 16;
-  static constexpr int XB = (R1 * 17 > 16 * (R1 + 1)) ? R1 * 17 : 16 * (R1 + 1);
+  static constexpr int BASE = (R1 * 17 > 16 * (R1 + 1)) ? R1 * 17 : 16 * (R1 + 1);
+  static constexpr int pad_to(int base, int residue) { return base + (((residue - base % 32) % 32) + 32) % 32; }
+  static constexpr int XB = pad_to(BASE, 16);
+  static constexpr int XB_STRIDED = pad_to(BASE, 18);
 };
 
 // One two-level FFT of a pencil: RA x RB = N, lanes t < RB enter with

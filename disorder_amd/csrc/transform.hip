@@ -361,6 +361,8 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
     a.k2 = v.k2;
     a.off = v.off;
     a.We = v.We;
+    a.rowptr = v.rowptr;
+    a.csr = v.csr;
     a.Ns = g.nsamples;
     a.yin = yin;
     a.yout = yout;

@@ -28,6 +28,8 @@ struct OpView {
   const int* k2;
   const int* off;
   const float* We;
+  const int* rowptr;  // per-state CSR of the samples by k1
+  const int* csr;
   const int* hoff;  // host copy of the offsets
 };
 
