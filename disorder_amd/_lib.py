@@ -54,6 +54,11 @@ SIGNATURES = {
     "dis_energy": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p, c_float_p, c_double_p, c_double_p]),
     "dis_cg": (c_int, [c_void_p, c_double_p, c_void_p, c_float_p, c_float_p, ctypes.c_double, c_void_p, c_int32,
                        c_int32, c_int32, ctypes.c_double, c_double_p, c_int32_p, c_int32_p]),
+    "dis_rigid_transform_grad": (c_int, [c_void_p, c_int32_p, c_int32_p, c_double_p, c_int32, c_void_p, c_void_p,
+                                         c_void_p]),
+    "dis_lm_partials": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p, c_double_p]),
+    "dis_lm": (c_int, [c_void_p, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_double, c_double_p,
+                       c_void_p, c_void_p, c_double_p, c_int32_p]),
     "dis_profile_enable": (c_int, [c_void_p, c_int32]),
     "dis_profile_read": (c_int, [c_void_p, c_int32, ctypes.POINTER(ctypes.c_int64), c_double_p, c_double_p]),
 }

@@ -501,3 +501,50 @@ extern "C" int dis_cg(dis_plan* p, const double* T, const void* y, const float* 
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DIS_OK;
 }
+
+extern "C" int dis_rigid_transform_grad(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], const double* T,
+                                        int32_t S, const void* x, void* Tx, void* G) {
+  if (!c) return DIS_E_INVALID;
+  if (!n || !Nfull || !T || !x || !G || S <= 0) return fail(c, DIS_E_INVALID, "dis_rigid_transform_grad: invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  Grid gr;
+  int st = make_grid(c, n, Nfull, &gr);
+  if (st) return st;
+  const size_t vol = sizeof(float2) * gr.nvox;
+  void* ws = nullptr;
+  st = ctx_workspace(c, vol * (size_t)S * (Tx ? 5 : 6), &ws);
+  if (st) return st;
+  float2* X0 = (float2*)ws;
+  float2* Y = X0 + 3 * gr.nvox * S;
+  float2* Txp = Tx ? (float2*)Tx : Y + 2 * gr.nvox * S;
+  std::vector<double> Th(T, T + 6 * (size_t)S);
+  st = rigid_gradient(c, gr, Th.data(), S, S, (const float2*)x, Txp, (float2*)G, X0, Y);
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+extern "C" int dis_lm_partials(dis_plan* p, const double* T, const void* x, const void* y, double* sums) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!T || !x || !y || !sums) return fail(c, DIS_E_INVALID, "dis_lm_partials: null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  int st = lm_partials(c, plan_view(p), T, (const float2*)x, (const float2*)y, sums);
+  if (st) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
+extern "C" int dis_lm(dis_plan* p, double* T, double* w, uint8_t* cT, double winit, const double* tL, const void* x,
+                      const void* y, double* En, int32_t* n_trials) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!T || !w || !cT || !tL || !x || !y || !En) return fail(c, DIS_E_INVALID, "dis_lm: null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  int nt = 0;
+  int st = run_lm(c, plan_view(p), T, w, cT, winit, tL, (const float2*)x, (const float2*)y, En, &nt);
+  if (st) return st;
+  if (n_trials) *n_trials = nt;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
   const int n1 = a.n1, n2 = a.n2;
   const int i3 = blockIdx.x;
   const int sl = blockIdx.y;
-  const int s = a.state0 + sl;
+  const int s = a.slist ? a.slist[sl] : a.state0 + sl;
   const int G = a.G;
   const long long nplane = (long long)n1 * n2;
   const long long nvox = nplane * a.n3;
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   const int i3 = (local / nstates) * 8 + xcd;
   const int sl = local - (local / nstates) * nstates;
   if (i3 >= a.n3) return;
-  const int s = a.state0 + sl;
+  const int s = a.slist ? a.slist[sl] : a.state0 + sl;
   const long long nplane = (long long)N * n2;
   const long long nvox = nplane * a.n3;
   const float inv_sqrt = rsqrtf((float)N * (float)n2);

@@ -35,9 +35,18 @@ struct PassArgs {
   const int* states;      // MAP: optional list of state indices for grid.y
   int accumulate;         // ACCUM: add into existing `out`
   int force_generic;      // use the LDS-staged engine even where the register path exists
+  // DUAL mode (derivative chains of the transform):
+  //   X1 = F in, X2 = F in2 (0 when in2 is null),
+  //   out  = F^-1(phi X1)              (skipped when out is null)
+  //   out2 = F^-1(phi (beta X1 + X2))  with beta = i (A2_s r_src + B2_s) k_axis
+  const float2* in2;
+  float2* out2;
+  long long in2_sstride, out2_sstride;
+  const double* coefA2;
+  const double* coefB2;
 };
 
-enum PassMode { MODE_MAP = 0, MODE_BCAST = 1, MODE_ACCUM = 2, MODE_FLIP = 3, MODE_DERIV = 4 };
+enum PassMode { MODE_MAP = 0, MODE_BCAST = 1, MODE_ACCUM = 2, MODE_FLIP = 3, MODE_DUAL = 4 };
 
 // Launch a pass of the given mode; grid_states = gridDim.y for MAP/FLIP.
 hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t st);
@@ -80,6 +89,7 @@ struct CoilArgs {
   int force_generic;      // use the LDS-staged engine even where the register path exists
   const int* rowptr;      // [S][n1+1] per-state row pointers of the samples sorted by k1
   const int* csr;         // [Ns] (k2 << 16) | (sample index within its state), sorted by k1 per state
+  const int* slist;       // optional [nstates] list of the states handled (U/V indexed by list position)
 };
 
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };

@@ -219,6 +219,78 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
         }
       }
     }
+  } else if (MODE == MODE_DUAL) {
+    const int s = a.states ? a.states[blockIdx.y] : (int)blockIdx.y;
+    pencil_coefs(cP, a, ti, s, tid);
+    __shared__ double cB[kMaxPencils];
+    {
+      const double A2 = a.coefA2 ? a.coefA2[s] : 0.0;
+      const double B2 = a.coefB2 ? a.coefB2[s] : 0.0;
+      for (int pp = tid; pp < P; pp += kThreads) {
+        double r = 0.0;
+        if (a.src >= 0) r = a.rscale * (double)(src_coord(a, ti, pp) + 1 - a.rcent);
+        cB[pp] = (A2 * r + B2) * 2.0 * M_PI * a.invN;  // gamma * 2 pi / Nfull: beta = i cB m
+      }
+    }
+    float2 x1[kE];
+    load_tile<CONTIG>(lds, a.in + a.in_sstride * s, ti, n, P, tid);
+    __syncthreads();
+    fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      x1[i] = (e < tot) ? lds[e] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 x2[kE];
+    if (a.in2) {
+      load_tile<CONTIG>(lds, a.in2 + a.in2_sstride * s, ti, n, P, tid);
+      __syncthreads();
+      fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+#pragma unroll
+      for (int i = 0; i < kE; ++i) {
+        const int e = tid + i * kThreads;
+        x2[i] = (e < tot) ? lds[e] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+    } else {
+#pragma unroll
+      for (int i = 0; i < kE; ++i) x2[i] = make_float2(0.f, 0.f);
+    }
+    for (int which = 0; which < 2; ++which) {
+      float2* dst = which == 0 ? a.out : a.out2;
+      if (!dst) continue;
+      const long long dss = which == 0 ? a.out_sstride : a.out2_sstride;
+#pragma unroll
+      for (int i = 0; i < kE; ++i) {
+        const int e = tid + i * kThreads;
+        if (e < tot) {
+          int p, j;
+          elem_pj<CONTIG>(e, n, P, p, j);
+          const int m = freq_index(j, n);
+          const float2 ph = phase_cycles(cP[p], m);
+          float2 val = x1[i];
+          if (which == 1) {
+            const float bm = (float)(cB[p] * (double)m);
+            val = cadd(make_float2(-bm * val.y, bm * val.x), x2[i]);  // i beta X1 + X2
+          }
+          lds[e] = cswap(cmul(val, ph));
+        }
+      }
+      __syncthreads();
+      fft_tile<CONTIG>(lds, tw, a.plan, P, tid);
+      float2* o = dst + dss * s;
+#pragma unroll
+      for (int i = 0; i < kE; ++i) {
+        const int e = tid + i * kThreads;
+        if (e < tot) {
+          int p, j;
+          elem_pj<CONTIG>(e, n, P, p, j);
+          if (p < ti.Pv) o[gaddr<CONTIG>(ti, e, n, P)] = cscale(cswap(lds[e]), scale);
+        }
+      }
+      __syncthreads();
+    }
   } else if (MODE == MODE_FLIP) {
     // Methods/flipping.m:11-22 along this axis: shift by h1=-c+1/2 with
     // c = n/2+1, reverse, shift by -h1.  Odd n gives integer shifts, which
@@ -306,7 +378,7 @@ hipError_t launch_pass_generic(const PassArgs& a, int mode, int grid_states, hip
   if (a.axis == 0) ntiles = (a.n2 * a.n3 + a.P - 1) / a.P;
   else ntiles = a.tiles_per_line * (a.axis == 1 ? a.n3 : a.n2);
   (void)n;
-  dim3 grid(ntiles, (mode == MODE_MAP || mode == MODE_FLIP) ? grid_states : 1);
+  dim3 grid(ntiles, (mode == MODE_MAP || mode == MODE_FLIP || mode == MODE_DUAL) ? grid_states : 1);
   const bool contig = a.axis == 0;
 #define DIS_LAUNCH(C, M) hipLaunchKernelGGL((pass_kernel<C, M>), grid, dim3(kThreads), 0, st, a)
   switch (mode) {
@@ -314,6 +386,7 @@ hipError_t launch_pass_generic(const PassArgs& a, int mode, int grid_states, hip
     case MODE_BCAST: if (contig) DIS_LAUNCH(true, MODE_BCAST); else DIS_LAUNCH(false, MODE_BCAST); break;
     case MODE_ACCUM: if (contig) DIS_LAUNCH(true, MODE_ACCUM); else DIS_LAUNCH(false, MODE_ACCUM); break;
     case MODE_FLIP: if (contig) DIS_LAUNCH(true, MODE_FLIP); else DIS_LAUNCH(false, MODE_FLIP); break;
+    case MODE_DUAL: if (contig) DIS_LAUNCH(true, MODE_DUAL); else DIS_LAUNCH(false, MODE_DUAL); break;
     default: return hipErrorInvalidValue;
   }
 #undef DIS_LAUNCH
@@ -405,6 +478,50 @@ __global__ __launch_bounds__(kThreads) void rpass_kernel(PassArgs a) {
       reg_fft<16, R1>(v, t, xb, tw);
       rstore<R1, CONTIG>(v, a.out + a.out_sstride * s, ti, p, t, scale, false);
     }
+  } else if (MODE == MODE_DUAL) {
+    const int s = a.states ? a.states[blockIdx.y] : (int)blockIdx.y;
+    const double c = lane_coef(a, ti, s, p);
+    double gam = 0.0;
+    {
+      const double A2 = a.coefA2 ? a.coefA2[s] : 0.0;
+      const double B2 = a.coefB2 ? a.coefB2[s] : 0.0;
+      double r = 0.0;
+      if (a.src >= 0) r = a.rscale * (double)(src_coord(a, ti, p) + 1 - a.rcent);
+      gam = (A2 * r + B2) * 2.0 * M_PI * a.invN;  // beta = i gam m
+    }
+    float2 x2[16];
+    rload<R1, CONTIG>(v, a.in + a.in_sstride * s, ti, p, t);
+    reg_fft<R1, 16>(v, t, xb, tw);
+    if (a.in2) {
+      rload<R1, CONTIG>(x2, a.in2 + a.in2_sstride * s, ti, p, t);
+      reg_fft<R1, 16>(x2, t, xb, tw);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x2[k] = make_float2(0.f, 0.f);
+    }
+    float2 ph[16];
+    if (t < R1) ramp16<R1>(ph, c, t);
+    if (a.out) {
+      float2 w[16];
+      if (t < R1) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) w[k] = cswap(cmul(v[k], ph[k]));
+      }
+      reg_fft<16, R1>(w, t, xb, tw);
+      rstore<R1, CONTIG>(w, a.out + a.out_sstride * s, ti, p, t, scale, false);
+    }
+    if (t < R1) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int j = t + R1 * k;
+        const int m = j < N / 2 ? j : j - N;
+        const float bm = (float)(gam * (double)m);
+        const float2 val = cadd(make_float2(-bm * v[k].y, bm * v[k].x), x2[k]);  // i beta X1 + X2
+        x2[k] = cswap(cmul(val, ph[k]));
+      }
+    }
+    reg_fft<16, R1>(x2, t, xb, tw);
+    rstore<R1, CONTIG>(x2, a.out2 + a.out2_sstride * s, ti, p, t, scale, false);
   } else if (MODE == MODE_ACCUM) {
     float2 acc[16];
 #pragma unroll
@@ -446,13 +563,14 @@ hipError_t launch_rpass(PassArgs a, int mode, int grid_states, hipStream_t st) {
   int ntiles;
   if (a.axis == 0) ntiles = (a.n2 * a.n3 + 15) / 16;
   else ntiles = a.tiles_per_line * (a.axis == 1 ? a.n3 : a.n2);
-  dim3 grid(ntiles, mode == MODE_MAP ? grid_states : 1);
+  dim3 grid(ntiles, (mode == MODE_MAP || mode == MODE_DUAL) ? grid_states : 1);
   const bool contig = a.axis == 0;
 #define DIS_RL(C, M) hipLaunchKernelGGL((rpass_kernel<R1, C, M>), grid, dim3(kThreads), 0, st, a)
   switch (mode) {
     case MODE_MAP: if (contig) DIS_RL(true, MODE_MAP); else DIS_RL(false, MODE_MAP); break;
     case MODE_BCAST: if (contig) DIS_RL(true, MODE_BCAST); else DIS_RL(false, MODE_BCAST); break;
     case MODE_ACCUM: if (contig) DIS_RL(true, MODE_ACCUM); else DIS_RL(false, MODE_ACCUM); break;
+    case MODE_DUAL: if (contig) DIS_RL(true, MODE_DUAL); else DIS_RL(false, MODE_DUAL); break;
     default: return hipErrorInvalidValue;
   }
 #undef DIS_RL

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <vector>
 
+#include "lm.hpp"
 #include "transform.hpp"
 
 namespace dis {
@@ -442,6 +443,436 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
       *eo->total = t;
     }
   }
+  return DIS_OK;
+}
+
+}  // namespace dis
+
+// ===========================================================================
+// Transform gradient (Methods/sincRigidTransformGradient.m:28-147) and the LM
+// step (Reconstruction/LMsolver.m:43-225).
+// ===========================================================================
+namespace dis {
+
+namespace {
+
+// Per-state coefficient arrays of the gradient chain, staged for one upload.
+struct GradCoefs {
+  size_t tanA[3], sinA[3], tc[3], mcos[3], tB[3], minus1;
+  size_t flist[3];
+  int nflip[3];
+};
+
+GradCoefs stage_grad(const double* T, int S, int Sld, Staging& st) {
+  GradCoefs gc{};
+  std::vector<double> b(S);
+  std::vector<StateParam> sp(S);
+  for (int s = 0; s < S; ++s) sp[s] = state_param(T, Sld, s);
+  for (int m = 0; m < 3; ++m) {
+    for (int s = 0; s < S; ++s) b[s] = sp[s].tan2[m];  // (-1)^(di-1) tan, di = 1
+    gc.tanA[m] = st.add(b.data(), sizeof(double) * S);
+    for (int s = 0; s < S; ++s) b[s] = -sp[s].sn[m];  // (-1)^di sin
+    gc.sinA[m] = st.add(b.data(), sizeof(double) * S);
+    for (int s = 0; s < S; ++s) b[s] = 0.5 * (1.0 + sp[s].tan2[m] * sp[s].tan2[m]);  // d tan(th/2)/d th
+    gc.tc[m] = st.add(b.data(), sizeof(double) * S);
+    for (int s = 0; s < S; ++s) {
+      // cos of the wrapped angle: tan(th/2) = t2  ->  cos th = (1 - t2^2) / (1 + t2^2)
+      const double t2 = sp[s].tan2[m];
+      b[s] = -(1.0 - t2 * t2) / (1.0 + t2 * t2);
+    }
+    gc.mcos[m] = st.add(b.data(), sizeof(double) * S);
+    for (int s = 0; s < S; ++s) b[s] = -sp[s].t[m];  // (-1)^di t
+    gc.tB[m] = st.add(b.data(), sizeof(double) * S);
+    std::vector<int> lst;
+    for (int s = 0; s < S; ++s)
+      if (sp[s].flip[m]) lst.push_back(s);
+    gc.nflip[m] = (int)lst.size();
+    gc.flist[m] = lst.empty() ? 0 : st.add(lst.data(), sizeof(int) * lst.size());
+  }
+  for (int s = 0; s < S; ++s) b[s] = -1.0;  // etg1 = -i k
+  gc.minus1 = st.add(b.data(), sizeof(double) * S);
+  return gc;
+}
+
+}  // namespace
+
+int rigid_gradient(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, const float2* x, float2* Tx,
+                   float2* G, float2* X0, float2* Y) {
+  Staging st;
+  GradCoefs gc = stage_grad(T, S, Sld, st);
+  const unsigned char* d = nullptr;
+  int rc = ctx_upload_bytes(c, st, &d);
+  if (rc) return rc;
+  hipStream_t stream = ctx_stream(c);
+  const long long nv = gr.nvox;
+  auto dp = [&](size_t off) { return (const double*)(d + off); };
+  auto base = [&](int axis, int src) {
+    Step stp{};
+    stp.axis = axis;
+    stp.src = src;
+    stp.flip_m = -1;
+    PassArgs a = pass_args(gr, stp, d, S);
+    return a;
+  };
+  auto map = [&](int axis, int src, const double* A, const double* B, const float2* in, long long in_ss,
+                 float2* out) -> int {
+    PassArgs a = base(axis, src);
+    a.coefA = A;
+    a.coefB = B;
+    a.in = in;
+    a.in_sstride = in_ss;
+    a.out = out;
+    a.out_sstride = nv;
+    const int ps = prof_begin(c, DIS_K_PASS);
+    DCHK(c, launch_pass(a, MODE_MAP, S, stream));
+    prof_end(c, ps, 16.0 * nv * S);
+    return DIS_OK;
+  };
+  auto dual = [&](int axis, int src, const double* A, const double* B, const double* A2, const double* B2,
+                  const float2* in, long long in_ss, const float2* in2, float2* out, float2* out2) -> int {
+    PassArgs a = base(axis, src);
+    a.coefA = A;
+    a.coefB = B;
+    a.coefA2 = A2;
+    a.coefB2 = B2;
+    a.in = in;
+    a.in_sstride = in_ss;
+    a.in2 = in2;
+    a.in2_sstride = nv;
+    a.out = out;
+    a.out_sstride = nv;
+    a.out2 = out2;
+    a.out2_sstride = nv;
+    const int ps = prof_begin(c, DIS_K_PASS);
+    DCHK(c, launch_pass(a, MODE_DUAL, S, stream));
+    prof_end(c, ps, 8.0 * nv * S * ((in2 ? 2 : 1) + (out ? 1 : 0) + 1));
+    return DIS_OK;
+  };
+  auto flips = [&](int m, float2* buf) -> int {
+    if (!gc.nflip[m]) return DIS_OK;
+    for (int ax : {kPer0[m], kPer1[m]}) {
+      Step f{};
+      f.axis = ax;
+      f.src = -1;
+      f.flip_m = m;
+      PassArgs a = pass_args(gr, f, d, S);
+      a.states = (const int*)(d + gc.flist[m]);
+      a.in = buf;
+      a.in_sstride = nv;
+      a.out = buf;
+      a.out_sstride = nv;
+      DCHK(c, launch_pass(a, MODE_FLIP, gc.nflip[m], stream));
+    }
+    return DIS_OK;
+  };
+  auto rotation = [&](int m, const float2* in, long long in_ss, float2* out, bool with_t2) -> int {
+    const int a = kPer0[m], b = kPer1[m];
+    int r = map(a, b, dp(gc.tanA[m]), nullptr, in, in_ss, out);
+    if (!r) r = map(b, a, dp(gc.sinA[m]), nullptr, out, nv, out);
+    if (!r) r = map(a, b, dp(gc.tanA[m]), with_t2 ? dp(gc.tB[1]) : nullptr, out, nv, out);
+    return r;
+  };
+  auto translate_rest = [&](float2* buf) -> int {  // t3 then t1 (t2 is fused into the last shear)
+    int r = map(2, -1, nullptr, dp(gc.tB[2]), buf, nv, buf);
+    if (!r) r = map(0, -1, nullptr, dp(gc.tB[0]), buf, nv, buf);
+    return r;
+  };
+  float2* X0m[3] = {X0, X0 + nv * S, X0 + 2 * nv * S};
+  float2* Y1 = Y;
+  float2* Y2 = Y + nv * S;
+  // inputs of the three rotations: X0[0] = flip(x) (x itself when no state flips), X0[1], X0[2]
+  const float2* in0 = x;
+  long long in0_ss = 0;
+  if (gc.nflip[0]) {
+    DCHK(c, launch_bcast(x, X0m[0], nv, S, stream));
+    if ((rc = flips(0, X0m[0]))) return rc;
+    in0 = X0m[0];
+    in0_ss = nv;
+  }
+  if ((rc = rotation(0, in0, in0_ss, X0m[1], false))) return rc;
+  if ((rc = flips(1, X0m[1]))) return rc;
+  if ((rc = rotation(1, X0m[1], nv, X0m[2], false))) return rc;
+  if ((rc = flips(2, X0m[2]))) return rc;
+  if ((rc = rotation(2, X0m[2], nv, Tx, true))) return rc;
+  if ((rc = translate_rest(Tx))) return rc;
+  // translation derivatives: d(T x)/dt_d = F_d^-1 (-i k_d) F_d (T x)   (:29-36)
+  for (int dd = 0; dd < 3; ++dd)
+    if ((rc = dual(dd, -1, nullptr, nullptr, nullptr, dp(gc.minus1), Tx, nv, nullptr, nullptr, G + dd * nv * S)))
+      return rc;
+  // rotation derivatives: product rule through the three shears (:39-52), then the
+  // later rotations and the translation
+  for (int m = 0; m < 3; ++m) {
+    const int a = kPer0[m], b = kPer1[m];
+    float2* Gm = G + (3 + m) * nv * S;
+    const float2* xin = m == 0 ? in0 : X0m[m];
+    const long long xin_ss = m == 0 ? in0_ss : nv;
+    if ((rc = dual(a, b, dp(gc.tanA[m]), nullptr, dp(gc.tc[m]), nullptr, xin, xin_ss, nullptr, Y1, Y2))) return rc;
+    if ((rc = dual(b, a, dp(gc.sinA[m]), nullptr, dp(gc.mcos[m]), nullptr, Y1, nv, Y2, Y1, Y2))) return rc;
+    if (m < 2) {
+      if ((rc = dual(a, b, dp(gc.tanA[m]), nullptr, dp(gc.tc[m]), nullptr, Y1, nv, Y2, nullptr, Gm))) return rc;
+      for (int mm = m + 1; mm < 3; ++mm) {
+        if ((rc = flips(mm, Gm))) return rc;
+        if ((rc = rotation(mm, Gm, nv, Gm, mm == 2))) return rc;
+      }
+    } else {
+      // the dim-2 translation joins the last product-rule pass (k-space along dim 2)
+      if ((rc = dual(a, b, dp(gc.tanA[m]), dp(gc.tB[1]), dp(gc.tc[m]), nullptr, Y1, nv, Y2, nullptr, Gm))) return rc;
+    }
+    if ((rc = translate_rest(Gm))) return rc;
+  }
+  return DIS_OK;
+}
+
+int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char* cT, double winit,
+                const double tL[2], const float2* x, const float2* y, double* En_out, int* n_trials,
+                double* sums_only);
+
+int run_lm(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char* cT, double winit, const double tL[2],
+           const float2* x, const float2* y, double* En_out, int* n_trials) {
+  return run_lm_impl(c, v, T, w, cT, winit, tL, x, y, En_out, n_trials, nullptr);
+}
+
+int lm_partials(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* y, double* sums) {
+  std::vector<double> Tc(T, T + 6 * (size_t)v.g.nmotion), w(v.g.nmotion, 1.0), En(v.g.nmotion);
+  std::vector<unsigned char> cT(v.g.nmotion, 0);
+  const double tL[2] = {0, 0};
+  return run_lm_impl(c, v, Tc.data(), w.data(), cT.data(), 1.0, tL, x, y, En.data(), nullptr, sums);
+}
+
+// One call of LMsolver for the motion states of the plan.  With sums_only,
+// stops after the partial derivatives and returns the raw per-state sums.
+int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char* cT, double winit,
+                const double tL[2], const float2* x, const float2* y, double* En_out, int* n_trials,
+                double* sums_only) {
+  const dis_geom& g = v.g;
+  Grid gr;
+  int rc = make_grid(c, g.n, g.Nfull, &gr);
+  if (rc) return rc;
+  hipStream_t stream = ctx_stream(c);
+  const int NM = g.nmotion;
+  const long long nv = gr.nvox;
+  const long long ny = (long long)g.nsamples * g.n[2] * g.ncoils;
+  const double NElY = (double)ny;
+  std::vector<int> est;
+  for (int s = 0; s < NM; ++s)
+    if (!cT[s]) est.push_back(s);
+  std::vector<double> EnPrev(NM, 0.0), En(NM, 0.0), sums((size_t)NM * kLmSums, 0.0);
+  if (n_trials) *n_trials = 0;
+  if (est.empty()) {
+    for (int s = 0; s < NM; ++s) En_out[s] = 0.0;
+    return DIS_OK;
+  }
+  // y-space buffers (slot 2): Yr, J[6]; partials; per-sample energies
+  const int nb = lm_partial_count();
+  const size_t ybytes = sizeof(float2) * ny;
+  const size_t pbytes = sizeof(double) * (size_t)NM * nb * kLmSums;
+  const size_t sbytes = sizeof(double) * (size_t)NM * kLmSums;
+  const size_t ebytes = sizeof(float) * (size_t)std::max(1, g.nsamples) * g.n[2];
+  const size_t esbytes = sizeof(double) * (size_t)std::max(1, g.nsamples);
+  const size_t stepbytes = sizeof(double) * (size_t)NM * 8;
+  void* yb = nullptr;
+  rc = ctx_buffer(c, 2, 7 * ybytes + pbytes + sbytes + ebytes + esbytes + stepbytes + 4096, &yb);
+  if (rc) return rc;
+  float2* Yr = (float2*)yb;
+  float2* J[6];
+  for (int gi = 0; gi < 6; ++gi) J[gi] = Yr + (gi + 1) * ny;
+  double* partial = (double*)((char*)yb + 7 * ybytes);
+  double* dsums = (double*)((char*)partial + pbytes);
+  float* epart = (float*)((char*)dsums + sbytes);
+  double* esamp = (double*)((char*)epart + ebytes);
+  double* dstep = (double*)((char*)esamp + esbytes);
+  // volume workspace (slot 0): per batch state Tx, G[6], X0[3], Y[2] = 12 volumes
+  const size_t vol = sizeof(float2) * nv;
+  int Sb = (int)est.size();
+  const size_t lim = ctx_workspace_limit(c);
+  if (lim) {
+    if (lim < 12 * vol) return ctx_fail(c, DIS_E_NOMEM, "workspace limit below the 12 volumes of one LM state");
+    Sb = (int)std::min<size_t>((size_t)Sb, lim / (12 * vol));
+  }
+  void* ws = nullptr;
+  rc = ctx_workspace(c, 12 * vol * (size_t)Sb, &ws);
+  if (rc) return rc;
+  float2* Tx = (float2*)ws;
+  float2* G = Tx + nv * Sb;
+  float2* X0 = G + 6 * nv * Sb;
+  float2* Yv = X0 + 3 * nv * Sb;
+  const int G16 = std::max(1, std::min(g.n[1], kMaxTile / g.n[0]));
+  auto coil_args = [&](const int* dlist, const float2* U, int mode) {
+    CoilArgs a{};
+    a.U = U;
+    a.us = nv;
+    a.us_rel = 1;
+    a.V = nullptr;
+    a.Smaps = v.S;
+    a.n1 = g.n[0];
+    a.n2 = g.n[1];
+    a.n3 = g.n[2];
+    a.C = g.ncoils;
+    a.k1 = v.k1;
+    a.k2 = v.k2;
+    a.off = v.off;
+    a.We = nullptr;  // LMsolver encodes without EH.We (LMsolver.m:72,99)
+    a.rowptr = v.rowptr;
+    a.csr = v.csr;
+    a.Ns = g.nsamples;
+    a.yin = y;
+    a.yout = nullptr;
+    a.epart = epart;
+    a.invN1 = 1.0 / (double)g.Nfull[0];
+    a.tw1 = gr.tw[0];
+    a.tw2 = gr.tw[1];
+    a.plan1 = gr.plan[0];
+    a.G = G16;
+    a.slist = dlist;
+    (void)mode;
+    return a;
+  };
+  // ---- partial derivatives of the estimated states (LMsolver.m:52-122)
+  for (size_t b0 = 0; b0 < est.size(); b0 += Sb) {
+    const int sb = (int)std::min<size_t>(Sb, est.size() - b0);
+    std::vector<double> Tb((size_t)sb * 6);
+    std::vector<int> lst(sb);
+    for (int i = 0; i < sb; ++i) {
+      lst[i] = est[b0 + i];
+      for (int gi = 0; gi < 6; ++gi) Tb[i + (size_t)sb * gi] = T[lst[i] + (size_t)NM * gi];
+    }
+    rc = rigid_gradient(c, gr, Tb.data(), sb, sb, x, Tx, G, X0, Yv);
+    if (rc) return rc;
+    Staging st;
+    const size_t ol = st.add(lst.data(), sizeof(int) * sb);
+    const unsigned char* d = nullptr;
+    rc = ctx_upload_bytes(c, st, &d);
+    if (rc) return rc;
+    const int* dlist = (const int*)(d + ol);
+    CoilArgs a = coil_args(dlist, Tx, COIL_ENCODE);
+    a.yout = Yr;
+    DCHK(c, launch_coil(a, COIL_ENCODE, sb, stream));
+    for (int gi = 0; gi < 6; ++gi) {
+      CoilArgs ag = coil_args(dlist, G + (long long)gi * nv * sb, COIL_ENCODE);
+      ag.yout = J[gi];
+      DCHK(c, launch_coil(ag, COIL_ENCODE, sb, stream));
+    }
+    LmReduceArgs ra{};
+    ra.Yr = Yr;
+    ra.y = y;
+    for (int gi = 0; gi < 6; ++gi) ra.J[gi] = J[gi];
+    ra.off = v.off;
+    ra.slist = dlist;
+    ra.Ns = g.nsamples;
+    ra.n3 = g.n[2];
+    ra.C = g.ncoils;
+    ra.partial = partial;
+    DCHK(c, launch_lm_reduce(ra, sb, partial, dsums, stream));
+    std::vector<double> h((size_t)sb * kLmSums);
+    DCHK(c, hipMemcpyAsync(h.data(), dsums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+    DCHK(c, hipStreamSynchronize(stream));
+    for (int i = 0; i < sb; ++i)
+      for (int q = 0; q < kLmSums; ++q) sums[(size_t)lst[i] * kLmSums + q] = h[(size_t)i * kLmSums + q];
+  }
+  if (sums_only) {
+    std::copy(sums.begin(), sums.end(), sums_only);
+    return DIS_OK;
+  }
+  for (int s : est) EnPrev[s] = sums[(size_t)s * kLmSums] / NElY;
+  En = EnPrev;
+  // ---- damped steps, trial energies, accept / reject (LMsolver.m:126-215)
+  std::vector<double> dGEff((size_t)NM * 6, 0.0);
+  std::vector<unsigned char> flagw(NM, 0);
+  for (int s = 0; s < NM; ++s)
+    if (cT[s]) flagw[s] = 1;  // En = EnPrev = 0 for converged states
+  std::vector<double> Tup((size_t)NM * 6);
+  for (int iter = 0; iter < 10000; ++iter) {
+    // solves for the non-converged states with w < 1e10 (FP64, on the device)
+    std::vector<int> sol;
+    for (int s : est)
+      if (w[s] < 1e10) sol.push_back(s);
+    if (!sol.empty()) {
+      const int ns = (int)sol.size();
+      std::vector<double> hs((size_t)ns * kLmSums), hw(ns);
+      for (int i = 0; i < ns; ++i) {
+        for (int q = 0; q < kLmSums; ++q) hs[(size_t)i * kLmSums + q] = sums[(size_t)sol[i] * kLmSums + q];
+        hw[i] = w[sol[i]];
+      }
+      Staging st;
+      const size_t os = st.add(hs.data(), sizeof(double) * hs.size());
+      const size_t ow = st.add(hw.data(), sizeof(double) * ns);
+      const unsigned char* d = nullptr;
+      rc = ctx_upload_bytes(c, st, &d);
+      if (rc) return rc;
+      DCHK(c, launch_lm_solve((const double*)(d + os), (const double*)(d + ow), ns, winit, dstep, stream));
+      std::vector<double> hstep((size_t)ns * 6);
+      DCHK(c, hipMemcpyAsync(hstep.data(), dstep, sizeof(double) * hstep.size(), hipMemcpyDeviceToHost, stream));
+      DCHK(c, hipStreamSynchronize(stream));
+      for (int i = 0; i < ns; ++i)
+        for (int gi = 0; gi < 6; ++gi) dGEff[(size_t)sol[i] * 6 + gi] = hstep[(size_t)i * 6 + gi];
+    }
+    // Tup = restrictTransform(T + dGEff) (rotations into [-pi, pi])
+    for (int s = 0; s < NM; ++s)
+      for (int gi = 0; gi < 6; ++gi) {
+        double vv = T[s + (size_t)NM * gi] + dGEff[(size_t)s * 6 + gi];
+        if (gi >= 3) {
+          while (vv > M_PI) vv -= 2 * M_PI;
+          while (vv < -M_PI) vv += 2 * M_PI;
+        }
+        Tup[s + (size_t)NM * gi] = vv;
+      }
+    // trial energies of the states neither converged nor accepted
+    std::vector<int> todo;
+    for (int s : est)
+      if (!flagw[s]) todo.push_back(s);
+    if (n_trials) *n_trials += (int)todo.size();
+    for (size_t b0 = 0; b0 < todo.size(); b0 += Sb) {
+      const int sb = (int)std::min<size_t>(Sb, todo.size() - b0);
+      std::vector<double> Tb((size_t)sb * 6);
+      std::vector<int> lst(sb);
+      for (int i = 0; i < sb; ++i) {
+        lst[i] = todo[b0 + i];
+        for (int gi = 0; gi < 6; ++gi) Tb[i + (size_t)sb * gi] = Tup[lst[i] + (size_t)NM * gi];
+      }
+      rc = rigid_forward(c, gr, Tb.data(), sb, sb, x, 0, Tx, false);
+      if (rc) return rc;
+      Staging st;
+      const size_t ol = st.add(lst.data(), sizeof(int) * sb);
+      const unsigned char* d = nullptr;
+      rc = ctx_upload_bytes(c, st, &d);
+      if (rc) return rc;
+      CoilArgs a = coil_args((const int*)(d + ol), Tx, COIL_ENERGY);
+      DCHK(c, launch_coil(a, COIL_ENERGY, sb, stream));
+      DCHK(c, launch_energy_reduce(epart, g.nsamples, g.n[2], nullptr, esamp, stream));
+      std::vector<double> he(std::max(1, g.nsamples));
+      DCHK(c, hipMemcpyAsync(he.data(), esamp, sizeof(double) * g.nsamples, hipMemcpyDeviceToHost, stream));
+      DCHK(c, hipStreamSynchronize(stream));
+      for (int s : lst) {
+        double e = 0;
+        for (int j = v.hoff[s]; j < v.hoff[s + 1]; ++j) e += he[j];
+        En[s] = e / NElY;
+      }
+    }
+    bool all = true;
+    for (int s = 0; s < NM; ++s) {
+      if (w[s] > 1e10) En[s] = EnPrev[s];
+      if (En[s] <= EnPrev[s]) flagw[s] = 1;
+      if (!flagw[s]) all = false;
+    }
+    if (!all) {
+      for (int s = 0; s < NM; ++s)
+        if (En[s] > EnPrev[s] && !cT[s]) w[s] *= 2.0;
+      continue;
+    }
+    for (int s = 0; s < NM; ++s) {
+      if (!cT[s]) w[s] /= 1.2;
+      if (w[s] < 1e-8) w[s] *= 1.2;
+    }
+    for (size_t k = 0; k < Tup.size(); ++k) T[k] = Tup[k];
+    for (int s = 0; s < NM; ++s) {
+      double tra = 0, rot = 0;
+      for (int gi = 0; gi < 3; ++gi) tra = std::max(tra, std::fabs(dGEff[(size_t)s * 6 + gi]));
+      for (int gi = 3; gi < 6; ++gi) rot = std::max(rot, std::fabs(dGEff[(size_t)s * 6 + gi] * 180.0 / M_PI));
+      if (tra < tL[0] && rot < tL[1]) cT[s] = 1;
+    }
+    break;
+  }
+  for (int s = 0; s < NM; ++s) En_out[s] = std::min(EnPrev[s], En[s]);
   return DIS_OK;
 }
 

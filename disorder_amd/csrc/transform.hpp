@@ -83,6 +83,18 @@ int rigid_inverse(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, c
 int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* yin, float2* yout,
              float2* xout, int mode, EnergyOut* eo);
 
+// Tx[s] = T_s x and G[g][s] = d(T_s x)/dT_g (g < 6) for states [0,S) of a
+// [Sld x 6] motion array.  Buffers: Tx [S], G [6 S], X0 [3 S], Y [2 S] volumes.
+int rigid_gradient(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, const float2* x, float2* Tx,
+                   float2* G, float2* X0, float2* Y);
+// One LMsolver call over the motion states (T [NM x 6] column-major, w, cT in/out).
+int run_lm(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char* cT, double winit, const double tL[2],
+           const float2* x, const float2* y, double* En_out, int* n_trials);
+
+// Raw per-state LM sums [NM x 28]: sum |r|^2, sum Re(J_g conj r) (6),
+// sum Re(J_i conj J_j) over the 21 pairs of LMsolver.m:26.
+int lm_partials(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* y, double* sums);
+
 int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const float* Se, const float* Ma,
            double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn);
 

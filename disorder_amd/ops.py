@@ -152,6 +152,22 @@ def sincRigidTransform(x: torch.Tensor, T, di: int = 1, sumR: bool = True, Nfull
     return out
 
 
+def sincRigidTransformGradient(x: torch.Tensor, T, Nfull=None, ctx: Context = None):
+    """Methods/sincRigidTransformGradient.m:1 with the factors of
+    precomputeFactorsSincRigidTransform(kG, rkG, T, 1, 1, 1, 1): returns
+    (T x as (n1,n2,n3,S), dT x/dT_g as (n1,n2,n3,S,6))."""
+    ctx = ctx or default_context(x.device.index or 0)
+    Tf, Tp = _tbuf(T)
+    S = Tf.size // 6
+    n = (ctypes.c_int32 * 3)(*x.shape[:3])
+    N = (ctypes.c_int32 * 3)(*(Nfull if Nfull is not None else x.shape[:3]))
+    Tx = empty_colmajor(tuple(x.shape[:3]) + (S,), x.device)
+    G = empty_colmajor(tuple(x.shape[:3]) + (S, 6), x.device)
+    ctx._prep()
+    check(ctx.lib.dis_rigid_transform_grad(ctx.h, n, N, Tp, S, _ptr(x), _ptr(Tx), _ptr(G)), ctx.h)
+    return Tx, G
+
+
 class Encoding:
     """The E / EH structures of the reconstruction path as one plan
     (dis_plan): coil maps Sf, per-segment sampling (E.Fs rows as FFT-order
@@ -273,6 +289,35 @@ def CGsolver(y, E: "Encoding", T, x=None, nIt=300, tolType="Energy", tol=1e-5, S
                            tt, float(tol), En.ctypes.data_as(_lib.c_double_p), ctypes.byref(nEn), ctypes.byref(nn)),
           E.ctx.h)
     return xo, En[:2 * nEn.value].reshape(-1, 2).T.copy(), nn.value
+
+
+def lm_partials(y, E: "Encoding", x, T):
+    """Reconstruction/LMsolver.m:52-122: per motion state the residual energy
+    EnPrev (divided by numel(y)), gradient dG (S,6) and Hessian entries dH
+    (S,21, pairs ha of LMsolver.m:26)."""
+    Tf, Tp = E._T(T)
+    sums = np.zeros((E.nmotion, 28), np.float64)
+    E.ctx._prep()
+    check(E.ctx.lib.dis_lm_partials(E.h, Tp, _ptr(x), _ptr(y), sums.ctypes.data_as(_lib.c_double_p)), E.ctx.h)
+    nel = E.Ns * E.n[2] * E.C
+    return sums[:, 0] / nel, sums[:, 1:7].copy(), sums[:, 7:].copy()
+
+
+def LMsolver(y, E: "Encoding", x, T, w, cT, winit=1e-3, tL=(0.05, 0.02)):
+    """Reconstruction/LMsolver.m:1 -- one Levenberg-Marquardt update of the
+    motion states.  Returns (T, w, cT, En, n_trials)."""
+    Tf, Tp = E._T(T)
+    wv = np.ascontiguousarray(w, dtype=np.float64).copy()
+    cv = np.ascontiguousarray(cT, dtype=np.uint8).copy()
+    tl = np.ascontiguousarray(tL, dtype=np.float64)
+    En = np.zeros(E.nmotion, np.float64)
+    nt = ctypes.c_int32(0)
+    E.ctx._prep()
+    check(E.ctx.lib.dis_lm(E.h, Tp, wv.ctypes.data_as(_lib.c_double_p), cv.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                           float(winit), tl.ctypes.data_as(_lib.c_double_p), _ptr(x), _ptr(y),
+                           En.ctypes.data_as(_lib.c_double_p), ctypes.byref(nt)), E.ctx.h)
+    Tn = Tf.reshape(6, E.nmotion).T.copy()
+    return Tn, wv, cv.astype(bool), En, nt.value
 
 
 def encode(x, E: Encoding, T):

@@ -123,6 +123,33 @@ int dis_energy(dis_plan* plan, const double* T, const void* x, const void* y, co
 int dis_cg(dis_plan* plan, const double* T, const void* y, const float* Se, const float* Ma, double lambda, void* x,
            int32_t x_init, int32_t nIt, int32_t tolType, double tol, double* En, int32_t* nEn, int32_t* nn);
 
+/* sincRigidTransformGradient (Methods/sincRigidTransformGradient.m:1) with
+ * the factors of precomputeFactorsSincRigidTransform(kG,rkG,T,1,1,1,1)
+ * (Reconstruction/LMsolver.m:60-66,96): x is one volume [n]; Tx (or NULL)
+ * receives T_s x [n,S]; G receives dT_s x / dT_g as [n,S,6] (g fastest
+ * after the state: G[:, s, g]).  Rotations beyond 90 degrees flip the
+ * volume as the forward transform does; the flip itself is not
+ * differentiated (as in the reference). */
+int dis_rigid_transform_grad(dis_ctx* ctx, const int32_t n[3], const int32_t Nfull[3], const double* T, int32_t S,
+                             const void* x, void* Tx, void* G);
+
+/* Per-state partial sums of LMsolver (Reconstruction/LMsolver.m:52-122) for
+ * the motion states of the plan: sums [nmotion x 28] (host, FP64, row per
+ * state): sum_j |E_s T_s x - y|^2, sum Re(J_g conj r) (g = 1..6), and
+ * sum Re(J_i conj J_j) over ha = [diag(1..6), nchoosek(1:6,2)] (:26), with
+ * J_g = E_s dT_s x / dT_g; sums over the state's samples, readout and coils. */
+int dis_lm_partials(dis_plan* plan, const double* T, const void* x, const void* y, double* sums);
+
+/* One LMsolver call (replaces Reconstruction/LMsolver.m:1) on the motion
+ * states: T [nmotion x 6] (host, in/out), w [nmotion] damping (E.w, in/out),
+ * cT [nmotion] convergence flags (E.cT, in/out), winit (E.winit), tL[2]
+ * (E.tL: translation voxels, rotation degrees).  Damped 6x6 normal equations
+ * are solved in FP64 on the device.  En [nmotion] receives min(EnPrev, En)
+ * (E.En), n_trials (or NULL) the number of trial-state evaluations.  EH.We is
+ * not applied (as in the reference's LM encodes). */
+int dis_lm(dis_plan* plan, double* T, double* w, uint8_t* cT, double winit, const double* tL, const void* x,
+           const void* y, double* En, int32_t* n_trials);
+
 /* Kernel classes for dis_profile_read. */
 #define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */
 #define DIS_K_BCAST 1  /* first forward pass, one input volume to all states */

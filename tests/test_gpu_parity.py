@@ -207,3 +207,54 @@ def test_cg_matches_oracle(dev, tolType, nIt, lam, use_pre):
                            lam=lam)
     xr2, _, _ = O.cg_solver(y, Sm, samp, T, Nf, x0=xr, nIt=2, tolType="None", Se=Se, Ma=Ma, lam=(lam or None))
     assert rel_l2(to_np(xg2), xr2) < 1e-4
+
+
+@pytest.mark.parametrize("n,Nfull,flip", [((16, 12, 10), None, False), ((32, 64, 16), (64, 128, 32), False),
+                                          ((15, 12, 14), None, True), ((32, 16, 12), None, True)])
+def test_gradient_matches_oracle(dev, n, Nfull, flip):
+    """sincRigidTransformGradient: every d(T x)/dT_g against the FP64 oracle."""
+    import disorder_amd as D
+    rng = np.random.default_rng(31)
+    Nf = Nfull or n
+    x = smooth_volume(rng, n) + 0.05 * crandn(rng, n)
+    T = random_motion(rng, 3, 4, 1.5)
+    if flip:
+        T[0, 3] = 2.1
+        T[2, 5] = -2.4
+    Tx, G = D.sincRigidTransformGradient(to_dev(x, dev), T, Nfull=Nf)
+    Tx, G = to_np(Tx), to_np(G)
+    for s in range(3):
+        f = O.Factors(T[s:s + 1], n, Nf, 1, cg=True)
+        xt, xB = O.sinc_rigid_transform(x, f, 1, keep=True)
+        Gr = O.sinc_rigid_transform_gradient(xB, f)
+        assert rel_l2(Tx[..., s], xt[..., 0]) < TOL
+        for g in range(6):
+            assert rel_l2(G[..., s, g], Gr[g][..., 0]) < 2e-5, (s, g)
+
+
+@pytest.mark.parametrize("n,C,S", [((16, 12, 8), 3, 4), ((32, 24, 10), 4, 4)])
+def test_lm_partials_and_step_match_oracle(dev, n, C, S):
+    """LMsolver: per-state energy, gradient and Hessian sums, then one full
+    accept/reject update, against the FP64 oracle."""
+    import disorder_amd as D
+    rng = np.random.default_rng(41)
+    x, Sm, samp, _, _, Nf = _problem(rng, n, C, S)
+    Tgt = random_motion(rng, S, 3, 1)
+    y = O.encode(x, Sm, samp, Tgt, Nf) + 0.01 * crandn(rng, (samp.Ns, n[2], C))
+    T0 = Tgt + random_motion(rng, S, 1.0, 0.3)
+    E = D.Encoding(to_dev(Sm, dev), samp.k1, samp.k2, samp.off, Nfull=Nf)
+    xd, yd = to_dev(x, dev), to_dev(y, dev)
+    En, dG, dH = D.lm_partials(yd, E, xd, T0)
+    Er, dGr, dHr = O.lm_partials(y, x, Sm, samp, T0, Nf)
+    assert rel_l2(En, Er) < 1e-5
+    assert rel_l2(dG, dGr) < 1e-4
+    assert rel_l2(dH, dHr) < 1e-4
+    w = np.full(S, 1e-3)
+    cT = np.zeros(S, bool)
+    cT[1] = True
+    Tg, wg, cg, Eg, _ = D.LMsolver(yd, E, xd, T0, w, cT, winit=1e-3, tL=(0.05, 0.02))
+    Tr, wr, cr, Ern = O.lm_solver(y, x, Sm, samp, T0, Nf, w, cT, 1e-3, (0.05, 0.02))
+    assert np.allclose(wg, wr) and np.array_equal(cg, cr)
+    assert np.abs(Tg - Tr).max() < 1e-5
+    assert np.allclose(Eg, Ern, rtol=1e-5, atol=1e-12)
+    assert np.all(Tg[1] == T0[1])  # converged states are untouched
