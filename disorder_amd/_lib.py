@@ -59,12 +59,18 @@ SIGNATURES = {
     "dis_lm_partials": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p, c_double_p]),
     "dis_lm": (c_int, [c_void_p, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_uint8), ctypes.c_double, c_double_p,
                        c_void_p, c_void_p, c_double_p, c_int32_p]),
+    "dis_coil_gram": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_double_p]),
+    "dis_coil_basis": (c_int, [c_double_p, c_int32, c_double_p, c_int32, c_int32_p, c_double_p, c_double_p]),
+    "dis_coil_apply": (c_int, [c_void_p, c_double_p, c_int32, c_int32, c_void_p, ctypes.c_int64, c_void_p, c_int32]),
+    "dis_compress_coils": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_double_p, c_int32, c_void_p,
+                                   ctypes.c_int64, c_void_p, c_void_p, c_int32, c_int32_p, c_double_p, c_double_p]),
     "dis_profile_enable": (c_int, [c_void_p, c_int32]),
     "dis_profile_read": (c_int, [c_void_p, c_int32, ctypes.POINTER(ctypes.c_int64), c_double_p, c_double_p]),
 }
 
-K_PASS, K_BCAST, K_ACCUM, K_COIL, K_FLIP = range(5)
-K_NAMES = {K_PASS: "pass", K_BCAST: "bcast_pass", K_ACCUM: "accum_pass", K_COIL: "coil_stage", K_FLIP: "flip_pass"}
+K_PASS, K_BCAST, K_ACCUM, K_COIL, K_FLIP, K_GRAM, K_GEMM = range(7)
+K_NAMES = {K_PASS: "pass", K_BCAST: "bcast_pass", K_ACCUM: "accum_pass", K_COIL: "coil_stage", K_FLIP: "flip_pass",
+           K_GRAM: "coil_gram", K_GEMM: "coil_apply"}
 
 _lib = None
 

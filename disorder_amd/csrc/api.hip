@@ -74,6 +74,8 @@ int ctx_fail(dis_ctx* c, int code, const char* msg) { return fail(c, code, msg);
 
 hipStream_t ctx_stream(dis_ctx* c) { return c->stream; }
 
+hipError_t ctx_set_device(dis_ctx* c) { return hipSetDevice(c->device); }
+
 int ctx_twiddles(dis_ctx* c, int n, const float2** out) {
   auto it = c->tw.find(n);
   if (it != c->tw.end()) {

@@ -53,6 +53,7 @@ struct Staging {
 // context helpers implemented in api.hip
 int ctx_fail(dis_ctx* c, int code, const char* msg);
 hipStream_t ctx_stream(dis_ctx* c);
+hipError_t ctx_set_device(dis_ctx* c);
 int ctx_twiddles(dis_ctx* c, int n, const float2** out);
 int ctx_workspace(dis_ctx* c, size_t bytes, void** out);
 size_t ctx_workspace_limit(dis_ctx* c);
@@ -94,6 +95,13 @@ int run_lm(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char* cT,
 // Raw per-state LM sums [NM x 28]: sum |r|^2, sum Re(J_g conj r) (6),
 // sum Re(J_i conj J_j) over the 21 pairs of LMsolver.m:26.
 int lm_partials(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* y, double* sums);
+
+// Coil compression: Gram matrix P (device, complex128 column-major C x C)
+// and the MFMA apply out(v,k) = sum_c A(k,c) in(v,c) with the FP32 real
+// operand Bm [2C][2Cc] on the device.
+int coil_gram(dis_ctx* c, const float2* S, long long nv, int C, double2* P_dev_out);
+int coil_apply(dis_ctx* c, const float* Bm_dev, int Cc, int C, const float2* in, long long nv, long long in_cs,
+               float2* out, long long out_cs, int terms);
 
 int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const float* Se, const float* Ma,
            double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn);

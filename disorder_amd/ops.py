@@ -338,3 +338,74 @@ def encodedecode(x, E: Encoding, T):
 def computeEnergy(y, x, E: Encoding, T, Mbe=None, per_sample=False):
     """Reconstruction/computeEnergy.m:34-57 (no=2, FP64 accumulation)."""
     return E.energy(x, y, T, Mbe=Mbe, per_sample=per_sample)
+
+
+# ---------------------------------------------------------------- coil compression
+def _coil_view(t, name):
+    """[..., C] column-major complex64 device tensor -> (nv, C)."""
+    if t.dim() < 2:
+        raise ValueError(f"{name} must have a trailing coil dimension")
+    _ptr(t)
+    C = t.shape[-1]
+    return t.numel() // max(C, 1), C
+
+
+def coil_gram(S, ctx: Context = None):
+    """Methods/compressCoils.m:78-87: P = sum_vox Shat S^H with
+    Shat = S / (sum_c |S_c|^2 + 1e-9); complex128 numpy [C x C]."""
+    ctx = ctx or default_context(S.device.index or 0)
+    nv, C = _coil_view(S, "S")
+    P = np.zeros((C, C), np.complex128, order="F")
+    ctx._prep()
+    check(ctx.lib.dis_coil_gram(ctx.h, _ptr(S), nv, C, P.ctypes.data_as(_lib.c_double_p)), ctx.h)
+    return P
+
+
+def coil_basis(P, perc):
+    """Methods/compressCoils.m:93-111: (U, D, C) from svd(P) on the host in FP64."""
+    P = np.asfortranarray(P, dtype=np.complex128)
+    C = P.shape[0]
+    pv = np.atleast_1d(np.asarray(perc, np.float64))
+    cc = np.zeros(len(pv), np.int32)
+    D = np.zeros(C, np.float64)
+    U = np.zeros((C, C), np.complex128, order="F")
+    lib = _lib.load()
+    check(lib.dis_coil_basis(P.ctypes.data_as(_lib.c_double_p), C, pv.ctypes.data_as(_lib.c_double_p), len(pv),
+                             cc.ctypes.data_as(_lib.c_int32_p), D.ctypes.data_as(_lib.c_double_p),
+                             U.ctypes.data_as(_lib.c_double_p)))
+    return U, D, cc
+
+
+def coil_apply(A, x, terms=3, ctx: Context = None):
+    """aplGPU(A, x, 4) of Methods/compressCoils.m:117-118: contracts the
+    trailing coil dimension of x with A [Cc x C] on the matrix cores."""
+    ctx = ctx or default_context(x.device.index or 0)
+    nv, C = _coil_view(x, "x")
+    A = np.asfortranarray(A, dtype=np.complex128)
+    if A.shape[1] != C:
+        raise ValueError("A must be [Cc x C]")
+    Cc = A.shape[0]
+    out = empty_colmajor(tuple(x.shape[:-1]) + (Cc,), x.device)
+    ctx._prep()
+    check(ctx.lib.dis_coil_apply(ctx.h, A.ctypes.data_as(_lib.c_double_p), Cc, C, _ptr(x), nv, _ptr(out),
+                                 int(terms)), ctx.h)
+    return out
+
+
+def compressCoils(S, perc, y=None, terms=3, ctx: Context = None):
+    """Methods/compressCoils.m:1 (useFold=0, useInvMot=0):
+    [S, y, C, D, U] = compressCoils(S, perc, y).  S [n1,n2,n3,C] and y [...,C]
+    are column-major complex64 device tensors; perc is an energy fraction
+    (< 1) or a component count (>= 1), or a list of them.  The compression
+    matrix is A = U(:, 1:max(C))^H.  ``terms`` picks the split-bf16 product
+    count of the MFMA contraction (3 or 6)."""
+    ctx = ctx or default_context(S.device.index or 0)
+    if perc is None:
+        return S, y, np.array([S.shape[-1]]), None, None
+    P = coil_gram(S, ctx)
+    U, D, cc = coil_basis(P, perc)
+    Cc = int(cc.max())
+    A = np.conj(U[:, :Cc]).T
+    Sc = coil_apply(A, S, terms, ctx)
+    yc = None if y is None else coil_apply(A, y, terms, ctx)
+    return Sc, yc, cc, D, U

@@ -150,13 +150,48 @@ int dis_lm_partials(dis_plan* plan, const double* T, const void* x, const void* 
 int dis_lm(dis_plan* plan, double* T, double* w, uint8_t* cT, double winit, const double* tL, const void* x,
            const void* y, double* En, int32_t* n_trials);
 
+/* ---- Coil-array compression (replaces Methods/compressCoils.m:1 with
+ * useFold = 0, useInvMot = 0; the unfolded Gram branch :74-90, taken when the
+ * coil maps and the data share their spatial size as in
+ * Reconstruction/solveXT.m:61).  Arrays are [nv x C] complex64 on the device
+ * with coil stride nv (the coil dimension last, as S [n1,n2,n3,C] and y). */
+
+/* Gram matrix P(i,j) = sum_v Shat_i(v) conj(S_j(v)), Shat = S/(sum_c|S_c|^2 +
+ * 1e-9) (compressCoils.m:78-87).  P: host complex128 (re, im pairs),
+ * column-major C x C.  FP32 products, FP64 sums.  C <= 64. */
+int dis_coil_gram(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, double* P);
+
+/* [U,D] = svd(P) and the component counts (compressCoils.m:93-111) on the
+ * host in FP64.  P is Hermitian PSD, so U holds its eigenvectors (column
+ * phases arbitrary, as for any SVD) and D [C] the |eigenvalues| descending.
+ * Ccount[n] = first k with cumsum(D)/sum(D) >= perc[n] when perc[n] < 1, else
+ * perc[n].  U: host complex128 column-major C x C (may be NULL); D may be NULL. */
+int dis_coil_basis(const double* P, int32_t C, const double* perc, int32_t nperc, int32_t* Ccount, double* D,
+                   double* U);
+
+/* out(v,k) = sum_c A(k,c) in(v,c) (aplGPU(A,S,4), compressCoils.m:117-118):
+ * A host complex128 column-major Cc x C; in [nv x C], out [nv x Cc] on the
+ * device.  BF16 MFMA with split operands accumulated in FP32: terms = 3
+ * (hi/lo split, three products) or 6 (three-way split, six products). */
+int dis_coil_apply(dis_ctx* ctx, const double* A, int32_t Cc, int32_t C, const void* in, int64_t nv, void* out,
+                   int32_t terms);
+
+/* The whole compressCoils (compressCoils.m:75-118): Gram, basis, then
+ * S_out = A S and y_out = A y with A = U(:,1:max(Ccount))^H.  S_out and y_out
+ * hold max(Ccount) <= C coil planes of nvS / nvy elements.  y may be NULL. */
+int dis_compress_coils(dis_ctx* ctx, const void* S, int64_t nvS, int32_t C, const double* perc, int32_t nperc,
+                       const void* y, int64_t nvy, void* S_out, void* y_out, int32_t terms, int32_t* Ccount,
+                       double* D, double* U);
+
 /* Kernel classes for dis_profile_read. */
 #define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */
 #define DIS_K_BCAST 1  /* first forward pass, one input volume to all states */
 #define DIS_K_ACCUM 2  /* last inverse pass with the state sum */
 #define DIS_K_COIL 3   /* coil stage (coil multiply, dim-1 FFT, sampled dim-2 DFT, adjoint) */
 #define DIS_K_FLIP 4   /* >90 degree flip pass */
-#define DIS_K_COUNT 5
+#define DIS_K_GRAM 5   /* coil-compression Gram matrix */
+#define DIS_K_GEMM 6   /* coil-compression apply (MFMA) */
+#define DIS_K_COUNT 7
 
 /* Per-kernel-class timing with HIP events recorded around every launch on the
  * context's stream (no counterpart in the reference, which only prints
