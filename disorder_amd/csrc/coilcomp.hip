@@ -9,7 +9,7 @@
 //    computed by cyclic complex Jacobi in FP64.  Component counts by energy
 //    fraction or fixed count (:107-111).
 //  * apply_kernel: out(v, k) = sum_c A(k, c) in(v, c), A = U(:,1:Cc)^H
-//    (:115-118, aplGPU along dim 4), as a real GEMM [nv x 2C] . [2C x 2Cc] on
+//    (:115-118, aplGPU along dim 4), as a real GEMM [2Cc x 2C] . [2C x nv] on
 //    the matrix cores (v_mfma_f32_16x16x16_bf16).  FP32 operands are split
 //    into bf16 terms and the products accumulate in FP32:
 //      TERMS = 3: a = h + l, ab ~ hh + hl + lh          (~2^-16 relative)
@@ -173,76 +173,115 @@ __device__ __forceinline__ void split_terms(float a, short (&t)[3]) {
   }
 }
 
-// One workgroup handles 64 voxels: four waves, one 16-voxel MFMA row tile
-// each, looping over 16-wide column tiles of the real output.  Bm is the
-// FP32 real operand [2C][2Cc] (row-major): Bm[2c][2k] = Re A(k,c),
-// Bm[2c+1][2k] = -Im A(k,c), Bm[2c][2k+1] = Im A(k,c), Bm[2c+1][2k+1] = Re A(k,c).
-template <int TERMS>
+// D = Aop . Bop with Aop[m][k] = Bm[k][m] (the compression matrix, real
+// output column m of the current 16-column tile, real input index k) and
+// Bop[k][n] = in(voxel n, k) (16 voxels of a tile).  The constant operand
+// stays in registers for the whole kernel; each wave walks voxel tiles with
+// all sixteen loads of a tile in flight.  Operand layouts of
+// v_mfma_f32_16x16x16_bf16: lane l holds A[l&15][4(l>>4)+j], B[4(l>>4)+j][l&15]
+// and D[4(l>>4)+j][l&15], j < 4.  With k = 2c + {re, im}, lane (row, kq)
+// needs coils 8 ks + 2 kq + {0, 1} of voxel `row` (two float2 loads per
+// K step, 16 lanes reading 128 contiguous bytes per coil), and returns
+// output coils 8 mt + 2 kq + {0, 1} of voxel `row` as float2 pairs.
+// Cc > 8 runs one sweep over the voxels per 16-column tile.
+constexpr int kApplyMaxKs = kMaxCoils / 8;  // K steps of 16 real inputs
+
+// KS = ceil(C / 8) K steps.  Loads are branch-free: out-of-range voxels and
+// coils read a clamped in-range address and are zeroed by a select.
+template <int TERMS, int KS>
 __global__ __launch_bounds__(256) void apply_kernel(const float2* __restrict__ in, long long nv, long long in_cs,
                                                    int C, const float* __restrict__ Bm, int Cc,
                                                    float2* __restrict__ out, long long out_cs) {
-  __shared__ float2 tile[64][kMaxCoils + 1];  // [voxel][coil]
-  __shared__ float otile[64][17];             // [voxel][n within the column tile]
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6;
-  const long long vb = (long long)blockIdx.x * 64;
-  for (int e = tid; e < 64 * C; e += 256) {
-    const int vv = e & 63, cc = e >> 6;
-    const long long v = vb + vv;
-    tile[vv][cc] = v < nv ? in[v + in_cs * cc] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  const int K = 2 * C;
-  const int Nn = 2 * Cc;
-  const int row = lane & 15;  // A: voxel row of the tile; B and D: column n
-  const int kq = lane >> 4;   // 4-element K slice
-  for (int n0 = 0; n0 < Nn; n0 += 16) {
-    f32x4 d = {0.f, 0.f, 0.f, 0.f};
-    const int n = n0 + row;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-      bf16x4 a0, a1, a2, b0, b1, b2;
+  const int lane = threadIdx.x & 63;
+  const int row = lane & 15, kq = lane >> 4;
+  const long long ntiles = (nv + 15) >> 4;
+  const long long w0 = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
+  const int K = 2 * C, Nn = 2 * Cc;
+  for (int mt = 0; mt * 16 < Nn; ++mt) {
+    // constant operand: rows m = 16 mt + row, columns k = 16 ks + 4 kq + j
+    bf16x4 a0[KS], a1[KS], a2[KS];
+    const int m = 16 * mt + row;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int k = k0 + 4 * kq + j;
-        float av = 0.f, bv = 0.f;
-        if (k < K) {
-          const float2 s = tile[16 * wv + row][k >> 1];
-          av = (k & 1) ? s.y : s.x;
-          if (n < Nn) bv = Bm[(long long)k * Nn + n];
-        }
-        short ta[3], tb[3];
-        split_terms<TERMS>(av, ta);
-        split_terms<TERMS>(bv, tb);
-        a0[j] = ta[0];
-        a1[j] = ta[1];
-        a2[j] = ta[2];
-        b0[j] = tb[0];
-        b1[j] = tb[1];
-        b2[j] = tb[2];
+        const int k = 16 * ks + 4 * kq + j;
+        const float av = (k < K && m < Nn) ? Bm[(long long)k * Nn + m] : 0.f;
+        short t[3];
+        split_terms<TERMS>(av, t);
+        a0[ks][j] = t[0];
+        a1[ks][j] = t[1];
+        a2[ks][j] = t[2];
       }
-      // smallest terms first
-      if (TERMS == 6) {
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b2, d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b0, d, 0, 0, 0);
-      }
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, d, 0, 0, 0);
-      d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, d, 0, 0, 0);
-    }
-    // D layout: lane holds D[m = 4 kq + j][n = row]
+    const int oc0 = 8 * mt + 2 * kq;  // output coils of this lane
+    for (long long t = w0; t < ntiles; t += wstride) {
+      const long long v = t * 16 + row;
+      const bool vok = v < nv;
+      const float2* __restrict__ v0 = in + (vok ? v : nv - 1);  // coil 0
+      const float2* __restrict__ src = v0 + in_cs * (2 * kq);
+      float2 x[KS][2];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) otile[16 * wv + 4 * kq + j][row] = d[j];
-    __syncthreads();
-    const int kc0 = n0 >> 1;
-    const int kcn = Cc - kc0 < 8 ? Cc - kc0 : 8;
-    for (int e = tid; e < 64 * kcn; e += 256) {
-      const int vv = e & 63, kk = e >> 6;
-      const long long v = vb + vv;
-      if (v < nv) out[v + out_cs * (kc0 + kk)] = make_float2(otile[vv][2 * kk], otile[vv][2 * kk + 1]);
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const bool cok = 8 * ks + 2 * kq + h < C;
+          const float2 val = *(cok ? src + in_cs * (8 * ks + h) : v0);
+          x[ks][h] = (vok && cok) ? val : make_float2(0.f, 0.f);
+        }
+      f32x4 d = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const float xv[4] = {x[ks][0].x, x[ks][0].y, x[ks][1].x, x[ks][1].y};
+        bf16x4 b0, b1, b2;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          short tt[3];
+          split_terms<TERMS>(xv[j], tt);
+          b0[j] = tt[0];
+          b1[j] = tt[1];
+          b2[j] = tt[2];
+        }
+        // smallest terms first
+        if (TERMS == 6) {
+          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ks], b1, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0[ks], b2, d, 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ks], b0, d, 0, 0, 0);
+        }
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0[ks], b1, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ks], b0, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0[ks], b0, d, 0, 0, 0);
+      }
+      if (vok) {
+        if (oc0 < Cc) out[v + out_cs * oc0] = make_float2(d[0], d[1]);
+        if (oc0 + 1 < Cc) out[v + out_cs * (oc0 + 1)] = make_float2(d[2], d[3]);
+      }
     }
-    __syncthreads();
   }
+}
+
+template <int TERMS, int KS>
+void launch_apply_ks(int ks, dim3 grid, hipStream_t s, const float2* in, long long nv, long long in_cs, int C,
+                     const float* Bm, int Cc, float2* out, long long out_cs) {
+  if constexpr (KS > 1) {
+    if (ks < KS) {
+      launch_apply_ks<TERMS, KS - 1>(ks, grid, s, in, nv, in_cs, C, Bm, Cc, out, out_cs);
+      return;
+    }
+  }
+  // one resident wave per tile stream: blocks = CUs x resident blocks per CU
+  static int resident = 0;
+  if (!resident) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, apply_kernel<TERMS, KS>, 256, 0) == hipSuccess)
+      resident = std::max(1, ncu * per_cu);
+    else
+      resident = 1024;
+  }
+  const unsigned nb = (unsigned)std::min<long long>(grid.x, resident);
+  hipLaunchKernelGGL((apply_kernel<TERMS, KS>), dim3(nb), dim3(256), 0, s, in, nv, in_cs, C, Bm, Cc, out, out_cs);
 }
 
 // ------------------------------------------------------------------ host eig
@@ -320,15 +359,15 @@ int coil_gram(dis_ctx* c, const float2* S, long long nv, int C, double2* P_dev_o
 int coil_apply(dis_ctx* c, const float* Bm_dev, int Cc, int C, const float2* in, long long nv, long long in_cs,
                float2* out, long long out_cs, int terms) {
   if (nv <= 0) return DIS_OK;
-  const long long nb = (nv + 63) / 64;
+  const long long ntiles = (nv + 15) / 16;
+  const dim3 grid((unsigned)std::min<long long>((ntiles + 3) / 4, 1 << 20));
+  const int ks = (C + 7) / 8;
   hipStream_t s = ctx_stream(c);
   const int ps = prof_begin(c, DIS_K_GEMM);
   if (terms == 6)
-    hipLaunchKernelGGL(apply_kernel<6>, dim3((unsigned)nb), dim3(256), 0, s, in, nv, in_cs, C, Bm_dev, Cc, out,
-                       out_cs);
+    launch_apply_ks<6, kApplyMaxKs>(ks, grid, s, in, nv, in_cs, C, Bm_dev, Cc, out, out_cs);
   else
-    hipLaunchKernelGGL(apply_kernel<3>, dim3((unsigned)nb), dim3(256), 0, s, in, nv, in_cs, C, Bm_dev, Cc, out,
-                       out_cs);
+    launch_apply_ks<3, kApplyMaxKs>(ks, grid, s, in, nv, in_cs, C, Bm_dev, Cc, out, out_cs);
   prof_end(c, ps, 8.0 * (double)nv * (C + Cc));
   if (hipGetLastError() != hipSuccess) return ctx_fail(c, DIS_E_HIP, "coil apply launch failed");
   return DIS_OK;
