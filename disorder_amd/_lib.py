@@ -64,6 +64,10 @@ SIGNATURES = {
     "dis_coil_apply": (c_int, [c_void_p, c_double_p, c_int32, c_int32, c_void_p, ctypes.c_int64, c_void_p, c_int32]),
     "dis_compress_coils": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_double_p, c_int32, c_void_p,
                                    ctypes.c_int64, c_void_p, c_void_p, c_int32, c_int32_p, c_double_p, c_double_p]),
+    "dis_resample": (c_int, [c_void_p, c_int32_p, c_int32_p, ctypes.c_int64, c_void_p, c_void_p]),
+    "dis_gather_samples": (c_int, [c_void_p, c_void_p, c_int32, ctypes.c_int64, c_int32_p, c_int32, ctypes.c_float,
+                                   c_void_p]),
+    "dis_coil_precond": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, ctypes.c_float, c_float_p]),
     "dis_profile_enable": (c_int, [c_void_p, c_int32]),
     "dis_profile_read": (c_int, [c_void_p, c_int32, ctypes.POINTER(ctypes.c_int64), c_double_p, c_double_p]),
 }

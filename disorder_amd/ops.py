@@ -409,3 +409,45 @@ def compressCoils(S, perc, y=None, terms=3, ctx: Context = None):
     Sc = coil_apply(A, S, terms, ctx)
     yc = None if y is None else coil_apply(A, y, terms, ctx)
     return Sc, yc, cc, D, U
+
+
+# ---------------------------------------------------------------- pyramid levels
+def resampling(x, Nres, ctx: Context = None):
+    """Methods/resampling.m:1 (fo=0, mirror=0) on the GPU: x [n1,n2,n3] or
+    [n1,n2,n3,B] column-major complex64 -> [Nres..., B]."""
+    ctx = ctx or default_context(x.device.index or 0)
+    _ptr(x)
+    nin = tuple(int(v) for v in x.shape[:3])
+    nout = tuple(int(v) for v in Nres[:3])
+    batch = int(np.prod(x.shape[3:])) if x.dim() > 3 else 1
+    out = empty_colmajor(nout + tuple(x.shape[3:]), x.device)
+    a = (ctypes.c_int32 * 3)(*nin)
+    b = (ctypes.c_int32 * 3)(*nout)
+    ctx._prep()
+    check(ctx.lib.dis_resample(ctx.h, a, b, batch, _ptr(x), _ptr(out)), ctx.h)
+    return out
+
+
+def gather_samples(y, rows, scale=1.0, ctx: Context = None):
+    """Rows of compact data y [Ns, n3, C] -> [len(rows), n3, C], times scale."""
+    ctx = ctx or default_context(y.device.index or 0)
+    _ptr(y)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    Ns = int(y.shape[0])
+    ncol = int(np.prod(y.shape[1:]))
+    out = empty_colmajor((len(rows),) + tuple(y.shape[1:]), y.device)
+    ctx._prep()
+    check(ctx.lib.dis_gather_samples(ctx.h, _ptr(y), Ns, ncol, rows.ctypes.data_as(_lib.c_int32_p), len(rows),
+                                     float(scale), _ptr(out)), ctx.h)
+    return out
+
+
+def coil_precond(S, eps=1e-9, ctx: Context = None):
+    """Reconstruction/solveXT.m:231-232: Se = 1/(sum_c |S_c|^2 + eps), float32 [n1,n2,n3]."""
+    ctx = ctx or default_context(S.device.index or 0)
+    nv, C = _coil_view(S, "S")
+    Se = torch.empty(tuple(S.shape[:-1])[::-1], dtype=torch.float32, device=S.device).permute(
+        *range(S.dim() - 2, -1, -1))
+    ctx._prep()
+    check(ctx.lib.dis_coil_precond(ctx.h, _ptr(S), nv, C, float(eps), _fptr(Se)), ctx.h)
+    return Se

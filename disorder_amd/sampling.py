@@ -62,3 +62,35 @@ def harmonic_sampling(H: np.ndarray, nstates: int, with_outer: bool = False):
     k1 = (order % n1).astype(np.int32)
     k2 = (order // n1).astype(np.int32)
     return k1, k2, off.astype(np.int32)
+
+
+def signed_freq(k, n):
+    """Signed frequency of FFT-order index k of a length-n transform (the
+    Nyquist bin of an even length is negative)."""
+    k = np.asarray(k)
+    return np.where(k < (n + 1) // 2, k, k - n)
+
+
+def level_sampling(k1, k2, off, N12, n12):
+    """Samples of a pyramid level (Methods/downsampleOperators.m:36, the
+    sampling resampled in k-space with resampling(A, NYRes(1:2), 1), then
+    Methods/buildHarmonicSampling.m:36-49 on the cropped plane).  A sample
+    survives when both signed frequencies lie in [-ceil((n-1)/2),
+    floor((n-1)/2)] of the level size; its level index is the frequency mod n.
+    Column-major order within a state is preserved by the crop, so the kept
+    samples stay in their full-resolution order.  Returns (rows, k1, k2, off)
+    with rows indexing the full-resolution samples."""
+    k1 = np.asarray(k1)
+    k2 = np.asarray(k2)
+    off = np.asarray(off)
+    f1 = signed_freq(k1, N12[0])
+    f2 = signed_freq(k2, N12[1])
+    keep = np.ones(len(k1), bool)
+    for f, n in ((f1, n12[0]), (f2, n12[1])):
+        keep &= (f >= -(n // 2)) & (f <= (n - 1) // 2)
+    rows = np.nonzero(keep)[0]
+    state = np.searchsorted(off, np.arange(len(k1)), side="right") - 1
+    offl = np.zeros(len(off), np.int64)
+    offl[1:] = np.cumsum(np.bincount(state[rows], minlength=len(off) - 1))
+    return (rows.astype(np.int32), np.mod(f1[rows], n12[0]).astype(np.int32),
+            np.mod(f2[rows], n12[1]).astype(np.int32), offl.astype(np.int32))

@@ -183,6 +183,26 @@ int dis_compress_coils(dis_ctx* ctx, const void* S, int64_t nvS, int32_t C, cons
                        const void* y, int64_t nvy, void* S_out, void* y_out, int32_t terms, int32_t* Ccount,
                        double* D, double* U);
 
+/* ---- Pyramid levels (Methods/downsampleOperators.m:34-60). */
+
+/* Fourier resampling (replaces Methods/resampling.m:1 with fo = 0, mirror = 0)
+ * of `batch` volumes [nin] -> [nout] (device complex64, column-major, volume
+ * stride prod(n)): per axis fft/nin, crop or zero-pad the spectrum about DC
+ * keeping frequencies [-ceil((m-1)/2), floor((m-1)/2)], m = min(nin, nout),
+ * then ifft*nout.  Sizes <= 1024 with prime factors <= 7. */
+int dis_resample(dis_ctx* ctx, const int32_t nin[3], const int32_t nout[3], int64_t batch, const void* x, void* out);
+
+/* Compact k-space rows of a pyramid level: out[r + Ns_out*col] =
+ * scale * y[rows[r] + Ns_in*col] for col < ncol (y [Ns_in x ncol] device
+ * complex64, rows host).  With the readout resampled by dis_resample this is
+ * downsampleOperators.m:48-51 on the sampled rows. */
+int dis_gather_samples(dis_ctx* ctx, const void* y, int32_t Ns_in, int64_t ncol, const int32_t* rows, int32_t Ns_out,
+                       float scale, void* out);
+
+/* CG preconditioner Se = 1/(sum_c |S_c|^2 + eps) per voxel
+ * (Reconstruction/solveXT.m:231-232); S [nv x C] device, Se [nv] device float. */
+int dis_coil_precond(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, float eps, float* Se);
+
 /* Kernel classes for dis_profile_read. */
 #define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */
 #define DIS_K_BCAST 1  /* first forward pass, one input volume to all states */

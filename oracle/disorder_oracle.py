@@ -785,6 +785,132 @@ def resample_fourier(x, nres):
     return x
 
 
+def resample_kspace(A, nres):
+    """Methods/resampling.m with fo=1 (k-space input, DC first): per axis
+    fftshift, crop or zero-pad about ceil((Nmax+1)/2), ifftshift; no FFT."""
+    A = np.asarray(A)
+    for m, nr in enumerate(nres):
+        no = A.shape[m]
+        if no == nr:
+            continue
+        nmin, nmax = min(no, nr), max(no, nr)
+        zero = math.ceil((nmax + 1) / 2)
+        orig = zero - math.ceil((nmin - 1) / 2)
+        fina = zero + math.floor((nmin - 1) / 2)
+        X = np.fft.fftshift(A, axes=m)
+        idx = [slice(None)] * A.ndim
+        idx[m] = slice(orig - 1, fina)
+        if no < nr:
+            shp = list(A.shape)
+            shp[m] = nr
+            Y = np.zeros(shp, A.dtype)
+            Y[tuple(idx)] = X
+        else:
+            Y = X[tuple(idx)]
+        A = np.fft.ifftshift(Y, axes=m)
+    return A
+
+
+def level_data(yc, samp: "Sampling", H, N, n):
+    """Compact data of a pyramid level, following the reference literally:
+    zero-filled unitary k-space (solveXT.m:342 convention) -> image space ->
+    resampling(y, NYRes) in image space (downsampleOperators.m:48-51) ->
+    unitary FFT along the phase encodes at the level size (solveXT.m:342) ->
+    buildHarmonicSampling of the k-space-cropped mask (downsampleOperators.m:36).
+    yc: [Ns, N3, C] compact samples of the full level.  Returns
+    (y_level [Ns', n3, C], k1, k2, off)."""
+    N1, N2, N3 = N
+    C = yc.shape[-1]
+    Y = np.zeros((N1, N2, N3, C), np.complex128)
+    Y[samp.k1, samp.k2] = yc
+    y_img = np.fft.ifft(np.fft.ifft(Y, axis=0), axis=1) * math.sqrt(N1 * N2)
+    y_lev = resample_fourier(y_img, tuple(n) + (C,))
+    Yl = np.fft.fft(np.fft.fft(y_lev, axis=0), axis=1) / math.sqrt(n[0] * n[1])
+    Hl = resample_kspace(H, n[:2])
+    S = int(H.max())
+    k1, k2, off, _ = build_harmonic_sampling(Hl, S)
+    ns = off[S]
+    return Yl[k1[:ns], k2[:ns]], k1[:ns], k2[:ns], off[:S + 1]
+
+
 def level_size(N, l):
     """Methods/downsampleOperators.m:34: ceil(N/2^l)."""
     return [int(math.ceil(v / 2 ** l)) for v in N]
+
+
+# ----------------------------------------------------------------------------
+# Pyramid driver (Reconstruction/solveXT.m:266-531, joint CG + LM per level)
+# ----------------------------------------------------------------------------
+def _level_kspace_rows(k1, k2, off, N12, n12):
+    """Full-resolution sample rows that survive the k-space crop of a level
+    and their level indices (resampling(A, NYRes(1:2), 1) followed by
+    buildHarmonicSampling), by signed frequency."""
+    def sf(k, n):
+        return np.where(k < (n + 1) // 2, k, k - n)
+    f1, f2 = sf(np.asarray(k1), N12[0]), sf(np.asarray(k2), N12[1])
+    keep = ((f1 >= -(n12[0] // 2)) & (f1 <= (n12[0] - 1) // 2) &
+            (f2 >= -(n12[1] // 2)) & (f2 <= (n12[1] - 1) // 2))
+    rows = np.nonzero(keep)[0]
+    state = np.searchsorted(np.asarray(off), np.arange(len(k1)), side="right") - 1
+    offl = np.zeros(len(off), np.int64)
+    offl[1:] = np.cumsum(np.bincount(state[rows], minlength=len(off) - 1))
+    return rows, np.mod(f1[rows], n12[0]), np.mod(f2[rows], n12[1]), offl
+
+
+def pyramid_levels(N, nlevels):
+    """Level sizes ceil(N/2^f), f = nlevels-1 .. 0 (downsampleOperators.m:34
+    with isotropic voxels, pyramidPlan.m:19-20), and resIso
+    (pyramidPlan.m:25, 1 mm voxels)."""
+    out = []
+    for l in range(nlevels):
+        f = nlevels - 1 - l
+        out.append((level_size(N, f), math.sqrt(2.0) * 2.0 ** f))
+    return out
+
+
+def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
+                  traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None):
+    """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
+    the loop semantics kept: per level the operators are downsampled
+    (downsampleOperators.m:34-60), x is resampled from the previous level,
+    w = winit and cT are reset (:313-326), states with fewer than echoesMin
+    samples are frozen (:322-324), tL = traLimXT * resIso(l) (:370); each
+    outer iteration resets motion convergence on the schedule of :376-381,
+    runs n_cg CG iterations with 'RelativeResidual2Error' and tol 0 (:431),
+    then one LMsolver call (:507); when every state has converged, a final
+    CG of final_cg iterations with 'Energy' tol 1e-3 closes the level when
+    an outer iteration is left for it (:425-431 with estT(l) = 0 and nIt(1),
+    then the break of :523).  Returns (x at full size, T)."""
+    S = len(off) - 1
+    T = np.zeros((S, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
+    x = None
+    for li, (n, resIso) in enumerate(pyramid_levels(N, nlevels)):
+        rows, l1, l2, loff = _level_kspace_rows(k1, k2, off, N[:2], n[:2])
+        scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
+        yl = resample_fourier(yc[rows] * scale, (len(rows), n[2], yc.shape[2]))
+        Sl = resample_fourier(Smaps, tuple(n) + (Smaps.shape[3],))
+        samp = Sampling(l1, l2, loff, n[0], n[1])
+        Se = 1.0 / (np.sum(np.abs(Sl) ** 2, axis=-1) + 1e-9)
+        x = np.zeros(n, np.complex128) if x is None else resample_fourier(x, n)
+        w = np.full(S, winit)
+        nsamp = np.diff(loff)
+        cT0 = nsamp < echoesMin
+        cT = cT0.copy()
+        tL = (traLimXT[0] * resIso, traLimXT[1] * resIso)
+        nReset, cont = 0, 0
+        for it in range(1, n_outer + 1):
+            if (nReset == 0 and cont == 0) or (nReset > 0 and cont % nReset == 0) or cT.all():
+                nReset += 1
+                cont = 0
+                cT = cT0.copy()
+            cont += 1
+            x, _, _ = cg_solver(yl, Sl, samp, T, N, x0=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0,
+                                Se=Se)
+            T, w, cT, _ = lm_solver(yl, x, Sl, samp, T, N, w, cT, winit, tL)
+            if callback is not None:
+                callback(li, it, x, T)
+            if cT.all():
+                if final_cg and it < n_outer:
+                    x, _, _ = cg_solver(yl, Sl, samp, T, N, x0=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
+                break
+    return resample_fourier(x, N), T

@@ -1,0 +1,162 @@
+"""Pyramid levels (Methods/resampling.m, Methods/downsampleOperators.m).
+
+CPU: the oracle's Fourier resampling against closed forms (band-limited
+complex exponentials resample exactly; up then down is the identity), the
+k-space mask crop, and the host level sampling (bit-exact against cropping the
+mask and re-running the harmonic sampling, as the reference does).
+GPU: dis_resample against the oracle (rel-L2 <= 1e-5) for down/up/odd sizes
+and batches, the per-level compact data against the oracle's literal
+image-space route, and the preconditioner."""
+import math
+
+import numpy as np
+import pytest
+
+from disorder_amd import sampling as SA
+from oracle import disorder_oracle as O
+from tests.helpers import checkered_sampling, crandn, rel_l2
+
+
+@pytest.mark.parametrize("N,n", [(16, 8), (16, 9), (15, 8), (15, 6), (12, 12), (8, 16), (7, 12)])
+def test_resample_band_limited_exponentials_exact(N, n):
+    nmin = min(N, n)
+    j = np.arange(N)
+    jn = np.arange(n)
+    for f in range(-(nmin // 2), (nmin - 1) // 2 + 1):
+        if N % 2 == 0 and f == -(N // 2) and n > N:
+            continue  # the input's Nyquist bin is not a resolvable exponential
+        x = np.exp(2j * np.pi * f * j / N)[:, None, None]
+        y = O.resample_fourier(x, (n, 1, 1))
+        ref = np.exp(2j * np.pi * f * jn / n)[:, None, None]
+        assert np.abs(y - ref).max() < 1e-12, (N, n, f)
+    # frequencies outside the kept band are removed
+    fo = (nmin - 1) // 2 + 1
+    if fo < (N + 1) // 2 and N > n:
+        x = np.exp(2j * np.pi * fo * j / N)[:, None, None]
+        assert np.abs(O.resample_fourier(x, (n, 1, 1))).max() < 1e-12
+
+
+def test_resample_kspace_mask_crop():
+    H = np.arange(1, 8 * 6 + 1).reshape(8, 6, order="F")
+    Hc = O.resample_kspace(H, (4, 3))
+    # fftshift, centre crop, ifftshift of the DC-first plane keeps the
+    # frequencies [-2, 1] x [-1, 1]
+    f1 = [0, 1, -2, -1]
+    f2 = [0, 1, -1]
+    for a, u in enumerate(f1):
+        for b, v in enumerate(f2):
+            assert Hc[a, b] == H[u % 8, v % 6]
+
+
+@pytest.mark.parametrize("N12,n12,t", [((16, 12), (8, 6), (4, 3)), ((16, 12), (4, 3), (4, 3)),
+                                       ((15, 9), (8, 5), (3, 3)), ((32, 24), (16, 12), (4, 4))])
+def test_level_sampling_bit_exact(N12, n12, t):
+    rng = np.random.default_rng(sum(N12))
+    S = t[0] * t[1]
+    H = checkered_sampling(rng, N12[0] - N12[0] % t[0], N12[1] - N12[1] % t[1], S, *t)
+    Hf = np.zeros(N12, np.int64)
+    Hf[:H.shape[0], :H.shape[1]] = H
+    k1, k2, off = SA.harmonic_sampling(Hf, S)
+    rows, l1, l2, loff = SA.level_sampling(k1, k2, off, N12, n12)
+    Hl = O.resample_kspace(Hf, n12)
+    r1, r2, roff, _ = O.build_harmonic_sampling(Hl, S)
+    ns = roff[S]
+    assert np.array_equal(l1, r1[:ns]) and np.array_equal(l2, r2[:ns])
+    assert np.array_equal(loff, roff[:S + 1])
+    # rows point at the same phase-encode positions at full resolution
+    assert np.array_equal(SA.signed_freq(k1[rows], N12[0]) % n12[0], l1)
+
+
+# ---------------------------------------------------------------- GPU
+def _to_dev(a, dev):
+    import disorder_amd as D
+    import torch
+    t = torch.from_numpy(np.asfortranarray(np.asarray(a, np.complex64)))
+    return D.colmajor(t.to(dev))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nin,nout,B", [((32, 24, 32), (16, 12, 16), 1), ((16, 12, 16), (32, 24, 32), 2),
+                                        ((15, 14, 9), (8, 7, 5), 3), ((64, 48, 64), (32, 24, 32), 1),
+                                        ((20, 18, 16), (20, 9, 16), 2), ((8, 6, 8), (8, 6, 8), 1),
+                                        ((256, 8, 4), (128, 8, 4), 1)])
+def test_resample_matches_oracle(dev, nin, nout, B):
+    import disorder_amd as D
+    rng = np.random.default_rng(sum(nin) + B)
+    x = crandn(rng, nin + (B,))
+    got = D.resampling(_to_dev(x, dev), nout).cpu().numpy()
+    ref = O.resample_fourier(x, nout + (B,))
+    assert rel_l2(got, ref) < 1e-5
+
+
+@pytest.mark.gpu
+def test_level_data_matches_oracle_route(dev):
+    import disorder_amd as D
+    rng = np.random.default_rng(3)
+    N = (16, 12, 10)
+    n = (8, 6, 5)
+    S, C = 12, 3
+    H = checkered_sampling(rng, N[0], N[1], S, 4, 3)
+    k1, k2, off = SA.harmonic_sampling(H, S)
+    samp = O.Sampling(k1, k2, off, N[0], N[1])
+    yc = crandn(rng, (len(k1), N[2], C))
+    yl_ref, r1, r2, roff = O.level_data(yc, samp, H, N, n)
+    rows, l1, l2, loff = SA.level_sampling(k1, k2, off, N[:2], n[:2])
+    assert np.array_equal(l1, r1) and np.array_equal(l2, r2)
+    scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
+    g = D.gather_samples(_to_dev(yc, dev), rows, scale)
+    got = D.resampling(g, (len(rows), n[2], C))
+    assert rel_l2(got.cpu().numpy(), yl_ref) < 1e-5
+
+
+@pytest.mark.gpu
+def test_coil_precond(dev):
+    import disorder_amd as D
+    rng = np.random.default_rng(4)
+    Sm = crandn(rng, (9, 7, 5, 4))
+    Se = D.coil_precond(_to_dev(Sm, dev)).cpu().numpy()
+    ref = 1.0 / (np.sum(np.abs(Sm.astype(np.complex64)) ** 2, axis=-1) + 1e-9)
+    assert np.abs(Se - ref).max() / ref.max() < 1e-6
+
+
+def _coil_maps_np(n, C, seed):
+    rng = np.random.default_rng(seed)
+    g = np.meshgrid(*[np.linspace(-1, 1, v) for v in n], indexing="ij")
+    maps = []
+    for c in range(C):
+        a = 2 * np.pi * c / C
+        ctr = (0.8 * np.cos(a), 0.8 * np.sin(a), rng.uniform(-0.3, 0.3))
+        r2 = sum((gi - ci) ** 2 for gi, ci in zip(g, ctr))
+        maps.append(np.exp(-r2 / 1.2) * np.exp(1j * (0.5 * g[0] * np.cos(a) + 0.5 * g[1] * np.sin(a))))
+    Sm = np.stack(maps, -1)
+    return Sm / np.sqrt(np.sum(np.abs(Sm) ** 2, -1, keepdims=True))
+
+
+@pytest.mark.gpu
+def test_pyramid_driver_matches_oracle(dev):
+    """Two levels, two outer iterations each (CG then LM): the GPU driver and
+    the FP64 oracle driver reach the same image (NRMSE <= 1e-4) and motion
+    (0.01 deg, 0.01 voxel) from the same data."""
+    import disorder_amd as D
+    from disorder_amd import pyramid as PY
+    from tests.helpers import smooth_volume
+    rng = np.random.default_rng(21)
+    N = (32, 24, 16)
+    S, C = 16, 4
+    x_gt = smooth_volume(rng, N, nblob=8)
+    Sm = _coil_maps_np(N, C, 5)
+    H = checkered_sampling(rng, N[0], N[1], S, 4, 4)
+    k1, k2, off = SA.harmonic_sampling(H, S)
+    T_gt = np.zeros((S, 6))
+    T_gt[:, :3] = rng.uniform(-1, 1, (S, 3))
+    T_gt[:, 3:] = rng.uniform(-2, 2, (S, 3)) * np.pi / 180
+    samp = O.Sampling(k1, k2, off, N[0], N[1])
+    y = O.encode(x_gt, Sm, samp, T_gt, N).astype(np.complex64).astype(np.complex128)
+    Sm32 = Sm.astype(np.complex64).astype(np.complex128)
+    x_ref, T_ref = O.pyramid_solve(y, Sm32, k1, k2, off, N, nlevels=2, n_outer=2, n_cg=4)
+    xg, Tg, info = PY.solveXT(_to_dev(y, dev), _to_dev(Sm, dev), k1, k2, off, N, nlevels=2, n_outer=2, n_cg=4)
+    xg = xg.cpu().numpy()
+    assert rel_l2(xg, x_ref) < 1e-4, rel_l2(xg, x_ref)
+    assert np.abs(Tg[:, :3] - T_ref[:, :3]).max() < 0.01
+    assert np.abs(Tg[:, 3:] - T_ref[:, 3:]).max() * 180 / np.pi < 0.01
+    assert [lv["n"] for lv in info["levels"]] == [[16, 12, 8], [32, 24, 16]]
