@@ -61,8 +61,9 @@ class Level:
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None):
     """Returns (x [N] on the device, T [S x 6], info).  callback(level, it, x,
-    T) runs after every outer iteration outside the timed region (its time
-    is reported in info['callback_s'] and excluded from info['solve_s']).
+    T, t_solve) runs after every outer iteration outside the timed region
+    (t_solve: solve seconds so far; callback time is reported in
+    info['callback_s'] and excluded from info['solve_s']).
     proposer(level_ops, x, T, w, cT) -> T' may propose motion before each LM
     call; LM starts from the proposal only where it lowers the per-state
     energy (computeEnergy), so the plain method stays the fallback."""
@@ -109,7 +110,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             if callback is not None:
                 torch.cuda.synchronize()
                 tc = time.perf_counter()
-                callback(li, it, x, T)
+                callback(li, it, x, T, tc - t0 - cb_time)
                 cb_time += time.perf_counter() - tc
             if cT.all():
                 if final_cg and it < n_outer:

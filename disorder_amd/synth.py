@@ -109,12 +109,16 @@ def disorder_sampling(n1, n2, S, t1, t2, seed):
     return H, k1, k2, off
 
 
-def noise_level(x, Smaps, snr_db):
-    """Simulation/SNRToLevels.m:23-33: sigma from the mean signal power and the
-    mean of 1/(sum|S|^2 + 1e-3) per voxel."""
+def noise_level(x, Smaps, snr_db, unitary=True):
+    """Simulation/SNRToLevels.m:23-33: sigma such that the SENSE-combined image
+    noise power 2 sigma^2 mean(1/(sum|S|^2 + 1e-3)) / K equals the mean signal
+    power over 10^(dB/10).  K = numel(x) for the simulator's unnormalised
+    DFT (the reference's formula); K = 1 for data in the reconstruction
+    path's unitary convention (solveXT.m:342), where k-space noise of
+    variance 2 sigma^2 maps to image noise of the same variance per coil."""
     meanS = (x.abs() ** 2).mean().item()
     meanN = meanS / (10 ** (snr_db / 10))
-    P = (1.0 / ((Smaps.abs() ** 2).sum(dim=-1) + 1e-3)).mean().item() / x.numel()
+    P = (1.0 / ((Smaps.abs() ** 2).sum(dim=-1) + 1e-3)).mean().item() / (1 if unitary else x.numel())
     return math.sqrt(meanN / (2 * P))
 
 
