@@ -30,8 +30,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--workload", choices=["ehe", "pyramid"], default="ehe",
-                    help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid")
+    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress"], default="ehe",
+                    help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid; "
+                         "compress: 64 -> 8 coil compression of maps and data")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--outer", type=int, default=10)
     ap.add_argument("--cg", type=int, default=8)
@@ -140,6 +141,70 @@ def bench_pyramid(args):
     return line
 
 
+def bench_compress(args):
+    """Coil compression at 256x192x256 with 64 coils -> 8 (compressCoils):
+    Gram matrix, host eigenbasis, and the split-bf16 MFMA contraction of the
+    maps [12.58M x 64] and of the compact data [49152 x 256 x 64].  Errors
+    of the compressed arrays against an FP64 product on a voxel sample."""
+    import numpy as np
+    import torch
+    import disorder_amd as D
+    from disorder_amd import synth
+
+    dev = torch.device("cuda", 0)
+    n, C, Cc, S = (256, 192, 256), 64, 8, args.shots
+    Sm = synth.coil_maps(n, C, seed=3, device=dev)
+    g = torch.Generator(device=dev).manual_seed(7)
+    y = D.empty_colmajor((n[0] * n[1], n[2], C), dev)
+    y.copy_(torch.randn(tuple(y.shape), dtype=torch.complex64, device=dev, generator=g))
+    ctx = D.default_context(0)
+    P = D.coil_gram(Sm, ctx)
+    U, Dv, cc = D.coil_basis(P, Cc)
+    A = np.conj(U[:, :Cc]).T
+    res = {}
+    for terms in (3, 6):
+        for _ in range(args.warmup):
+            D.coil_apply(A, Sm, terms, ctx)
+        ctx.profile(True)
+        for _ in range(args.steps):
+            out = D.coil_apply(A, Sm, terms, ctx)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        nl, kms, kb = prof["coil_apply"]
+        # FP64 check on 4096 sampled voxels
+        rng = np.random.default_rng(0)
+        idx = torch.from_numpy(rng.choice(Sm.shape[0] * Sm.shape[1] * Sm.shape[2], 4096, replace=False)).to(dev)
+        Xs = Sm.reshape(-1, C)[idx].cpu().numpy().astype(np.complex128)
+        Os = out.reshape(-1, Cc)[idx].cpu().numpy().astype(np.complex128)
+        ref = Xs @ A.T
+        err = float(np.linalg.norm(Os - ref) / np.linalg.norm(ref))
+        res[terms] = {"avg_launch_ms": kms / nl, "GBps": kb / (kms * 1e6), "frac": kb / (kms * 1e6) / HBM_PEAK_GBS,
+                      "rel_l2_vs_fp64": err}
+        del out
+    ctx.profile(True)
+    for _ in range(max(1, args.steps // 2)):
+        D.coil_gram(Sm, ctx)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    gl, gms, _ = prof["coil_gram"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Sc, yc, cc2, _, _ = D.compressCoils(Sm, Cc, y, 3, ctx)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter() - t0
+    gram_flop = 8.0 * Sm.numel() * C
+    line = {"metric": "coil compression 64->8 at 256x192x256 (apply GB/s)", "value": res[3]["GBps"], "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "dtype": "bf16x3 MFMA, fp32 accumulate", "data": "synthetic (64 Gaussian coil maps, random data)",
+            "config": {"workload": "compressCoils: Gram [12.58M x 64]^H [12.58M x 64], apply [12.58M x 64].[64 x 8] "
+                                   "on maps and [12.58M x 64].[64 x 8] on data"},
+            "roofline": {"bound": "hbm", "kernel": "coil_apply", "achieved": res[3]["GBps"], "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": res[3]["frac"], "traffic": None},
+            "apply": res, "gram_ms": gms / gl, "gram_TFLOPs": gram_flop / (gms / gl * 1e9),
+            "compressCoils_total_s": t_all, "components": int(cc2[0])}
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -148,6 +213,9 @@ def main():
         sys.exit(2)
     if args.workload == "pyramid":
         print(json.dumps(bench_pyramid(args)), flush=True)
+        return
+    if args.workload == "compress":
+        print(json.dumps(bench_compress(args)), flush=True)
         return
     import numpy as np
     import torch
@@ -189,8 +257,13 @@ def main():
     achieved = kb / (kms * 1e6)
     V = 8 * n[0] * n[1] * n[2]
     passmodel = S * V * (4 * 10 + 6 * C + 3)
+    traffic = None
+    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "ehe_traffic.json")
+    if os.path.exists(tf) and S == 64:
+        with open(tf) as fh:
+            traffic = json.load(fh)["bytes_per_launch"].get(dom)
     line = {
-        "metric": "E^H E apps/s at 256x256x192x8 coils",
+        "metric": "E^H E apps/s at 256x256x192x8 coils; time to 1e-3 NRMSE, learned vs plain LM",
         "value": 1000.0 / ms,
         "unit": "E^H E apps/s",
         "n_gpus": 1,
@@ -206,7 +279,8 @@ def main():
         "config": {"workload": f"E^H E (dis_encodedecode) at 256x192x256 (PE1 x PE2 x RO), {C} coils, {S} shots "
                                f"of {off[1] - off[0]} lines", "grid": list(n), "coils": C, "shots": S},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/ehe_traffic.json (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
                      "algorithmic_bytes_per_launch": kb / nl, "avg_launch_ms": kms / nl},
         "pass_model_v1": {"bytes_per_app": passmodel, "GBps": passmodel / (ms * 1e6),
                           "frac_of_peak": passmodel / (ms * 1e6) / HBM_PEAK_GBS},
