@@ -30,12 +30,18 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress"], default="ehe",
+    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched"], default="ehe",
                     help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid; "
-                         "compress: 64 -> 8 coil compression of maps and data")
+                         "compress: 64 -> 8 coil compression of maps and data; batched: six uneven volumes in one "
+                         "batched CG with hipGraph-captured iterations")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--outer", type=int, default=10)
     ap.add_argument("--cg", type=int, default=8)
+    ap.add_argument("--final-cg", dest="final_cg", type=int, default=300,
+                    help="CG iterations (Energy, tol 1e-3) closing a level whose states all converged (nIt(1))")
+    ap.add_argument("--learned", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "disorder_amd",
+                                                      "weights", "motion_net.npy"),
+                    help="weights of the learned proposal for --workload pyramid ('none' to skip)")
     return ap.parse_args()
 
 
@@ -87,16 +93,49 @@ def reconstruction_problem(dev, n=(256, 192, 256), C_in=32, C=8, S=64, snr_db=25
     return {"x_gt": x_gt, "S": Sc, "y": yc, "k1": k1, "k2": k2, "off": off, "T_gt": T_gt, "n": n, "C": C, "S_states": S}
 
 
-def bench_pyramid(args):
-    """Time to 1e-3 NRMSE of the plain joint reconstruction (4 levels, 10 LM
-    iterations per level, 8 CG iterations per outer step).  x_ref is the
-    method's own converged image; NRMSE_t = ||x_t - x_ref|| / ||x_ref|| of
-    the image after each outer iteration, upsampled to the full grid (the
-    upsampling and snapshot copies are outside the timed solve)."""
-    import numpy as np
+def _run_pyramid(P, args, proposer=None, x_ref=None):
+    """One joint reconstruction with per-outer-iteration snapshots (taken
+    outside the timed solve).  Returns (x, T, info, curve)."""
     import torch
     import disorder_amd as D
     from disorder_amd import pyramid as PY
+    n = P["n"]
+    xg = P["x_gt"]
+    ng = torch.linalg.vector_norm(xg).item()
+    snaps = []
+
+    def cb(level, it, x, T, t_solve):
+        xf = D.resampling(x, n) if list(x.shape) != list(n) else x.clone()
+        e_gt = torch.linalg.vector_norm(xf - xg).item() / ng
+        e_ref = None
+        if x_ref is not None:
+            e_ref = torch.linalg.vector_norm(xf - x_ref).item() / torch.linalg.vector_norm(x_ref).item()
+        snaps.append({"level": level, "outer": it, "t_s": t_solve, "nrmse_vs_gt": e_gt, "nrmse_vs_ref": e_ref,
+                      "x": xf})
+        print(f"[pyramid] {'learned' if proposer else 'plain'} level {level} outer {it}: solve {t_solve:.2f} s, "
+              f"NRMSE vs GT {e_gt:.5f}", file=sys.stderr, flush=True)
+
+    x, T, info = PY.solveXT(P["y"], P["S"], P["k1"], P["k2"], P["off"], n, nlevels=args.levels,
+                            n_outer=args.outer, n_cg=args.cg, final_cg=args.final_cg, callback=cb, proposer=proposer)
+    for sn in snaps:
+        if sn["x"] is not None:
+            sn["nrmse_vs_own_final"] = torch.linalg.vector_norm(sn["x"] - x).item() / torch.linalg.vector_norm(x).item()
+            sn["x"] = None
+    return x, T, info, snaps
+
+
+def bench_pyramid(args):
+    """Joint reconstruction at 256x192x256 (32 coils compressed to 8, 64
+    shots, 4 levels, 10 LM iterations per level, 8 CG iterations per outer
+    step, a level closed by CG to 'Energy' tol 1e-3 once every state
+    converged), plain and with the learned proposal.  value = the plain
+    method's solve time until its image is within 1e-3 NRMSE of its own
+    converged image; the learned run reports the same time for itself, its
+    ratio to the plain one, and the difference of the final NRMSE against the
+    ground truth.  Snapshots are taken outside the timed solve."""
+    import numpy as np
+    import torch
+    from disorder_amd import learned as LN
 
     dev = torch.device("cuda", 0)
     torch.cuda.synchronize()
@@ -105,39 +144,44 @@ def bench_pyramid(args):
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - tg
     n = P["n"]
-    snaps = []
 
-    def cb(level, it, x, T, t_solve):
-        xf = D.resampling(x, n) if list(x.shape) != list(n) else x.clone()
-        snaps.append((level, it, t_solve, xf))
-        print(f"[pyramid] level {level} outer {it}: solve {t_solve:.2f} s", file=sys.stderr, flush=True)
+    def summary(x, T, info, snaps):
+        T_err = P["T_gt"] - T
+        T_err = T_err - T_err.mean(axis=0, keepdims=True)
+        return {"solve_s": info["solve_s"], "levels": info["levels"], "lm_trials": info["lm_trials"],
+                "proposals_accepted": info["proposals_accepted"],
+                "final_nrmse_vs_gt": snaps[-1]["nrmse_vs_gt"],
+                "motion_err_max_vox": float(np.abs(T_err[:, :3]).max()),
+                "motion_err_max_deg": float(np.abs(T_err[:, 3:]).max() * 180 / math.pi)}
 
-    x, T, info = PY.solveXT(P["y"], P["S"], P["k1"], P["k2"], P["off"], n, nlevels=args.levels,
-                            n_outer=args.outer, n_cg=args.cg, callback=cb)
-    nr = torch.linalg.vector_norm(x).item()
-    xg = P["x_gt"]
-    curve = []
-    tts = None
-    for lv, it, ts, xs in snaps:
-        e = torch.linalg.vector_norm(xs - x).item() / nr
-        curve.append({"level": lv, "outer": it, "t_s": ts, "nrmse_vs_final": e})
-        if tts is None and e <= 1e-3:
-            tts = ts
-    del snaps
-    T_err = P["T_gt"] - T
-    T_err = T_err - T_err.mean(axis=0, keepdims=True)
-    nrmse_gt = torch.linalg.vector_norm(x - xg).item() / torch.linalg.vector_norm(xg).item()
+    xp, Tp, ip, sp = _run_pyramid(P, args)
+    tts = next((sn["t_s"] for sn in sp if sn["nrmse_vs_own_final"] <= 1e-3), None)
+    plain = summary(xp, Tp, ip, sp)
+    plain["time_to_1e-3_own_final_s"] = tts
+    plain["curve"] = sp
     line = {"metric": "time to 1e-3 NRMSE (plain LM), s", "value": tts, "unit": "s", "n_gpus": 1,
             "higher_is_better": False, "dtype": "complex64",
             "data": "synthetic (seeded phantom, 32 Gaussian coils compressed to 8, DISORDER 8x8 sampling, "
                     "random-walk motion, 25 dB noise)",
             "config": {"workload": f"joint CG+LM pyramid at {n[0]}x{n[1]}x{n[2]}, 8 coils, {args.shots} shots, "
-                                   f"{args.levels} levels, {args.outer} outer, {args.cg} CG"},
-            "solve_s": info["solve_s"], "data_gen_s": gen_s, "levels": info["levels"], "lm_trials": info["lm_trials"],
-            "nrmse_vs_gt": nrmse_gt,
-            "motion_err_max_vox": float(np.abs(T_err[:, :3]).max()),
-            "motion_err_max_deg": float(np.abs(T_err[:, 3:]).max() * 180 / math.pi),
-            "curve": curve}
+                                   f"{args.levels} levels, {args.outer} outer, {args.cg} CG, final CG "
+                                   f"{args.final_cg} (Energy 1e-3)"},
+            "data_gen_s": gen_s, "plain": plain}
+    x_ref = xp
+    wpath = args.learned if args.learned not in (None, "", "none") else None
+    if wpath is not None and os.path.exists(wpath):
+        prop = LN.Proposer(LN.MotionNet.load(wpath))
+        xl, Tl, il, sl = _run_pyramid(P, args, proposer=prop, x_ref=x_ref)
+        ttl = next((sn["t_s"] for sn in sl if sn["nrmse_vs_own_final"] <= 1e-3), None)
+        learned = summary(xl, Tl, il, sl)
+        learned["time_to_1e-3_own_final_s"] = ttl
+        learned["final_nrmse_vs_plain_final"] = sl[-1]["nrmse_vs_ref"]
+        learned["proposals"] = {"proposed": prop.proposed, "accepted": prop.accepted}
+        learned["curve"] = sl
+        line["learned"] = learned
+        line["learned_nrmse_gt_minus_plain"] = learned["final_nrmse_vs_gt"] - plain["final_nrmse_vs_gt"]
+        if tts and ttl:
+            line["learned_vs_plain_time_ratio"] = ttl / tts
     return line
 
 
@@ -205,6 +249,74 @@ def bench_compress(args):
     return line
 
 
+def bench_batched(args):
+    """Six independent reconstructions of uneven sizes (160x120x160,
+    2x 192x144x192, 224x160x224, 2x 256x192x256; 40-96 shots with 10 %
+    truncated; 8 coils) solved by one batched CG of 12 iterations whose
+    iterations 2..12 replay one captured hipGraph.  The autotuner picks the
+    pencil-pass shapes of each grid first.  Reports the batched time per
+    iteration (graph and eager replay), aggregate throughput and each
+    volume's latency when solved alone."""
+    import numpy as np
+    import torch
+    import disorder_amd as D
+    from disorder_amd import synth
+
+    dev = torch.device("cuda", 0)
+    nIt = 12
+    P = synth.batched_problem(dev, C=8, seed=11)
+    ctx = D.default_context(0)
+    t0 = time.perf_counter()
+    tune = {}
+    for p in P:
+        key = "x".join(str(v) for v in p["n"])
+        if key not in tune:
+            tune[key] = D.autotune(p["n"], S=16, ctx=ctx)
+    tune_s = time.perf_counter() - t0
+    Es = [p["E"] for p in P]
+    ys = [p["y"] for p in P]
+    Ts = [p["T"] for p in P]
+    Ses = [p["Se"] for p in P]
+    D.CGsolver_batched(ys, Es, Ts, Ses, nIt=2, use_graph=True)  # warm-up
+    runs = {}
+    for mode in ("graph", "eager"):
+        best = None
+        for _ in range(max(1, args.steps // 5)):
+            torch.cuda.synchronize()
+            tw = time.perf_counter()
+            xs, res, ms = D.CGsolver_batched(ys, Es, Ts, Ses, nIt=nIt, use_graph=(mode == "graph"))
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - tw
+            if best is None or wall < best[0]:
+                best = (wall, ms, res)
+        wall, ms, res = best
+        runs[mode] = {"wall_s": wall, "setup_ms": ms[0], "first_iteration_ms": ms[1],
+                      "replayed_iterations_ms": ms[2], "ms_per_iteration_replayed": ms[2] / (nIt - 1),
+                      "final_relres": [float(r[-1]) for r in res]}
+    # per-volume latency alone (same batched entry point, one volume)
+    per_vol = []
+    for p in P:
+        torch.cuda.synchronize()
+        tw = time.perf_counter()
+        _, _, ms = D.CGsolver_batched([p["y"]], [p["E"]], [p["T"]], [p["Se"]], nIt=nIt, use_graph=True)
+        torch.cuda.synchronize()
+        per_vol.append({"n": list(p["n"]), "shots": p["S"], "lines": [int(p["lines"].min()), int(p["lines"].max())],
+                        "latency_s": time.perf_counter() - tw, "ms_per_iteration": ms[2] / (nIt - 1)})
+    vox = sum(int(np.prod(p["n"])) for p in P)
+    g = runs["graph"]
+    line = {"metric": "batched CG iterations/s over six uneven volumes", "value": 1000.0 / g["ms_per_iteration_replayed"],
+            "unit": "batched iterations/s", "n_gpus": 1, "steps": nIt, "higher_is_better": True,
+            "dtype": "complex64 (FP64 CG scalars)", "data": "synthetic (seeded phantoms, 8 coils, truncated DISORDER shots)",
+            "config": {"workload": "batched CG, 12 iterations, 6 volumes: " +
+                                   ", ".join(f"{p['n'][0]}x{p['n'][1]}x{p['n'][2]}/{p['S']} shots" for p in P),
+                       "total_voxels": vox},
+            "aggregate_volume_iterations_per_s": 6 * 1000.0 / g["ms_per_iteration_replayed"],
+            "aggregate_Gvoxel_iterations_per_s": vox / (g["ms_per_iteration_replayed"] * 1e6),
+            "graph": runs["graph"], "eager": runs["eager"], "per_volume": per_vol, "autotune": tune,
+            "autotune_s": tune_s}
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -216,6 +328,9 @@ def main():
         return
     if args.workload == "compress":
         print(json.dumps(bench_compress(args)), flush=True)
+        return
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args)), flush=True)
         return
     import numpy as np
     import torch

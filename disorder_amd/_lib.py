@@ -54,6 +54,12 @@ SIGNATURES = {
     "dis_energy": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p, c_float_p, c_double_p, c_double_p]),
     "dis_cg": (c_int, [c_void_p, c_double_p, c_void_p, c_float_p, c_float_p, ctypes.c_double, c_void_p, c_int32,
                        c_int32, c_int32, ctypes.c_double, c_double_p, c_int32_p, c_int32_p]),
+    "dis_autotune": (c_int, [c_void_p, c_int32_p, c_int32_p, c_int32, c_int32_p, c_double_p]),
+    "dis_autotune_clear": (c_int, [c_void_p]),
+    "dis_set_pass_shape": (c_int, [c_void_p, c_int32_p, c_int32, c_int32, c_int32, c_int32]),
+    "dis_cg_batched": (c_int, [c_void_p, c_int32, ctypes.POINTER(c_void_p), ctypes.POINTER(c_double_p),
+                               ctypes.POINTER(c_void_p), ctypes.POINTER(c_float_p), ctypes.POINTER(c_void_p), c_int32,
+                               c_int32, c_double_p, c_double_p]),
     "dis_rigid_transform_grad": (c_int, [c_void_p, c_int32_p, c_int32_p, c_double_p, c_int32, c_void_p, c_void_p,
                                          c_void_p]),
     "dis_lm_partials": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p, c_double_p]),
@@ -68,6 +74,12 @@ SIGNATURES = {
     "dis_gather_samples": (c_int, [c_void_p, c_void_p, c_int32, ctypes.c_int64, c_int32_p, c_int32, ctypes.c_float,
                                    c_void_p]),
     "dis_coil_precond": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, ctypes.c_float, c_float_p]),
+    "dis_mlp_param_count": (c_int, []),
+    "dis_mlp_forward": (c_int, [c_void_p, c_float_p, c_float_p, c_int32, c_int32_p, c_int32, c_float_p]),
+    "dis_mlp_backward": (c_int, [c_void_p, c_float_p, c_float_p, c_int32, c_int32_p, c_int32, c_float_p, c_float_p,
+                                 c_float_p]),
+    "dis_mlp_train_step": (c_int, [c_void_p, c_float_p, c_float_p, c_float_p, c_int32, ctypes.c_float, c_float_p,
+                                   c_float_p, c_int32, c_int32_p, c_int32, c_float_p]),
     "dis_profile_enable": (c_int, [c_void_p, c_int32]),
     "dis_profile_read": (c_int, [c_void_p, c_int32, ctypes.POINTER(ctypes.c_int64), c_double_p, c_double_p]),
 }

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -28,6 +29,13 @@ struct dis_ctx {
   void* dcoef = nullptr;
   size_t dcoef_cap = 0;
   std::map<int, std::pair<void*, size_t>> bufs;
+  // upload tape: while recording every staged upload gets its own persistent
+  // buffer; while replaying the same sequence of uploads returns those
+  // buffers without copies or synchronisation (for graph capture)
+  std::map<std::array<int, 4>, PassTune> tune;  // tuned pass shapes per (n1, n2, n3, axis)
+  int tape_mode = 0;  // 0 off, 1 record, 2 replay
+  std::vector<std::pair<void*, size_t>> tape;
+  size_t tape_pos = 0;
   // profiling
   bool prof = false;
   struct Rec {
@@ -137,6 +145,20 @@ int ctx_buffer(dis_ctx* c, int slot, size_t bytes, void** out) {
 // the buffer may be reused by the next upload.
 int ctx_upload_bytes(dis_ctx* c, const Staging& stg, const unsigned char** dptr) {
   const size_t nbytes = std::max<size_t>(stg.bytes.size(), 16);
+  if (c->tape_mode == 1) {
+    void* p = nullptr;
+    HIPCHK(c, hipMalloc(&p, nbytes));
+    c->tape.emplace_back(p, nbytes);
+    if (!stg.bytes.empty()) HIPCHK(c, hipMemcpy(p, stg.bytes.data(), stg.bytes.size(), hipMemcpyHostToDevice));
+    *dptr = (const unsigned char*)p;
+    return DIS_OK;
+  }
+  if (c->tape_mode == 2) {
+    if (c->tape_pos >= c->tape.size() || c->tape[c->tape_pos].second != nbytes)
+      return fail(c, DIS_E_INVALID, "upload tape replay diverged from the recording");
+    *dptr = (const unsigned char*)c->tape[c->tape_pos++].first;
+    return DIS_OK;
+  }
   if (nbytes > c->dcoef_cap) {
     if (c->dcoef) {
       HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -153,6 +175,36 @@ int ctx_upload_bytes(dis_ctx* c, const Staging& stg, const unsigned char** dptr)
   *dptr = (const unsigned char*)c->dcoef;
   return DIS_OK;
 }
+
+bool ctx_tune_get(dis_ctx* c, const int n[3], int axis, PassTune* t) {
+  auto it = c->tune.find({n[0], n[1], n[2], axis});
+  if (it == c->tune.end()) return false;
+  *t = it->second;
+  return true;
+}
+
+void ctx_tune_set(dis_ctx* c, const int n[3], int axis, const PassTune& t) { c->tune[{n[0], n[1], n[2], axis}] = t; }
+
+void ctx_tape_mode(dis_ctx* c, int mode) {
+  c->tape_mode = mode;
+  c->tape_pos = 0;
+}
+
+void ctx_tape_free(dis_ctx* c) {
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& b : c->tape) (void)hipFree(b.first);
+  c->tape.clear();
+  c->tape_pos = 0;
+  c->tape_mode = 0;
+}
+
+bool ctx_profiling(dis_ctx* c, int on) {
+  const bool was = c->prof;
+  if (on >= 0) c->prof = on != 0;
+  return was;
+}
+
+int ctx_device(dis_ctx* c) { return c->device; }
 
 int prof_begin(dis_ctx* c, int kclass) {
   if (!c->prof) return -1;
@@ -236,6 +288,7 @@ int dis_ctx_destroy(dis_ctx* c) {
   for (auto& kv : c->bufs)
     if (kv.second.first) (void)hipFree(kv.second.first);
   if (c->dcoef) (void)hipFree(c->dcoef);
+  for (auto& b : c->tape) (void)hipFree(b.first);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return DIS_OK;
@@ -502,6 +555,53 @@ extern "C" int dis_cg(dis_plan* p, const double* T, const void* y, const float* 
   if (nn) *nn = it;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DIS_OK;
+}
+
+extern "C" int dis_autotune(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], int32_t S, int32_t* choice,
+                            double* ms) {
+  if (!c) return DIS_E_INVALID;
+  if (!n || S <= 0) return fail(c, DIS_E_INVALID, "dis_autotune: invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int32_t* Nf = Nfull ? Nfull : n;
+  return autotune_grid(c, n, Nf, S, choice, ms);
+}
+
+extern "C" int dis_set_pass_shape(dis_ctx* c, const int32_t n[3], int32_t axis, int32_t generic, int32_t P,
+                                  int32_t maxradix) {
+  if (!c) return DIS_E_INVALID;
+  if (!n || axis < 0 || axis > 2 || P < 0 || (maxradix != 4 && maxradix != 8 && maxradix != 16))
+    return fail(c, DIS_E_INVALID, "dis_set_pass_shape: invalid argument");
+  if (P && (P & (P - 1))) return fail(c, DIS_E_INVALID, "dis_set_pass_shape: P must be a power of two");
+  PassTune t;
+  t.valid = 1;
+  t.generic = generic ? 1 : 0;
+  t.P = P;
+  t.maxradix = maxradix;
+  const int nn[3] = {n[0], n[1], n[2]};
+  ctx_tune_set(c, nn, axis, t);
+  return DIS_OK;
+}
+
+extern "C" int dis_autotune_clear(dis_ctx* c) {
+  if (!c) return DIS_E_INVALID;
+  c->tune.clear();
+  return DIS_OK;
+}
+
+extern "C" int dis_cg_batched(dis_ctx* c, int32_t nvol, dis_plan* const* plans, const double* const* T,
+                              const void* const* y, const float* const* Se, void* const* x, int32_t nIt,
+                              int32_t use_graph, double* res, double* ms) {
+  if (!c) return DIS_E_INVALID;
+  if (nvol <= 0 || !plans || !T || !y || !x || nIt < 0) return fail(c, DIS_E_INVALID, "dis_cg_batched: invalid argument");
+  std::vector<OpView> views(nvol);
+  for (int i = 0; i < nvol; ++i) {
+    if (!plans[i] || plans[i]->ctx != c || !y[i] || !x[i] || (plans[i]->g.nmotion > 0 && !T[i]))
+      return fail(c, DIS_E_INVALID, "dis_cg_batched: every plan must belong to the context and have y, x and T");
+    views[i] = plan_view(plans[i]);
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  return run_cg_batched(c, nvol, views.data(), T, (const float2* const*)y, Se, (float2* const*)x, nIt, use_graph != 0,
+                        res, ms);
 }
 
 extern "C" int dis_rigid_transform_grad(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], const double* T,

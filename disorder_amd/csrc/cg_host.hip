@@ -121,3 +121,178 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
 }
 
 }  // namespace dis
+
+namespace dis {
+
+// Per-volume CG state of the batched solver (device).
+struct CgVol {
+  float2 *r, *z, *p, *q;
+  CgScalars* sc;
+  double2 *part, *part2;
+  double* hist;
+  int* cnt;
+  long long n;
+  bool active;
+  void* base;
+};
+
+int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* const* T, const float2* const* y,
+                   const float* const* Se, float2* const* x, int nIt, bool use_graph, double* res, double* ms) {
+  hipStream_t st = ctx_stream(c);
+  std::vector<CgVol> vs(nvol);
+  auto release = [&]() {
+    (void)hipStreamSynchronize(st);
+    for (auto& v : vs)
+      if (v.base) (void)hipFree(v.base);
+    ctx_tape_free(c);
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr, e1b = nullptr, e2 = nullptr;
+  CCHK(c, hipEventCreate(&e0));
+  CCHK(c, hipEventCreate(&e1));
+  CCHK(c, hipEventCreate(&e1b));
+  CCHK(c, hipEventCreate(&e2));
+  auto destroy_events = [&]() {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e1b);
+    (void)hipEventDestroy(e2);
+  };
+  auto fail_all = [&](int rc) {
+    release();
+    destroy_events();
+    return rc;
+  };
+  CCHK(c, hipEventRecord(e0, st));
+  // setup: b = decode(y), x = 0, r = b, z = Se r, p = z (CGsolver.m:46-68)
+  for (int i = 0; i < nvol; ++i) {
+    CgVol& v = vs[i];
+    v.n = (long long)views[i].g.n[0] * views[i].g.n[1] * views[i].g.n[2];
+    const size_t vb = sizeof(float2) * v.n;
+    const size_t scal = 4 * vb, part = scal + 256, hist = part + 2 * sizeof(double2) * kCgPartials;
+    const size_t cnt = hist + sizeof(double) * (nIt + 1), bytes = cnt + 256;
+    if (hipMalloc(&v.base, bytes) != hipSuccess) {
+      v.base = nullptr;
+      (void)hipGetLastError();
+      return fail_all(ctx_fail(c, DIS_E_NOMEM, "batched CG: state allocation failed"));
+    }
+    v.r = (float2*)v.base;
+    v.z = v.r + v.n;
+    v.p = v.z + v.n;
+    v.q = v.p + v.n;
+    v.sc = (CgScalars*)((char*)v.base + scal);
+    v.part = (double2*)((char*)v.base + part);
+    v.part2 = v.part + kCgPartials;
+    v.hist = (double*)((char*)v.base + hist);
+    v.cnt = (int*)((char*)v.base + cnt);
+    int rc = op_apply(c, views[i], T[i], nullptr, y[i], nullptr, v.r, COIL_DECODE, nullptr);
+    if (rc) return fail_all(rc);
+    CCHK(c, hipMemsetAsync(x[i], 0, vb, st));
+    CCHK(c, hipMemsetAsync(v.cnt, 0, sizeof(int), st));
+    CCHK(c, cg_residual(v.r, nullptr, v.r, v.z, v.p, Se ? Se[i] : nullptr, v.n, v.part, v.part2, &v.sc->rsold,
+                        &v.sc->bnorm2, st));
+  }
+  std::vector<CgScalars> hs(nvol);
+  for (int i = 0; i < nvol; ++i) CCHK(c, hipMemcpyAsync(&hs[i], vs[i].sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st));
+  CCHK(c, hipStreamSynchronize(st));
+  int nact = 0;
+  for (int i = 0; i < nvol; ++i) {
+    vs[i].active = std::hypot(hs[i].rsold.x, hs[i].rsold.y) >= 1e-6;  // early termination (:70-75)
+    nact += vs[i].active;
+  }
+  CCHK(c, hipEventRecord(e1, st));
+  // one iteration of every active volume (CGsolver.m:80-97)
+  auto iteration = [&]() -> int {
+    for (int i = 0; i < nvol; ++i) {
+      CgVol& v = vs[i];
+      if (!v.active) continue;
+      int rc = op_apply(c, views[i], T[i], v.p, nullptr, nullptr, v.q, COIL_EHE, nullptr);
+      if (rc) return rc;
+      CCHK(c, cg_constrain_dot(v.q, v.p, nullptr, 0.f, v.n, v.part, &v.sc->pq, st));
+      CCHK(c, cg_step_guarded(v.sc, st));
+      CCHK(c, cg_update(x[i], v.r, v.z, v.p, v.q, Se ? Se[i] : nullptr, v.sc, v.n, v.part, v.part2, st));
+      CCHK(c, cg_direction(v.p, v.z, v.sc, v.n, st));
+      CCHK(c, cg_record(v.sc, v.hist, v.cnt, st));
+    }
+    return DIS_OK;
+  };
+  if (nact > 0 && nIt > 0) {
+    // iteration 1 records every coefficient upload on the tape
+    ctx_tape_mode(c, 1);
+    int rc = iteration();
+    if (rc) return fail_all(rc);
+    CCHK(c, hipStreamSynchronize(st));
+    CCHK(c, hipEventRecord(e1b, st));
+    if (nIt > 1) {
+      const bool prof = ctx_profiling(c, 0);
+      if (use_graph) {
+        ctx_tape_mode(c, 2);
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        CCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rc = iteration();
+        hipGraph_t gc = nullptr;
+        const hipError_t ec = hipStreamEndCapture(st, &gc);
+        g = gc;
+        if (rc || ec != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          ctx_profiling(c, prof ? 1 : 0);
+          return fail_all(rc ? rc : ctx_fail(c, DIS_E_HIP, "batched CG: graph capture failed"));
+        }
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+          (void)hipGraphDestroy(g);
+          ctx_profiling(c, prof ? 1 : 0);
+          return fail_all(ctx_fail(c, DIS_E_HIP, "batched CG: graph instantiation failed"));
+        }
+        for (int k = 2; k <= nIt; ++k) {
+          if (hipGraphLaunch(ge, st) != hipSuccess) {
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+            ctx_profiling(c, prof ? 1 : 0);
+            return fail_all(ctx_fail(c, DIS_E_HIP, "batched CG: graph launch failed"));
+          }
+        }
+        CCHK(c, hipStreamSynchronize(st));
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+      } else {
+        for (int k = 2; k <= nIt; ++k) {
+          ctx_tape_mode(c, 2);
+          rc = iteration();
+          if (rc) {
+            ctx_profiling(c, prof ? 1 : 0);
+            return fail_all(rc);
+          }
+        }
+      }
+      ctx_profiling(c, prof ? 1 : 0);
+    }
+  } else {
+    CCHK(c, hipEventRecord(e1b, st));
+  }
+  CCHK(c, hipEventRecord(e2, st));
+  CCHK(c, hipStreamSynchronize(st));
+  if (ms) {
+    float a = 0, b = 0, d = 0;
+    CCHK(c, hipEventElapsedTime(&a, e0, e1));
+    CCHK(c, hipEventElapsedTime(&b, e1, e1b));
+    CCHK(c, hipEventElapsedTime(&d, e1b, e2));
+    ms[0] = a;
+    ms[1] = b;
+    ms[2] = d;
+  }
+  if (res) {
+    for (int i = 0; i < nvol; ++i) {
+      std::vector<double> h(nIt, 0.0);
+      int cnt = 0;
+      CCHK(c, hipMemcpy(&cnt, vs[i].cnt, sizeof(int), hipMemcpyDeviceToHost));
+      if (cnt > 0) CCHK(c, hipMemcpy(h.data(), vs[i].hist, sizeof(double) * std::min(cnt, nIt), hipMemcpyDeviceToHost));
+      const double bn = hs[i].bnorm2.x;
+      for (int k = 0; k < nIt; ++k) res[(size_t)i * nIt + k] = (k < cnt && bn > 0) ? std::sqrt(h[k] / bn) : 0.0;
+    }
+  }
+  release();
+  destroy_events();
+  return DIS_OK;
+}
+
+}  // namespace dis

@@ -181,6 +181,21 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
 // g = rsold / pq   (CGsolver.m:82)
 __global__ void step_kernel(CgScalars* sc) { sc->g = cdiv(sc->rsold, sc->pq); }
 
+// Fixed-count iterations (no host test between them): once |rsold| < 1e-6
+// (the break of CGsolver.m:101) the step is zero, so x and r stay where the
+// loop would have stopped.
+__global__ void step_guarded_kernel(CgScalars* sc) {
+  const double2 rs = sc->rsold;
+  const bool stop = hypot(rs.x, rs.y) < 1e-6 || (sc->pq.x == 0.0 && sc->pq.y == 0.0);
+  sc->g = stop ? make_double2(0.0, 0.0) : cdiv(rs, sc->pq);
+}
+
+__global__ void record_kernel(const CgScalars* sc, double* hist, int* cnt) {
+  const int k = *cnt;
+  hist[k] = sc->zr_abs.x;
+  *cnt = k + 1;
+}
+
 // rs from the reduction; d = rs / (rsold + eps); rsold = rs  (CGsolver.m:94-97)
 __global__ void beta_kernel(CgScalars* sc) {
   const double2 rs = sc->rs;
@@ -227,6 +242,16 @@ hipError_t cg_residual(const float2* b, const float2* q, float2* r, float2* z, f
 
 hipError_t cg_step(CgScalars* sc, hipStream_t st) {
   hipLaunchKernelGGL(step_kernel, dim3(1), dim3(1), 0, st, sc);
+  return hipGetLastError();
+}
+
+hipError_t cg_step_guarded(CgScalars* sc, hipStream_t st) {
+  hipLaunchKernelGGL(step_guarded_kernel, dim3(1), dim3(1), 0, st, sc);
+  return hipGetLastError();
+}
+
+hipError_t cg_record(const CgScalars* sc, double* hist, int* cnt, hipStream_t st) {
+  hipLaunchKernelGGL(record_kernel, dim3(1), dim3(1), 0, st, sc, hist, cnt);
   return hipGetLastError();
 }
 

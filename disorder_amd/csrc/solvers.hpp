@@ -25,6 +25,9 @@ hipError_t cg_constrain_dot(float2* q, const float2* p, const float* Ma, float l
 hipError_t cg_residual(const float2* b, const float2* q, float2* r, float2* z, float2* p, const float* Se, long long n,
                        double2* partial, double2* partial_b, double2* rs_out, double2* bn_out, hipStream_t st);
 hipError_t cg_step(CgScalars* sc, hipStream_t st);
+hipError_t cg_step_guarded(CgScalars* sc, hipStream_t st);
+// hist[*cnt] = zr_abs; ++*cnt  (device-side residual history)
+hipError_t cg_record(const CgScalars* sc, double* hist, int* cnt, hipStream_t st);
 hipError_t cg_update(float2* x, float2* r, float2* z, const float2* p, const float2* q, const float* Se, CgScalars* sc,
                      long long n, double2* partial, double2* partial_abs, hipStream_t st);
 hipError_t cg_direction(float2* p, const float2* z, const CgScalars* sc, long long n, hipStream_t st);

@@ -149,8 +149,11 @@ PassArgs pass_args(const Grid& gr, const Step& stp, const unsigned char* dbase, 
   const int n = gr.n[stp.axis];
   int P = 1;
   while (P * 2 * n <= kMaxTile && P * 2 <= 512) P *= 2;
+  const PassTune& tu = gr.tune[stp.axis];
+  if (tu.valid && tu.P > 0 && tu.P < P) P = tu.P;
   if (stp.axis != 0 && P > gr.n[0]) P = gr.n[0];
   a.P = P;
+  a.force_generic = tu.valid ? tu.generic : 0;
   a.tiles_per_line = (gr.n[0] + P - 1) / P;
   a.tw = gr.tw[stp.axis];
   a.plan = gr.plan[stp.axis];
@@ -213,8 +216,9 @@ int run_steps(dis_ctx* c, const Grid& gr, const std::vector<Step>& steps, const 
 
 }  // namespace
 
-int make_plan(int n, FftPlan* pl) {
+int make_plan(int n, FftPlan* pl, int maxradix) {
   if (n <= 0 || n > 1024) return DIS_E_UNSUPPORTED;
+  const int lr = maxradix >= 16 ? 4 : maxradix >= 8 ? 3 : maxradix >= 4 ? 2 : 1;  // log2 of the largest radix
   pl->n = n;
   pl->nst = 0;
   int m = n;
@@ -232,13 +236,11 @@ int make_plan(int n, FftPlan* pl) {
   }
   if (m != 1) return DIS_E_UNSUPPORTED;
   std::vector<int> two;
-  while (a >= 4) {
-    two.push_back(16);
-    a -= 4;
+  while (a >= lr) {
+    two.push_back(1 << lr);
+    a -= lr;
   }
-  if (a == 3) two.push_back(8);
-  if (a == 2) two.push_back(4);
-  if (a == 1) two.push_back(2);
+  if (a > 0) two.push_back(1 << a);
   // powers of two first (the largest radix leads)
   std::vector<int> all(two.begin(), two.end());
   for (int s = 0; s < pl->nst; ++s) all.push_back(pl->radix[s]);
@@ -253,7 +255,11 @@ int make_grid(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], Grid* g) {
     if (n[d] <= 0 || Nfull[d] <= 0) return ctx_fail(c, DIS_E_INVALID, "grid sizes must be positive");
     g->n[d] = n[d];
     g->Nfull[d] = Nfull[d];
-    if (make_plan(n[d], &g->plan[d]) != DIS_OK)
+    g->tune[d] = PassTune{};
+    PassTune t;
+    const int nn[3] = {n[0], n[1], n[2]};
+    if (ctx_tune_get(c, nn, d, &t)) g->tune[d] = t;
+    if (make_plan(n[d], &g->plan[d], g->tune[d].valid ? g->tune[d].maxradix : 16) != DIS_OK)
       return ctx_fail(c, DIS_E_UNSUPPORTED, "grid size must be <= 1024 with prime factors 2, 3, 5, 7");
     int st = ctx_twiddles(c, n[d], &g->tw[d]);
     if (st) return st;
@@ -874,6 +880,86 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
   }
   for (int s = 0; s < NM; ++s) En_out[s] = std::min(EnPrev[s], En[s]);
   return DIS_OK;
+}
+
+
+int autotune_grid(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], int S, int32_t* out, double* ms_out) {
+  hipStream_t stream = ctx_stream(c);
+  const long long nv = (long long)n[0] * n[1] * n[2];
+  void* ws = nullptr;
+  int rc = ctx_workspace(c, sizeof(float2) * nv * S * 2, &ws);
+  if (rc) return rc;
+  DCHK(c, hipMemsetAsync(ws, 0, sizeof(float2) * nv * S * 2, stream));
+  float2* in = (float2*)ws;
+  float2* outv = in + nv * S;
+  hipEvent_t e0, e1;
+  DCHK(c, hipEventCreate(&e0));
+  DCHK(c, hipEventCreate(&e1));
+  const bool prof = ctx_profiling(c, 0);
+  const int nn[3] = {n[0], n[1], n[2]};
+  for (int axis = 0; axis < 3; ++axis) {
+    const int len = n[axis];
+    int Pmax = 1;
+    while (Pmax * 2 * len <= kMaxTile && Pmax * 2 <= 512) Pmax *= 2;
+    if (axis != 0) Pmax = std::min(Pmax, n[0]);
+    std::vector<PassTune> cands;
+    if (reg_radix(len) && (axis == 0 || n[0] >= 16)) {
+      PassTune t;
+      t.valid = 1;
+      cands.push_back(t);  // register path
+    }
+    for (int mr : {4, 8, 16})
+      for (int P = Pmax; P >= 1; P /= 2) {
+        PassTune t;
+        t.valid = 1;
+        t.generic = 1;
+        t.P = P;
+        t.maxradix = mr;
+        cands.push_back(t);
+        if (P <= Pmax / 16) break;  // tiles below 1/16 of the largest are never faster
+      }
+    double best = 1e30;
+    PassTune bt;
+    for (const PassTune& t : cands) {
+      ctx_tune_set(c, nn, axis, t);
+      Grid gr;
+      rc = make_grid(c, n, Nfull, &gr);
+      if (rc) break;
+      Step stp{};
+      stp.axis = axis;
+      stp.src = (axis + 1) % 3;
+      stp.flip_m = -1;
+      PassArgs a = pass_args(gr, stp, nullptr, S);
+      a.in = in;
+      a.in_sstride = nv;
+      a.out = outv;
+      a.out_sstride = nv;
+      DCHK(c, launch_pass(a, MODE_MAP, S, stream));  // warm-up
+      DCHK(c, hipEventRecord(e0, stream));
+      const int reps = 3;
+      for (int r = 0; r < reps; ++r) DCHK(c, launch_pass(a, MODE_MAP, S, stream));
+      DCHK(c, hipEventRecord(e1, stream));
+      DCHK(c, hipEventSynchronize(e1));
+      float ms = 0;
+      DCHK(c, hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < best) {
+        best = ms / reps;
+        bt = t;
+      }
+    }
+    if (rc) break;
+    ctx_tune_set(c, nn, axis, bt);
+    if (out) {
+      out[3 * axis] = bt.generic;
+      out[3 * axis + 1] = bt.generic ? bt.P : 16;
+      out[3 * axis + 2] = bt.generic ? bt.maxradix : reg_radix(len);
+    }
+    if (ms_out) ms_out[axis] = best;
+  }
+  ctx_profiling(c, prof ? 1 : 0);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 }  // namespace dis

@@ -11,6 +11,14 @@
 
 namespace dis {
 
+// Tuned launch shape of the pencil passes along one axis of one grid.
+struct PassTune {
+  int valid = 0;
+  int generic = 0;   // LDS Stockham engine even where the register path exists
+  int P = 0;         // pencils per workgroup tile (generic engine)
+  int maxradix = 16; // largest power-of-two radix of the generic plan
+};
+
 // Level grid with its FFT plans and twiddle tables.
 struct Grid {
   int n[3];
@@ -18,6 +26,7 @@ struct Grid {
   long long nvox;
   FftPlan plan[3];
   const float2* tw[3];
+  PassTune tune[3];
 };
 
 // Device-side view of a plan (sampling + coil maps).
@@ -60,12 +69,26 @@ size_t ctx_workspace_limit(dis_ctx* c);
 // Named device buffers that outlive one call (slot 0 is the op workspace).
 int ctx_buffer(dis_ctx* c, int slot, size_t bytes, void** out);
 int ctx_upload_bytes(dis_ctx* c, const Staging& st, const unsigned char** dptr);
+// Upload tape (0 off, 1 record, 2 replay from the start) and its release.
+void ctx_tape_mode(dis_ctx* c, int mode);
+void ctx_tape_free(dis_ctx* c);
+// Switch event profiling on (1) / off (0) / leave it (-1); returns the old state.
+bool ctx_profiling(dis_ctx* c, int on);
+int ctx_device(dis_ctx* c);
 
 // Event timing around a launch when profiling is enabled (no-ops otherwise).
 int prof_begin(dis_ctx* c, int kclass);  // returns a slot or -1
 void prof_end(dis_ctx* c, int slot, double bytes);
 
-int make_plan(int n, FftPlan* pl);
+int make_plan(int n, FftPlan* pl, int maxradix = 16);
+// Tuned pass shapes per (n1, n2, n3, axis), kept by the context.
+bool ctx_tune_get(dis_ctx* c, const int n[3], int axis, PassTune* t);
+void ctx_tune_set(dis_ctx* c, const int n[3], int axis, const PassTune& t);
+// Time the candidate pass shapes of every axis of a grid on S states and
+// keep the fastest (register path; LDS engine with max radix 4/8/16 and
+// 1..Pmax pencils per tile).  out[3*axis + {0,1,2}] = generic, P, maxradix;
+// ms_out[3] (or null) the chosen shapes' times.
+int autotune_grid(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], int S, int32_t* out, double* ms_out);
 int make_grid(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], Grid* g);
 
 // T: host [Sld x 6] column-major; the states transformed are [0, S).
@@ -102,6 +125,14 @@ int lm_partials(dis_ctx* c, const OpView& v, const double* T, const float2* x, c
 int coil_gram(dis_ctx* c, const float2* S, long long nv, int C, double2* P_dev_out);
 int coil_apply(dis_ctx* c, const float* Bm_dev, int Cc, int C, const float2* in, long long nv, long long in_cs,
                float2* out, long long out_cs, int terms);
+
+// Batched CG over independent problems (x0 = 0, nIt iterations of
+// 'RelativeResidual2Error' with tol 0); iterations 2..nIt are replayed from
+// a captured hipGraph when use_graph.  res [nvol x nIt] (or null) receives
+// sqrt(sum|conj(z) r| / bnorm2) after each iteration; ms[3] (or null) the
+// setup, first (recording) iteration and remaining iterations times.
+int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* const* T, const float2* const* y,
+                   const float* const* Se, float2* const* x, int nIt, bool use_graph, double* res, double* ms);
 
 int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const float* Se, const float* Ma,
            double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn);

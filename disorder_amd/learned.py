@@ -1,0 +1,275 @@
+"""Learned motion proposal for the joint reconstruction.
+
+A small network proposes per-shot motion updates from the shots'
+sincRigidTransformGradient-based features at the current estimate; the
+proposal of a shot is kept only where computeEnergy of that shot decreases,
+and LMsolver (Reconstruction/LMsolver.m:1) runs as before on every shot whose
+proposal was rejected, so the plain method stays the fallback and the judge
+of the error.
+
+Features per shot (13): from the LM partial sums at the current (x, T)
+(Reconstruction/LMsolver.m:82-115, via dis_lm_partials):
+  [0:6]   diagonal Newton step -g_i / H_ii (translations in voxels,
+          rotations in degrees),
+  [6:12]  log(H_ii / mean over shots of H_ii),
+  [12]    log(residual energy / mean over shots).
+Target: T_true - T_current in the same units.
+
+Network, training and inference run in libdisorder_hip.so
+(dis_mlp_forward / dis_mlp_backward / dis_mlp_train_step): MLP
+13 -> 256 -> 256 -> 6 (GELU, BF16 MFMA products, FP32 weights), then a 1D
+convolution along shot order (kernel 5, 6 channels); Adam.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+import torch
+
+from . import _lib, ops, synth
+from ._lib import check
+
+NIN, NHID, NOUT, NK = 13, 256, 6, 5
+DEG = 180.0 / math.pi
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "motion_net.npy")
+
+
+def _fp(t):
+    if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+        raise TypeError("contiguous float32 device tensors are required")
+    return ctypes.cast(ctypes.c_void_p(t.data_ptr()), _lib.c_float_p)
+
+
+def _offs(scan_off):
+    so = np.ascontiguousarray(scan_off, dtype=np.int32)
+    return so, so.ctypes.data_as(_lib.c_int32_p), len(so) - 1
+
+
+class MotionNet:
+    """Parameters and Adam state on one device."""
+
+    def __init__(self, params=None, seed=0, ctx: ops.Context = None, device=0):
+        self.ctx = ctx or ops.default_context(device)
+        self.lib = self.ctx.lib
+        self.n = self.lib.dis_mlp_param_count()
+        dev = self.ctx.device
+        if params is None:
+            params = self.init_params(seed)
+        p = np.asarray(params, np.float32).ravel()
+        if p.size != self.n:
+            raise ValueError(f"expected {self.n} parameters, got {p.size}")
+        self.p = torch.from_numpy(p.copy()).to(dev)
+        self.m = torch.zeros_like(self.p)
+        self.v = torch.zeros_like(self.p)
+        self.step = 0
+
+    @staticmethod
+    def init_params(seed):
+        """Uniform(+-1/sqrt(fan_in)) weights, zero biases; the convolution
+        starts as the identity on its centre tap."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for fo, fi in ((NHID, NIN), (NHID, NHID), (NOUT, NHID)):
+            b = 1.0 / math.sqrt(fi)
+            parts += [rng.uniform(-b, b, (fo, fi)).ravel(), np.zeros(fo)]
+        Wc = rng.normal(0.0, 0.01, (NOUT, NOUT, NK))
+        Wc[np.arange(NOUT), np.arange(NOUT), NK // 2] += 1.0
+        parts += [Wc.ravel(), np.zeros(NOUT)]
+        return np.concatenate(parts).astype(np.float32)
+
+    @classmethod
+    def load(cls, path=WEIGHTS, ctx=None, device=0):
+        p = np.load(path, allow_pickle=False)
+        return cls(p, ctx=ctx, device=device)
+
+    def save(self, path=WEIGHTS):
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.save(path, self.p.cpu().numpy())
+
+    def forward(self, X, scan_off):
+        X = X.contiguous()
+        M = int(X.shape[0])
+        Y = torch.empty((M, NOUT), dtype=torch.float32, device=X.device)
+        so, sp, nb = _offs(scan_off)
+        self.ctx._prep()
+        check(self.lib.dis_mlp_forward(self.ctx.h, _fp(self.p), _fp(X), M, sp, nb, _fp(Y)), self.ctx.h)
+        return Y
+
+    def backward(self, X, scan_off, dY):
+        X = X.contiguous()
+        dY = dY.contiguous()
+        M = int(X.shape[0])
+        G = torch.empty_like(self.p)
+        dX = torch.empty_like(X)
+        so, sp, nb = _offs(scan_off)
+        self.ctx._prep()
+        check(self.lib.dis_mlp_backward(self.ctx.h, _fp(self.p), _fp(X), M, sp, nb, _fp(dY), _fp(G), _fp(dX)),
+              self.ctx.h)
+        return G, dX
+
+    def train_step(self, X, target, scan_off, lr=1e-3):
+        self.step += 1
+        loss = ctypes.c_float(0.0)
+        so, sp, nb = _offs(scan_off)
+        self.ctx._prep()
+        check(self.lib.dis_mlp_train_step(self.ctx.h, _fp(self.p), _fp(self.m), _fp(self.v), self.step, float(lr),
+                                          _fp(X.contiguous()), _fp(target.contiguous()), int(X.shape[0]), sp, nb,
+                                          ctypes.byref(loss)), self.ctx.h)
+        return loss.value
+
+
+def features(sums):
+    """[S x 28] LM partial sums (energy, 6 gradient, 21 Hessian entries with
+    the diagonal first) -> [S x 13] float32 features."""
+    s = np.asarray(sums, np.float64)
+    en, g, hd = s[:, 0], s[:, 1:7], s[:, 7:13]
+    tiny = 1e-30
+    hd = np.maximum(hd, tiny)
+    F = np.zeros((s.shape[0], NIN))
+    step = -g / hd
+    step[:, 3:] *= DEG
+    F[:, 0:6] = step
+    F[:, 6:12] = np.log(hd / np.maximum(hd.mean(axis=0, keepdims=True), tiny))
+    F[:, 12] = np.log(np.maximum(en, tiny) / max(en.mean(), tiny))
+    return np.nan_to_num(F, nan=0.0, posinf=0.0, neginf=0.0).astype(np.float32)
+
+
+def to_update(Y):
+    """Network output (vox, deg) -> motion update (vox, rad)."""
+    d = np.asarray(Y, np.float64).copy()
+    d[:, 3:] /= DEG
+    return d
+
+
+def lm_sums(y, E, x, T):
+    """Raw per-state LM partial sums [S x 28] (dis_lm_partials)."""
+    Tf, Tp = E._T(T)
+    sums = np.zeros((E.nmotion, 28), np.float64)
+    E.ctx._prep()
+    check(E.ctx.lib.dis_lm_partials(E.h, Tp, ops._ptr(x), ops._ptr(y), sums.ctypes.data_as(_lib.c_double_p)),
+          E.ctx.h)
+    return sums
+
+
+def state_energy(E, x, y, T, off):
+    ps = E.energy(x, y, T, per_sample=True)
+    return np.add.reduceat(np.asarray(ps, np.float64), off[:-1]) if len(ps) else np.zeros(len(off) - 1)
+
+
+class Proposer:
+    """Per-iteration proposal step for pyramid.solveXT: returns the motion
+    after the proposal and the mask of states that took it (their energy
+    decreased); LMsolver then handles the other states."""
+
+    def __init__(self, net: MotionNet):
+        self.net = net
+        self.accepted = 0
+        self.proposed = 0
+
+    def __call__(self, L, x, T, cT):
+        active = ~cT
+        if not active.any():
+            return T, np.zeros(len(T), bool)
+        sums = lm_sums(L.y, L.E, x, T)
+        # translations in voxels of the level grid (the network is trained at
+        # the native resolution of its scans)
+        scale = max(Nf / n for Nf, n in zip(L.E.Nfull, L.n))
+        Fh = features(sums)
+        Fh[:, 0:3] /= scale
+        F = torch.from_numpy(Fh).to(x.device)
+        Y = self.net.forward(F, [0, F.shape[0]]).cpu().numpy()
+        Y[:, 0:3] *= scale
+        Tp = T + to_update(Y)
+        Tp[:, 3:] = (Tp[:, 3:] + math.pi) % (2 * math.pi) - math.pi  # restrictTransform.m:28-29
+        e_new = state_energy(L.E, x, L.y, Tp, L.off)
+        take = active & (e_new < sums[:, 0])
+        self.proposed += int(active.sum())
+        self.accepted += int(take.sum())
+        Tn = np.where(take[:, None], Tp, T)
+        return Tn, take
+
+
+# ------------------------------------------------------------------ training data
+def simulate_scan(rng, n, S, C, dev, snr_db=30.0):
+    """One simulated scan (the method's own synthesis): phantom, coils,
+    DISORDER sampling, random-walk motion with step sigma U(0.2, 1.5) deg /
+    voxel plus 5 % abrupt jumps up to 10 deg / 10 voxels, noise."""
+    seed = int(rng.integers(1 << 30))
+    t1 = int(2 ** math.floor(math.log2(math.sqrt(S))))
+    t2 = S // t1
+    _, k1, k2, off = synth.disorder_sampling(n[0], n[1], S, t1, t2, seed=seed)
+    x = synth.phantom(n, int(rng.integers(6, 16)), seed=seed + 1, device=dev)
+    Sm = synth.coil_maps(n, C, seed=seed + 2, device=dev)
+    step_deg, step_vox = rng.uniform(0.2, 1.5), rng.uniform(0.2, 1.5)
+    T = np.zeros((S, 6))
+    for s in range(1, S):
+        T[s, :3] = T[s - 1, :3] + rng.normal(0, step_vox, 3)
+        T[s, 3:] = T[s - 1, 3:] + rng.normal(0, step_deg, 3) * math.pi / 180
+        if rng.uniform() < 0.05:
+            T[s, :3] += rng.uniform(-10, 10, 3)
+            T[s, 3:] += rng.uniform(-10, 10, 3) * math.pi / 180
+    T -= T.mean(axis=0, keepdims=True)
+    E = ops.Encoding(Sm, k1, k2, off)
+    y = E.encode(x, T)
+    y = synth.add_noise(y, synth.noise_level(x, Sm, snr_db), seed=seed + 3)
+    return E, x, y, T, off, Sm
+
+
+def training_examples(rng, E, x, y, T_true, off, n_cg=8, n_states=3):
+    """Features and targets at n_states current estimates, with the image
+    from n_cg CG iterations at that estimate (what the proposal sees inside
+    solveXT).  70 % of the estimates sit near the solution (per-shot errors
+    N(0, s) plus a slow walk, s ~ U(0.05, 0.6) voxel / degree: the later LM
+    iterations of a level), 30 % are far (a T_true + walk, a ~ U(0, 1): the
+    first iterations)."""
+    S = T_true.shape[0]
+    out = []
+    for _ in range(n_states):
+        if rng.uniform() < 0.7:
+            sig = rng.uniform(0.05, 0.6)
+            e = rng.normal(0, sig, (S, 6)) + np.cumsum(rng.normal(0, sig / 4, (S, 6)), axis=0)
+            e[:, 3:] *= math.pi / 180
+            Tc = T_true + e
+        else:
+            e = np.cumsum(rng.normal(0, 0.2, (S, 6)), axis=0)
+            e[:, 3:] *= math.pi / 180
+            Tc = rng.uniform(0.0, 1.0) * T_true + e
+        xc, _, _ = ops.CGsolver(y, E, Tc, x=None, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0)
+        sums = lm_sums(y, E, xc, Tc)
+        F = features(sums)
+        tgt = T_true - Tc
+        tgt[:, 3:] *= DEG
+        out.append((F, tgt.astype(np.float32)))
+    return out
+
+
+def train(net: MotionNet, data, epochs=30, batch=512, lr=1e-3, seed=0, log=None):
+    """Adam on the L2 loss to the true update.  data: list of (F [S x 13],
+    target [S x 6]) scans; a batch holds whole scans up to `batch` rows."""
+    rng = np.random.default_rng(seed)
+    dev = net.p.device
+    losses = []
+    for ep in range(epochs):
+        order = rng.permutation(len(data))
+        i = 0
+        tot, cnt = 0.0, 0
+        while i < len(order):
+            Fs, Ts, so = [], [], [0]
+            while i < len(order) and (so[-1] == 0 or so[-1] + data[order[i]][0].shape[0] <= batch):
+                F, t = data[order[i]]
+                Fs.append(F)
+                Ts.append(t)
+                so.append(so[-1] + F.shape[0])
+                i += 1
+            X = torch.from_numpy(np.concatenate(Fs)).to(dev)
+            Tg = torch.from_numpy(np.concatenate(Ts)).to(dev)
+            loss = net.train_step(X, Tg, so, lr)
+            tot += loss * so[-1]
+            cnt += so[-1]
+        losses.append(tot / max(cnt, 1))
+        if log is not None:
+            log(ep, losses[-1])
+    return losses

@@ -180,6 +180,7 @@ class Encoding:
         n = tuple(Smaps.shape[:3])
         C = Smaps.shape[3]
         self.n = n
+        self.Nfull = tuple(int(v) for v in Nfull) if Nfull is not None else n
         self.C = C
         self.k1 = np.ascontiguousarray(k1, dtype=np.int32)
         self.k2 = np.ascontiguousarray(k2, dtype=np.int32)
@@ -289,6 +290,42 @@ def CGsolver(y, E: "Encoding", T, x=None, nIt=300, tolType="Energy", tol=1e-5, S
                            tt, float(tol), En.ctypes.data_as(_lib.c_double_p), ctypes.byref(nEn), ctypes.byref(nn)),
           E.ctx.h)
     return xo, En[:2 * nEn.value].reshape(-1, 2).T.copy(), nn.value
+
+
+def autotune(n, Nfull=None, S=16, ctx: Context = None):
+    """Pick the pencil-pass shape of every axis of grid n by timing the
+    candidates (dis_autotune).  Returns [{generic, P, radix, ms}] per axis."""
+    ctx = ctx or default_context()
+    a = (ctypes.c_int32 * 3)(*[int(v) for v in n])
+    b = (ctypes.c_int32 * 3)(*[int(v) for v in (Nfull or n)])
+    ch = (ctypes.c_int32 * 9)()
+    ms = (ctypes.c_double * 3)()
+    check(ctx.lib.dis_autotune(ctx.h, a, b, int(S), ch, ms), ctx.h)
+    return [{"generic": bool(ch[3 * d]), "P": ch[3 * d + 1], "radix": ch[3 * d + 2], "ms": ms[d]} for d in range(3)]
+
+
+def CGsolver_batched(ys, Es, Ts, Ses=None, nIt=12, use_graph=True, ctx: Context = None):
+    """Batched CG over independent reconstructions (dis_cg_batched): volume i
+    solves E_i^H E_i x = E_i^H y_i with preconditioner Ses[i], x0 = 0, exactly
+    nIt iterations ('RelativeResidual2Error', tol 0).  Returns (xs,
+    res [nvol x nIt], ms [setup, first iteration, remaining iterations])."""
+    ctx = ctx or Es[0].ctx
+    nv = len(Es)
+    xs = [empty_colmajor(E.n, ys[i].device) for i, E in enumerate(Es)]
+    Tb = [E._T(T) for E, T in zip(Es, Ts)]
+    plans = (ctypes.c_void_p * nv)(*[E.h.value for E in Es])
+    Tp = (_lib.c_double_p * nv)(*[tp for _, tp in Tb])
+    yp = (ctypes.c_void_p * nv)(*[_ptr(y).value for y in ys])
+    xp = (ctypes.c_void_p * nv)(*[_ptr(x).value for x in xs])
+    sp = None
+    if Ses is not None:
+        sp = (_lib.c_float_p * nv)(*[_fptr(s) if s is not None else None for s in Ses])
+    res = np.zeros((nv, max(1, nIt)), np.float64)
+    ms = np.zeros(3, np.float64)
+    ctx._prep()
+    check(ctx.lib.dis_cg_batched(ctx.h, nv, plans, Tp, yp, sp, xp, int(nIt), 1 if use_graph else 0,
+                                 res.ctypes.data_as(_lib.c_double_p), ms.ctypes.data_as(_lib.c_double_p)), ctx.h)
+    return xs, res[:, :nIt], ms
 
 
 def lm_partials(y, E: "Encoding", x, T):

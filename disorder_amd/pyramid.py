@@ -61,12 +61,14 @@ class Level:
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None):
     """Returns (x [N] on the device, T [S x 6], info).  callback(level, it, x,
-    T, t_solve) runs after every outer iteration outside the timed region
+    T, t_solve) runs after every outer iteration (and after a level's final
+    CG, with outer = last + 1) outside the timed region
     (t_solve: solve seconds so far; callback time is reported in
     info['callback_s'] and excluded from info['solve_s']).
-    proposer(level_ops, x, T, w, cT) -> T' may propose motion before each LM
-    call; LM starts from the proposal only where it lowers the per-state
-    energy (computeEnergy), so the plain method stays the fallback."""
+    proposer(level_ops, x, T, cT) -> (T', take) may propose motion before
+    each LM call: states in `take` kept a proposal that lowered their energy
+    (computeEnergy) and skip this LM call; LMsolver runs on the others, so
+    the plain method stays the fallback."""
     import torch
     ctx = ctx or ops.default_context(y.device.index or 0)
     S_states = len(off) - 1
@@ -96,15 +98,15 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             cont += 1
             x, _, _ = ops.CGsolver(L.y, L.E, T, x=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0, Se=L.Se)
             if proposer is not None:
-                Tp = proposer(L, x, T, w, cT)
-                if Tp is not None:
-                    e_old = L.E.energy(x, L.y, T, per_sample=True)
-                    e_new = L.E.energy(x, L.y, Tp, per_sample=True)
-                    so, sn = _per_state(e_old, L.off), _per_state(e_new, L.off)
-                    take = (sn < so) & ~cT
-                    T = np.where(take[:, None], Tp, T)
-                    info["proposals_accepted"] += int(take.sum())
-            T, w, cT, _, ntr = ops.LMsolver(L.y, L.E, x, T, w, cT, winit=winit, tL=tL)
+                T_before = T.copy()
+                T, take = proposer(L, x, T, cT)
+                info["proposals_accepted"] += int(take.sum())
+                T, w, cT_lm, _, ntr = ops.LMsolver(L.y, L.E, x, T, w, cT | take, winit=winit, tL=tL)
+                d = np.abs(T - T_before)
+                small = (d[:, :3].max(axis=1) < tL[0]) & (d[:, 3:].max(axis=1) * 180 / math.pi < tL[1])
+                cT = np.where(take, small, cT_lm)
+            else:
+                T, w, cT, _, ntr = ops.LMsolver(L.y, L.E, x, T, w, cT, winit=winit, tL=tL)
             info["lm_trials"] += ntr
             its = it
             if callback is not None:
@@ -115,6 +117,11 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             if cT.all():
                 if final_cg and it < n_outer:
                     x, _, _ = ops.CGsolver(L.y, L.E, T, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=L.Se)
+                    if callback is not None:
+                        torch.cuda.synchronize()
+                        tc = time.perf_counter()
+                        callback(li, it + 1, x, T, tc - t0 - cb_time)
+                        cb_time += time.perf_counter() - tc
                 break
         info["levels"].append({"n": n, "outer": its, "states_converged": int(cT.sum())})
         L.E.close()

@@ -129,3 +129,53 @@ def add_noise(y, sigma, seed):
     # torch's complex randn has variance 1 split over re/im; plugNoise draws
     # N(0,1) for each part
     return colmajor(y + sigma * math.sqrt(2.0) * nz)
+
+
+def truncate_shots(H, frac, seed, keep=(0.3, 0.8)):
+    """Uneven shots: a fraction `frac` of the shots, chosen at random, lose the
+    tail of their samples (they keep a U(keep) share, in column-major order);
+    the dropped positions become unsampled (H = 0)."""
+    rng = np.random.default_rng(seed)
+    H = np.array(H, copy=True)
+    S = int(H.max())
+    nt = max(1, int(round(frac * S)))
+    for s in rng.choice(np.arange(1, S + 1), nt, replace=False):
+        idx = np.nonzero(H.reshape(-1, order="F") == s)[0]
+        k = int(round(rng.uniform(*keep) * len(idx)))
+        drop = idx[k:]
+        Hf = H.reshape(-1, order="F")
+        Hf[drop] = 0
+        H = Hf.reshape(H.shape, order="F")
+    return H
+
+
+def batched_volumes():
+    """The batched form's six volumes [PE1, PE2, RO] and shot tilings (t1 x t2
+    rectangular DISORDER tiles, 40-96 shots)."""
+    return [((160, 120, 160), (5, 8)), ((192, 144, 192), (8, 9)), ((192, 144, 192), (8, 12)),
+            ((224, 160, 224), (7, 8)), ((256, 192, 256), (8, 8)), ((256, 192, 256), (8, 6))]
+
+
+def batched_problem(dev, C=8, seed=0, volumes=None, trunc=0.1, snr_db=25.0):
+    """Six independent seeded reconstructions of uneven sizes: phantom, coils,
+    rectangular-tile DISORDER sampling with 10 % of shots truncated,
+    random-walk motion, noisy data.  Returns a list of dicts."""
+    from . import ops
+    out = []
+    for i, (n, (t1, t2)) in enumerate(volumes or batched_volumes()):
+        sd = seed + 100 * i
+        S = t1 * t2
+        rng = np.random.default_rng(sd)
+        perms = sampling.permutation_table(rng, (n[0] // t1) * (n[1] // t2), S)
+        H = sampling.random_checkered(n[0], n[1], t1, t2, perms)
+        H = truncate_shots(H, trunc, sd + 1)
+        k1, k2, off = sampling.harmonic_sampling(H, S)
+        x = phantom(n, 40, seed=sd + 2, device=dev)
+        Sm = coil_maps(n, C, seed=sd + 3, device=dev)
+        T = random_walk_motion(S, seed=sd + 4)
+        E = ops.Encoding(Sm, k1, k2, off)
+        y = E.encode(x, T)
+        y = add_noise(y, noise_level(x, Sm, snr_db), seed=sd + 5)
+        out.append({"n": n, "S": S, "tiles": (t1, t2), "E": E, "x_gt": x, "y": y, "T": T, "Smaps": Sm,
+                    "Se": ops.coil_precond(Sm), "lines": np.diff(off)})
+    return out

@@ -123,6 +123,38 @@ int dis_energy(dis_plan* plan, const double* T, const void* x, const void* y, co
 int dis_cg(dis_plan* plan, const double* T, const void* y, const float* Se, const float* Ma, double lambda, void* x,
            int32_t x_init, int32_t nIt, int32_t tolType, double tol, double* En, int32_t* nEn, int32_t* nn);
 
+/* Several independent reconstructions in one batched CG (replaces nvol calls
+ * of Reconstruction/CGsolver.m:1 with A.Se, no mask, no regulariser,
+ * tolType 'RelativeResidual2Error', tol 0 and x0 = 0, i.e. exactly nIt
+ * iterations, as solveXT.m:431 runs them with motion).  plans[i] (all on
+ * ctx), T[i] (host [nmotion_i x 6]), y[i], Se[i] (DEVICE, Se or NULL) and
+ * x[i] (DEVICE output) describe volume i; sizes may differ.  Iteration 1 runs
+ * eagerly and records every coefficient upload; with use_graph, iterations
+ * 2..nIt of all volumes are one captured hipGraph replayed nIt-1 times (no
+ * host work between iterations), otherwise they are launched eagerly from the
+ * recording.  A volume whose CGsolver.m:101 break condition (|rs| < 1e-6)
+ * is met keeps its iterate from then on.  res (host [nvol x nIt] or NULL)
+ * receives sqrt(sum|conj(z) r| / sum|conj(Se b) b|) after each iteration;
+ * ms (host [3] or NULL): setup, first iteration, remaining iterations. */
+int dis_cg_batched(dis_ctx* ctx, int32_t nvol, dis_plan* const* plans, const double* const* T, const void* const* y,
+                   const float* const* Se, void* const* x, int32_t nIt, int32_t use_graph, double* res, double* ms);
+
+/* Autotuner (no counterpart in the reference, which has no kernels to tune):
+ * for every axis of the grid n (motion grid Nfull, or n when NULL) times the
+ * fused pencil pass on S states for each candidate shape -- the register
+ * path where it exists, and the LDS Stockham engine with largest radix 4, 8
+ * or 16 and 1..Pmax pencils per workgroup tile (LDS tile = P x n) -- and
+ * keeps the fastest in the context; every later call on that grid uses it.
+ * choice (host [9] or NULL): per axis {generic, pencils per tile, radix};
+ * ms (host [3] or NULL): the chosen pass times. */
+int dis_autotune(dis_ctx* ctx, const int32_t n[3], const int32_t Nfull[3], int32_t S, int32_t* choice, double* ms);
+/* Set the pass shape of one axis of grid n by hand (generic: LDS engine;
+ * P: pencils per tile, a power of two, 0 = largest that fits; maxradix 4, 8
+ * or 16). */
+int dis_set_pass_shape(dis_ctx* ctx, const int32_t n[3], int32_t axis, int32_t generic, int32_t P, int32_t maxradix);
+/* Forget every tuned shape (back to the built-in choice). */
+int dis_autotune_clear(dis_ctx* ctx);
+
 /* sincRigidTransformGradient (Methods/sincRigidTransformGradient.m:1) with
  * the factors of precomputeFactorsSincRigidTransform(kG,rkG,T,1,1,1,1)
  * (Reconstruction/LMsolver.m:60-66,96): x is one volume [n]; Tx (or NULL)
@@ -202,6 +234,29 @@ int dis_gather_samples(dis_ctx* ctx, const void* y, int32_t Ns_in, int64_t ncol,
 /* CG preconditioner Se = 1/(sum_c |S_c|^2 + eps) per voxel
  * (Reconstruction/solveXT.m:231-232); S [nv x C] device, Se [nv] device float. */
 int dis_coil_precond(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, float eps, float* Se);
+
+/* ---- Learned motion proposal (no counterpart in the reference; proposes
+ * per-shot motion updates that Reconstruction/LMsolver.m then accepts or
+ * rejects).  Network per shot row: MLP 13 -> 256 -> 256 -> 6 with GELU,
+ * then a 1D convolution along shot order (kernel 5, 6 -> 6 channels, zero
+ * padding inside each scan).  Parameters are one flat FP32 device buffer of
+ * dis_mlp_param_count() floats: W1 [256x13], b1, W2 [256x256], b2, W3 [6x256],
+ * b3, Wc [6x6x5] (out, in, tap), bc.  Matrix products are BF16 MFMA with FP32
+ * accumulation.  X [M x 13], Y/target [M x 6] are row-major FP32 device
+ * arrays; scan_off [nscan+1] (host) delimits the scans (runs of rows in shot
+ * order). */
+int dis_mlp_param_count(void);
+int dis_mlp_forward(dis_ctx* ctx, const float* params, const float* X, int32_t M, const int32_t* scan_off,
+                    int32_t nscan, float* Y);
+/* Gradients of sum(dY .* Y) with respect to every parameter (grads, flat like
+ * params) and to X (dX [M x 13], or NULL). */
+int dis_mlp_backward(dis_ctx* ctx, const float* params, const float* X, int32_t M, const int32_t* scan_off,
+                     int32_t nscan, const float* dY, float* grads, float* dX);
+/* One Adam step (beta 0.9/0.999, eps 1e-8, bias-corrected, step >= 1) on the
+ * mean squared error between Y and target; loss (host, or NULL) receives it. */
+int dis_mlp_train_step(dis_ctx* ctx, float* params, float* adam_m, float* adam_v, int32_t step, float lr,
+                       const float* X, const float* target, int32_t M, const int32_t* scan_off, int32_t nscan,
+                       float* loss);
 
 /* Kernel classes for dis_profile_read. */
 #define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */

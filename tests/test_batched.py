@@ -1,0 +1,89 @@
+"""Batched CG over independent reconstructions of uneven sizes with
+hipGraph-captured iterations (dis_cg_batched).
+
+GPU: the batched solver (graph and eager replay) against one dis_cg call per
+volume with the same settings ('RelativeResidual2Error', tol 0, x0 = 0):
+the same kernels run on the same data, so the results are bit-identical;
+and one volume against the FP64 oracle's PCG (rel-L2 <= 1e-5).  The graph is
+replayed in two separate calls with eager work in between, each checked."""
+import numpy as np
+import pytest
+
+from disorder_amd import sampling as SA
+from oracle import disorder_oracle as O
+from tests.helpers import rel_l2
+
+VOLS = [((16, 12, 10), (4, 3)), ((20, 16, 12), (4, 4)), ((24, 18, 16), (2, 3))]
+
+
+def test_truncate_shots_is_uneven_and_disjoint():
+    from disorder_amd import synth
+    rng = np.random.default_rng(0)
+    perms = SA.permutation_table(rng, 4 * 4, 16)
+    H = SA.random_checkered(16, 16, 4, 4, perms)
+    Ht = synth.truncate_shots(H, 0.25, 3)
+    counts = np.bincount(Ht.ravel(), minlength=17)[1:]
+    assert (counts < 16).sum() == 4 and (counts == 16).sum() == 12
+    assert np.all((Ht == H) | (Ht == 0))
+
+
+@pytest.mark.gpu
+def test_batched_matches_single_and_oracle(dev):
+    import disorder_amd as D
+    from disorder_amd import synth
+    P = synth.batched_problem(dev, C=4, seed=3, volumes=VOLS, trunc=0.25)
+    Es = [p["E"] for p in P]
+    ys = [p["y"] for p in P]
+    Ts = [p["T"] for p in P]
+    Ses = [p["Se"] for p in P]
+    nIt = 6
+    xg, resg, msg = D.CGsolver_batched(ys, Es, Ts, Ses, nIt=nIt, use_graph=True)
+    # eager work between the two graph runs
+    singles = [D.CGsolver(y, E, T, x=None, nIt=nIt, tolType="RelativeResidual2Error", tol=0.0, Se=Se)[0]
+               for y, E, T, Se in zip(ys, Es, Ts, Ses)]
+    xg2, resg2, _ = D.CGsolver_batched(ys, Es, Ts, Ses, nIt=nIt, use_graph=True)
+    xe, rese, _ = D.CGsolver_batched(ys, Es, Ts, Ses, nIt=nIt, use_graph=False)
+    for i in range(len(P)):
+        a, b, c, s = (t.cpu().numpy() for t in (xg[i], xg2[i], xe[i], singles[i]))
+        assert np.array_equal(a, s), (i, np.abs(a - s).max())
+        assert np.array_equal(b, s) and np.array_equal(c, s)
+    assert np.array_equal(resg, rese) and np.array_equal(resg, resg2)
+    assert np.all(np.diff(resg, axis=1) < 0)  # residual decreases every iteration
+    # one volume against the FP64 oracle
+    p = P[1]
+    E = p["E"]
+    samp = O.Sampling(E.k1, E.k2, E.off, p["n"][0], p["n"][1])
+    Sm = p["Smaps"].cpu().numpy().astype(np.complex128)
+    yn = p["y"].cpu().numpy().astype(np.complex128)
+    Se = p["Se"].cpu().numpy().astype(np.float64)
+    xo, _, _ = O.cg_solver(yn, Sm, samp, p["T"], p["n"], nIt=nIt, tolType="RelativeResidual2Error", tol=0.0, Se=Se)
+    assert rel_l2(xg[1].cpu().numpy(), xo) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [(32, 24, 20), (20, 18, 14)])
+def test_every_pass_shape_is_correct_and_autotune_picks_one(dev, n):
+    """Each candidate shape the autotuner may pick (register path, LDS
+    engine with radix 4/8/16 and 1..Pmax pencils) transforms correctly."""
+    import ctypes
+    import disorder_amd as D
+    from tests.helpers import crandn, random_motion
+    from tests.test_gpu_parity import to_dev, to_np
+    rng = np.random.default_rng(sum(n))
+    x = crandn(rng, n)
+    T = random_motion(rng, 3)
+    ref = O.rigid_forward(x, T, n, n)
+    ctx = D.default_context(0)
+    nn = (ctypes.c_int32 * 3)(*n)
+    shapes = [(1, P, r) for r in (4, 8, 16) for P in (1, 2, 8, 0)]
+    for g, P, r in shapes:
+        for axis in range(3):
+            D._lib.check(ctx.lib.dis_set_pass_shape(ctx.h, nn, axis, g, P, r), ctx.h)
+        out = D.sincRigidTransform(to_dev(x, dev), T, di=1)
+        assert rel_l2(to_np(out), ref) < 1e-5, (g, P, r)
+    D._lib.check(ctx.lib.dis_autotune_clear(ctx.h), ctx.h)
+    choice = D.autotune(n, S=3)
+    assert len(choice) == 3 and all(c["ms"] > 0 for c in choice)
+    out = D.sincRigidTransform(to_dev(x, dev), T, di=1)
+    assert rel_l2(to_np(out), ref) < 1e-5
+    D._lib.check(ctx.lib.dis_autotune_clear(ctx.h), ctx.h)

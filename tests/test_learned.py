@@ -1,0 +1,160 @@
+"""Learned motion proposal: the network's hand-written HIP forward, backward
+and Adam kernels against the FP64 CPU oracle (oracle/nn_oracle.py), whose
+backward is itself pinned by central finite differences.
+
+Tolerances: the device rounds both operands of every matrix product to
+bfloat16 and accumulates in FP32; the oracle applies the same rounding and
+sums exactly.  Most outputs then agree to FP32 accumulation error (median
+relative error <= 1e-6), but an intermediate activation computed in FP32 on
+the device and in FP64 in the oracle can sit on either side of a bf16
+rounding boundary (probability ~1e-7 / 2^-8 per value); such a flip moves one
+operand by one bf16 ulp, so the whole-array bounds are rel-L2 <= 2e-4
+(outputs) and <= 5e-4 (gradients)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import nn_oracle as NO
+from tests.helpers import rel_l2
+
+
+def _params(seed, scale=1.0):
+    from disorder_amd.learned import MotionNet
+    p = MotionNet.init_params(seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 1)
+    return p + scale * 0.05 * rng.standard_normal(p.size)  # non-zero biases, perturbed conv
+
+
+def _batch(seed, lens=(7, 3, 12)):
+    rng = np.random.default_rng(seed)
+    so = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    X = rng.standard_normal((so[-1], NO.NIN))
+    return X, so
+
+
+def test_oracle_backward_matches_finite_differences():
+    p = _params(3)
+    X, so = _batch(4, (4, 2))
+    rng = np.random.default_rng(5)
+    dY = rng.standard_normal((so[-1], NO.NOUT))
+    g, dX = NO.backward(p, X, so, dY, bf16=False)
+
+    def f(pp, XX):
+        return float(np.sum(NO.forward(pp, XX, so, bf16=False)[0] * dY))
+    h = 1e-6
+    idx = np.concatenate([rng.choice(NO.NPARAMS, 40, replace=False),
+                          np.arange(NO.NPARAMS - 186, NO.NPARAMS)])  # random entries + all conv parameters
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        fd = (f(p + e, X) - f(p - e, X)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+    for r, c in [(0, 0), (3, 12), (5, 7)]:
+        E = np.zeros_like(X)
+        E[r, c] = h
+        fd = (f(p, X + E) - f(p, X - E)) / (2 * h)
+        assert abs(fd - dX[r, c]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_bf16_rounding_is_nearest_even():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1.0 + 2 ** -8 + 2 ** -20], np.float32)
+    r = NO.bf16_round(x)
+    assert r[0] == 1.0 and r[1] == 1.0            # tie -> even
+    assert r[2] == 1.0 + 4 * 2 ** -8                # tie -> even (up)
+    assert r[3] == -2.5
+    assert r[4] == 1.0 + 2 ** -7                    # above the tie -> up
+
+
+def test_features_diagonal_newton_step():
+    from disorder_amd.learned import features
+    S = 5
+    rng = np.random.default_rng(2)
+    h = rng.uniform(1, 10, (S, 6))
+    d = rng.normal(0, 1, (S, 6))          # true update (vox, rad)
+    g = -h * d                            # gradient of a diagonal quadratic at the current point
+    sums = np.zeros((S, 28))
+    sums[:, 0] = rng.uniform(1, 2, S)
+    sums[:, 1:7] = g
+    sums[:, 7:13] = h
+    F = features(sums)
+    assert np.allclose(F[:, :3], d[:, :3], rtol=1e-5)
+    assert np.allclose(F[:, 3:6], d[:, 3:] * 180 / math.pi, rtol=1e-5)
+    assert np.allclose(F[:, 6:12].mean(axis=0), np.log(h / h.mean(0)).mean(0), atol=1e-5)
+
+
+# ---------------------------------------------------------------- GPU
+@pytest.mark.gpu
+@pytest.mark.parametrize("lens", [(7, 3, 12), (1,), (64, 48, 33, 5)])
+def test_forward_backward_match_oracle(dev, lens):
+    import torch
+    from disorder_amd.learned import MotionNet
+    p = _params(11)
+    X, so = _batch(12, lens)
+    rng = np.random.default_rng(13)
+    dY = rng.standard_normal((so[-1], NO.NOUT))
+    net = MotionNet(p.astype(np.float32))
+    Xd = torch.from_numpy(X.astype(np.float32)).to(dev)
+    Y = net.forward(Xd, so).cpu().numpy()
+    p32 = p.astype(np.float32).astype(np.float64)
+    X32 = X.astype(np.float32).astype(np.float64)
+    Yr, _ = NO.forward(p32, X32, so, bf16=True)
+    assert rel_l2(Y, Yr) < 2e-4
+    assert np.median(np.abs(Y - Yr) / (np.abs(Yr) + 1e-3)) < 1e-6
+    G, dX = net.backward(Xd, so, torch.from_numpy(dY.astype(np.float32)).to(dev))
+    Gr, dXr = NO.backward(p32, X32, so, dY.astype(np.float32).astype(np.float64), bf16=True)
+    G = G.cpu().numpy()
+    q, qr = NO.unpack(G), NO.unpack(Gr)
+    for name, _ in NO.SHAPES:
+        assert rel_l2(q[name], qr[name]) < 5e-4, name
+    assert rel_l2(dX.cpu().numpy(), dXr) < 5e-4
+    # the bf16 products differ from the plain FP64 network by ~2^-9, far above the bounds above
+    Y64, _ = NO.forward(p32, X32, so, bf16=False)
+    assert rel_l2(Y, Y64) > 5 * rel_l2(Y, Yr)
+
+
+@pytest.mark.gpu
+def test_adam_step_matches_oracle(dev):
+    import torch
+    from disorder_amd.learned import MotionNet
+    p = _params(21)
+    X, so = _batch(22, (10, 6))
+    rng = np.random.default_rng(23)
+    tgt = rng.standard_normal((so[-1], NO.NOUT))
+    net = MotionNet(p.astype(np.float32))
+    Xd = torch.from_numpy(X.astype(np.float32)).to(dev)
+    Td = torch.from_numpy(tgt.astype(np.float32)).to(dev)
+    loss = net.train_step(Xd, Td, so, lr=1e-3)
+    p32 = p.astype(np.float32).astype(np.float64)
+    X32 = X.astype(np.float32).astype(np.float64)
+    T32 = tgt.astype(np.float32).astype(np.float64)
+    Y, _ = NO.forward(p32, X32, so, bf16=True)
+    n = Y.size
+    assert abs(loss - np.mean((Y - T32) ** 2)) < 1e-5 * max(1.0, loss)
+    g, _ = NO.backward(p32, X32, so, 2 * (Y - T32) / n, bf16=True)
+    pn, _, _ = NO.adam_step(p32, g, np.zeros_like(g), np.zeros_like(g), 1, 1e-3)
+    # the first Adam step moves every parameter by ~lr * sign(g); compare the moves
+    mv, mvr = net.p.cpu().numpy() - p32, pn - p32
+    big = np.abs(g) > 1e-3 * np.abs(g).max()
+    assert np.allclose(mv[big], mvr[big], atol=2e-6)
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss(dev):
+    """Adam on a smooth synthetic regression (target = 0.5 * first six
+    features plus the mean of the neighbours) lowers the loss by 5x in 40
+    epochs of 64 scans."""
+    import torch
+    from disorder_amd.learned import MotionNet, train
+    rng = np.random.default_rng(31)
+    data = []
+    for _ in range(64):
+        S = int(rng.integers(16, 40))
+        F = rng.standard_normal((S, NO.NIN)).astype(np.float32)
+        t = 0.5 * F[:, :6]
+        t[1:-1] += 0.25 * (F[:-2, :6] + F[2:, :6])
+        data.append((F, t.astype(np.float32)))
+    net = MotionNet(seed=0)
+    losses = train(net, data, epochs=40, batch=512, lr=1e-3)
+    assert losses[-1] < 0.2 * losses[0], losses[::8]
+    assert all(b <= a * 1.05 for a, b in zip(losses[::8], losses[8::8]))
