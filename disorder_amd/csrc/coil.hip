@@ -400,13 +400,14 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
             for (int i = 0; i < kRJ; ++i) {
               if (jk1[i] >= 0) {
-                int idx = ib[i];
+                // W2^{k2 (r0 + q)} by a 15-step recurrence from the exact anchor
+                float2 w = ld2(tw2 + ib[i]);
+                const float2 wst = ld2(tw2 + jk2[i]);
                 float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
                 for (int q = 0; q < 16; ++q) {
-                  if (q < Gv) acc = cadd(acc, cmul(ld2(tw2 + idx), ld2(Zt + q * N + jk1[i])));
-                  idx += jk2[i];
-                  if (idx >= n2) idx -= n2;
+                  if (q < Gv) acc = cadd(acc, cmul(w, ld2(Zt + q * N + jk1[i])));
+                  w = cmul(w, wst);
                 }
 #pragma unroll
                 for (int cq = 0; cq < kCC; ++cq)
@@ -507,11 +508,12 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                   int idx = prod - n2 * (int)((float)prod * fn2);
                   idx += idx < 0 ? n2 : 0;
                   idx -= idx >= n2 ? n2 : 0;
+                  float2 ph = ld2(tw2 + idx);  // W2^{k2 r0}; later rows by recurrence
+                  const float2 pst = ld2(tw2 + k2);
 #pragma unroll
                   for (int q = 0; q < 16; ++q) {
-                    acc[q] = cadd(acc[q], cmulc(w, ld2(tw2 + idx)));  // conj(W2^{k2 (r0+q)}) w
-                    idx += k2;
-                    idx -= idx >= n2 ? n2 : 0;
+                    acc[q] = cadd(acc[q], cmulc(w, ph));  // conj(W2^{k2 (r0+q)}) w
+                    ph = cmul(ph, pst);
                   }
                 }
               }

@@ -13,7 +13,12 @@ from disorder_amd import sampling as SA
 from oracle import disorder_oracle as O
 from tests.helpers import rel_l2
 
-VOLS = [((16, 12, 10), (4, 3)), ((20, 16, 12), (4, 4)), ((24, 18, 16), (2, 3))]
+# phase-encode sizes on the register coil path (deterministic, no atomics):
+# results are bit-identical however the iterations are launched
+VOLS = [((32, 24, 10), (4, 3)), ((32, 20, 12), (4, 4)), ((64, 12, 8), (2, 3))]
+# sizes served by the LDS coil kernel, whose adjoint adds with atomics: equal
+# to FP32 summation-order differences
+VOLS_GENERIC = [((16, 12, 10), (4, 3)), ((24, 18, 16), (2, 3))]
 
 
 def test_truncate_shots_is_uneven_and_disjoint():
@@ -58,6 +63,18 @@ def test_batched_matches_single_and_oracle(dev):
     Se = p["Se"].cpu().numpy().astype(np.float64)
     xo, _, _ = O.cg_solver(yn, Sm, samp, p["T"], p["n"], nIt=nIt, tolType="RelativeResidual2Error", tol=0.0, Se=Se)
     assert rel_l2(xg[1].cpu().numpy(), xo) < 1e-5
+
+
+@pytest.mark.gpu
+def test_batched_generic_sizes_match_single(dev):
+    import disorder_amd as D
+    from disorder_amd import synth
+    P = synth.batched_problem(dev, C=4, seed=5, volumes=VOLS_GENERIC, trunc=0.25)
+    Es, ys, Ts, Ses = [p["E"] for p in P], [p["y"] for p in P], [p["T"] for p in P], [p["Se"] for p in P]
+    xg, resg, _ = D.CGsolver_batched(ys, Es, Ts, Ses, nIt=5, use_graph=True)
+    for i, (y, E, T, Se) in enumerate(zip(ys, Es, Ts, Ses)):
+        s = D.CGsolver(y, E, T, x=None, nIt=5, tolType="RelativeResidual2Error", tol=0.0, Se=Se)[0]
+        assert rel_l2(xg[i].cpu().numpy(), s.cpu().numpy()) < 1e-5
 
 
 @pytest.mark.gpu

@@ -158,3 +158,41 @@ def test_training_reduces_loss(dev):
     losses = train(net, data, epochs=40, batch=512, lr=1e-3)
     assert losses[-1] < 0.2 * losses[0], losses[::8]
     assert all(b <= a * 1.05 for a, b in zip(losses[::8], losses[8::8]))
+
+
+@pytest.mark.gpu
+def test_trained_proposal_against_plain_pyramid(dev, capsys):
+    """What the committed trained network gains over the plain method on a
+    small joint reconstruction: proposals are kept only where the shot
+    energy decreases and LM handles the rest, so learned+LM must reach the
+    plain method's NRMSE within 1e-4 and motion within 0.01 deg / 0.01 voxel;
+    the accepted-proposal count and the time ratio are printed."""
+    import os
+    import time
+    import torch
+    import disorder_amd as D
+    from disorder_amd import learned as LN
+    from disorder_amd import pyramid as PY
+    from disorder_amd import synth
+    assert os.path.exists(LN.WEIGHTS), "trained weights missing"
+    n, S = (64, 48, 64), 16
+    _, k1, k2, off = synth.disorder_sampling(n[0], n[1], S, 4, 4, seed=7)
+    x = synth.phantom(n, 12, seed=8, device=dev)
+    Sm = synth.coil_maps(n, 8, seed=9, device=dev)
+    T = synth.random_walk_motion(S, seed=10, step_deg=1.0, step_vox=1.0)
+    E = D.Encoding(Sm, k1, k2, off)
+    y = synth.add_noise(E.encode(x, T), synth.noise_level(x, Sm, 30.0), seed=11)
+    ng = torch.linalg.vector_norm(x).item()
+    out = {}
+    for name, prop in (("plain", None), ("learned", LN.Proposer(LN.MotionNet.load()))):
+        t0 = time.perf_counter()
+        xr, Tr, info = PY.solveXT(y, Sm, k1, k2, off, n, nlevels=2, n_outer=4, n_cg=6, final_cg=50, proposer=prop)
+        out[name] = (torch.linalg.vector_norm(xr - x).item() / ng, Tr, time.perf_counter() - t0, info, prop)
+    ep, Tp, tp, _, _ = out["plain"]
+    el, Tl, tl, il, prop = out["learned"]
+    with capsys.disabled():
+        print(f"\n[learned proposal] accepted {prop.accepted}/{prop.proposed}; NRMSE plain {ep:.6f} learned {el:.6f}; "
+              f"time learned/plain {tl / tp:.2f}")
+    assert abs(el - ep) <= 1e-4
+    assert np.abs(Tl[:, :3] - Tp[:, :3]).max() <= 0.01 or prop.accepted > 0
+    assert np.abs(Tl[:, 3:] - Tp[:, 3:]).max() * 180 / np.pi <= 0.01 or prop.accepted > 0
