@@ -578,6 +578,10 @@ hipError_t launch_rcoil(const CoilArgs& a, int mode, int nstates, hipStream_t st
   return hipGetLastError();
 }
 
+int coil_chunk(int n1, int n2) {
+  return (n2 <= 1024 && reg_radix(n1)) ? kRJ * kThreads : kMaxJ * kThreads;
+}
+
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
   if (a.n2 <= 1024 && !a.force_generic) {
     switch (reg_radix(a.n1)) {

@@ -95,6 +95,9 @@ struct CoilArgs {
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st);
+// Samples per chunk of the coil-stage kernel that launch_coil picks for a
+// (n1, n2) plane: a state with more samples is processed in several chunks.
+int coil_chunk(int n1, int n2);
 hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStream_t st);
 
 // ---------------------------------------------------------------- utilities

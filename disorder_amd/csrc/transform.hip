@@ -327,8 +327,10 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
   const size_t vol = sizeof(float2) * nv;
   int maxnec = 0;
   for (int s = 0; s < S; ++s) maxnec = std::max(maxnec, v.hoff[s + 1] - v.hoff[s]);
-  // E^H E runs in place (U -> U) only when one workgroup pass covers all coils and samples
-  const bool inplace = !(mode == COIL_EHE && (g.ncoils > 8 || maxnec > 4 * kThreads));
+  // E^H E runs in place (U -> U) only when one workgroup pass covers all
+  // coils and samples: with several chunks the adjoint of the first would
+  // overwrite the U that the forward of the next still reads
+  const bool inplace = !(mode == COIL_EHE && (g.ncoils > 8 || maxnec > coil_chunk(g.n[0], g.n[1])));
   const int vols_per_state = inplace ? 1 : 2;
 
   // workspace: [Sb (x2) volumes][energy partials Ns*n3 floats][per-sample doubles Ns]

@@ -110,7 +110,11 @@ def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
                                                     ((32, 24, 16), 3, 4, None, True),
                                                     ((64, 20, 8), 8, 5, 4, True),
                                                     ((192, 16, 6), 2, 4, None, False),
-                                                    ((256, 12, 4), 10, 4, None, True)])
+                                                    ((256, 12, 4), 10, 4, None, True),
+                                                    # states of ~900 and ~1280 samples: several sample
+                                                    # chunks per workgroup on the register path
+                                                    ((64, 28, 4), 8, 2, None, True),
+                                                    ((64, 40, 4), 8, 2, None, False)])
 def test_encode_decode_ehe(dev, n, C, S, nmotion, weights):
     import disorder_amd as D
     rng = np.random.default_rng(11)
