@@ -586,7 +586,9 @@ hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st)
   if (a.n2 <= 1024 && !a.force_generic) {
     switch (reg_radix(a.n1)) {
       case 16: return launch_rcoil<16>(a, mode, nstates, st);
+      case 14: return launch_rcoil<14>(a, mode, nstates, st);
       case 12: return launch_rcoil<12>(a, mode, nstates, st);
+      case 10: return launch_rcoil<10>(a, mode, nstates, st);
       case 8: return launch_rcoil<8>(a, mode, nstates, st);
       case 4: return launch_rcoil<4>(a, mode, nstates, st);
       case 2: return launch_rcoil<2>(a, mode, nstates, st);

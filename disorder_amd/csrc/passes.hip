@@ -548,7 +548,9 @@ __global__ __launch_bounds__(kThreads) void rpass_kernel(PassArgs a) {
 int reg_radix(int n) {
   switch (n) {
     case 256: return 16;
+    case 224: return 14;
     case 192: return 12;
+    case 160: return 10;
     case 128: return 8;
     case 64: return 4;
     case 32: return 2;
@@ -583,7 +585,9 @@ hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t
   if (R1 && mode != MODE_FLIP && (a.axis == 0 || a.n1 >= 16) && !a.force_generic) {
     switch (R1) {
       case 16: return launch_rpass<16>(a, mode, grid_states, st);
+      case 14: return launch_rpass<14>(a, mode, grid_states, st);
       case 12: return launch_rpass<12>(a, mode, grid_states, st);
+      case 10: return launch_rpass<10>(a, mode, grid_states, st);
       case 8: return launch_rpass<8>(a, mode, grid_states, st);
       case 4: return launch_rpass<4>(a, mode, grid_states, st);
       case 2: return launch_rpass<2>(a, mode, grid_states, st);

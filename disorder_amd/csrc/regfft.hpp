@@ -47,6 +47,54 @@ template <> struct Dft<12> {
   }
 };
 
+template <> struct Dft<10> {
+  // r = 2a + b, k = c + 5d:  W10^{rk} = W5^{ac} W10^{bc} W2^{bd}
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 e[5], o[5];
+#pragma unroll
+    for (int a = 0; a < 5; ++a) {
+      e[a] = v[2 * a];
+      o[a] = v[2 * a + 1];
+    }
+    DftOdd<5>::run(e);
+    DftOdd<5>::run(o);
+    o[1] = wconst(o[1], 0.80901699437494745f, 0.58778525229247314f);  // W10^1
+    o[2] = wconst(o[2], 0.30901699437494745f, 0.95105651629515353f);  // W10^2
+    o[3] = wconst(o[3], -0.30901699437494734f, 0.95105651629515364f);  // W10^3
+    o[4] = wconst(o[4], -0.80901699437494734f, 0.58778525229247325f);  // W10^4
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+      v[c] = cadd(e[c], o[c]);
+      v[c + 5] = csub(e[c], o[c]);
+    }
+  }
+};
+
+template <> struct Dft<14> {
+  // r = 2a + b, k = c + 7d:  W14^{rk} = W7^{ac} W14^{bc} W2^{bd}
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 e[7], o[7];
+#pragma unroll
+    for (int a = 0; a < 7; ++a) {
+      e[a] = v[2 * a];
+      o[a] = v[2 * a + 1];
+    }
+    DftOdd<7>::run(e);
+    DftOdd<7>::run(o);
+    o[1] = wconst(o[1], 0.90096886790241915f, 0.43388373911755812f);  // W14^1
+    o[2] = wconst(o[2], 0.62348980185873359f, 0.7818314824680298f);  // W14^2
+    o[3] = wconst(o[3], 0.22252093395631445f, 0.97492791218182362f);  // W14^3
+    o[4] = wconst(o[4], -0.22252093395631434f, 0.97492791218182362f);  // W14^4
+    o[5] = wconst(o[5], -0.62348980185873348f, 0.78183148246802991f);  // W14^5
+    o[6] = wconst(o[6], -0.90096886790241903f, 0.43388373911755823f);  // W14^6
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      v[c] = cadd(e[c], o[c]);
+      v[c + 7] = csub(e[c], o[c]);
+    }
+  }
+};
+
 // Exchange-buffer size (elements) per pencil for both factorisations,
 // padded so that the lanes of a half-wavefront (32 lanes, LDS b64 accesses
 // are served per half) hit distinct banks:

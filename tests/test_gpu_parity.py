@@ -26,7 +26,10 @@ def to_np(t):
                                      ((28, 20, 21), (56, 40, 42)), ((64, 48, 64), None),
                                      # register-path lengths 32, 64, 128, 192, 256 on every axis
                                      ((32, 64, 128), (64, 128, 256)), ((192, 16, 32), None), ((256, 32, 20), None),
-                                     ((24, 256, 16), None), ((16, 18, 192), None)])
+                                     ((24, 256, 16), None), ((16, 18, 192), None),
+                                     # radix-10 and radix-14 register paths (160, 224)
+                                     ((160, 20, 16), None), ((20, 224, 12), (40, 224, 24)), ((16, 18, 160), None),
+                                     ((224, 16, 10), None)])
 def test_rigid_forward_inverse(dev, n, Nfull):
     import disorder_amd as D
     rng = np.random.default_rng(sum(n))
@@ -114,7 +117,10 @@ def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
                                                     # states of ~900 and ~1280 samples: several sample
                                                     # chunks per workgroup on the register path
                                                     ((64, 28, 4), 8, 2, None, True),
-                                                    ((64, 40, 4), 8, 2, None, False)])
+                                                    ((64, 40, 4), 8, 2, None, False),
+                                                    # register coil stage at n1 = 160 and 224
+                                                    ((160, 16, 4), 4, 4, None, True),
+                                                    ((224, 12, 4), 3, 3, None, False)])
 def test_encode_decode_ehe(dev, n, C, S, nmotion, weights):
     import disorder_amd as D
     rng = np.random.default_rng(11)
