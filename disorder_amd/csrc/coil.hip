@@ -400,15 +400,17 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
             for (int i = 0; i < kRJ; ++i) {
               if (jk1[i] >= 0) {
-                // W2^{k2 (r0 + q)} by a 15-step recurrence from the exact anchor
-                float2 w = ld2(tw2 + ib[i]);
-                const float2 wst = ld2(tw2 + jk2[i]);
+                // sum_q W2^{k2 (r0 + q)} Z[q] = W2^{k2 r0} * (Horner in a = W2^{k2})
+                const float2 a1 = ld2(tw2 + jk2[i]);
                 float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                  if (q < Gv) acc = cadd(acc, cmul(w, ld2(Zt + q * N + jk1[i])));
-                  w = cmul(w, wst);
+                for (int q = 15; q >= 0; --q) {
+                  if (q < Gv) {
+                    const float2 z = ld2(Zt + q * N + jk1[i]);
+                    acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
+                  }
                 }
+                acc = cmul(acc, ld2(tw2 + ib[i]));
 #pragma unroll
                 for (int cq = 0; cq < kCC; ++cq)
                   if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
@@ -508,12 +510,13 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                   int idx = prod - n2 * (int)((float)prod * fn2);
                   idx += idx < 0 ? n2 : 0;
                   idx -= idx >= n2 ? n2 : 0;
-                  float2 ph = ld2(tw2 + idx);  // W2^{k2 r0}; later rows by recurrence
+                  // conj(W2^{k2 (r0+q)}) w = (w conj(W2^{k2 r0})) conj(W2^{k2})^q
+                  float2 wq = cmulc(w, ld2(tw2 + idx));
                   const float2 pst = ld2(tw2 + k2);
 #pragma unroll
                   for (int q = 0; q < 16; ++q) {
-                    acc[q] = cadd(acc[q], cmulc(w, ph));  // conj(W2^{k2 (r0+q)}) w
-                    ph = cmul(ph, pst);
+                    acc[q] = cadd(acc[q], wq);
+                    wq = cmulc(wq, pst);
                   }
                 }
               }
