@@ -811,14 +811,22 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
       std::vector<double> hstep((size_t)ns * 6);
       DCHK(c, hipMemcpyAsync(hstep.data(), dstep, sizeof(double) * hstep.size(), hipMemcpyDeviceToHost, stream));
       DCHK(c, hipStreamSynchronize(stream));
-      for (int i = 0; i < ns; ++i)
-        for (int gi = 0; gi < 6; ++gi) dGEff[(size_t)sol[i] * 6 + gi] = hstep[(size_t)i * 6 + gi];
+      for (int i = 0; i < ns; ++i) {
+        // a singular damped system (mldivide returns Inf/NaN in the
+        // reference) gives no step here
+        bool finite = true;
+        for (int gi = 0; gi < 6; ++gi) finite = finite && std::isfinite(hstep[(size_t)i * 6 + gi]);
+        for (int gi = 0; gi < 6; ++gi) dGEff[(size_t)sol[i] * 6 + gi] = finite ? hstep[(size_t)i * 6 + gi] : 0.0;
+      }
     }
     // Tup = restrictTransform(T + dGEff) (rotations into [-pi, pi])
     for (int s = 0; s < NM; ++s)
       for (int gi = 0; gi < 6; ++gi) {
         double vv = T[s + (size_t)NM * gi] + dGEff[(size_t)s * 6 + gi];
         if (gi >= 3) {
+          // restrictTransform.m:28-29 (repeated +-2 pi); far-out values are
+          // first reduced with fmod so the loop stays bounded
+          if (std::fabs(vv) > 64 * M_PI) vv = std::fmod(vv, 2 * M_PI);
           while (vv > M_PI) vv -= 2 * M_PI;
           while (vv < -M_PI) vv += 2 * M_PI;
         }

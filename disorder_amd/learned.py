@@ -159,6 +159,12 @@ def state_energy(E, x, y, T, off):
     return np.add.reduceat(np.asarray(ps, np.float64), off[:-1]) if len(ps) else np.zeros(len(off) - 1)
 
 
+def level_scale(L):
+    """Full-resolution voxels per level voxel (translations of the features
+    and updates are expressed in level voxels)."""
+    return max(Nf / n for Nf, n in zip(L.E.Nfull, L.n))
+
+
 class Proposer:
     """Per-iteration proposal step for pyramid.solveXT: returns the motion
     after the proposal and the mask of states that took it (their energy
@@ -174,9 +180,8 @@ class Proposer:
         if not active.any():
             return T, np.zeros(len(T), bool)
         sums = lm_sums(L.y, L.E, x, T)
-        # translations in voxels of the level grid (the network is trained at
-        # the native resolution of its scans)
-        scale = max(Nf / n for Nf, n in zip(L.E.Nfull, L.n))
+        # translations in voxels of the level grid
+        scale = level_scale(L)
         Fh = features(sums)
         Fh[:, 0:3] /= scale
         F = torch.from_numpy(Fh).to(x.device)

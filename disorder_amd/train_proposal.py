@@ -37,6 +37,7 @@ def main(argv=None):
     ap.add_argument("--log", default=None)
     ap.add_argument("--save-data", default=None, help="write the simulated examples (.npz, float16) and stop")
     ap.add_argument("--load-data", nargs="*", default=None, help="train on saved examples instead of simulating")
+    ap.add_argument("--init", default=None, help="start from these weights (fine-tuning)")
     a = ap.parse_args(argv)
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(a.seed)
@@ -70,7 +71,7 @@ def main(argv=None):
     nval = int(len(data) * a.val)
     val = [data[i] for i in order[:nval]]
     trn = [data[i] for i in order[nval:]]
-    net = LN.MotionNet(seed=a.seed)
+    net = LN.MotionNet.load(a.init) if a.init else LN.MotionNet(seed=a.seed)
     t1 = time.perf_counter()
 
     def log(ep, loss):

@@ -111,6 +111,8 @@ def restrict_transform(T):
     by repeated +-2pi."""
     T = np.array(T, dtype=np.float64, copy=True)
     ang = T[:, 3:6]
+    far = np.abs(ang) > 64 * np.pi  # bounded loop for far-out values
+    ang[far] = np.fmod(ang[far], 2 * np.pi)
     while np.any(ang > np.pi):
         ang[ang > np.pi] -= 2 * np.pi
     while np.any(ang < -np.pi):
@@ -689,7 +691,12 @@ def lm_step(dH, dG, w, winit):
             MH[i, i] = (1 + w) * dH[h]
         else:
             MH[i, j] = MH[j, i] = dH[h]
-    return -(winit / w) * np.linalg.solve(MH, dG)
+    try:
+        step = -(winit / w) * np.linalg.solve(MH, dG)
+    except np.linalg.LinAlgError:
+        step = np.full(6, np.nan)
+    # a singular system (Inf/NaN from mldivide in the reference) takes no step
+    return step if np.all(np.isfinite(step)) else np.zeros(6)
 
 
 def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None):

@@ -341,3 +341,15 @@ def test_outer_segments_use_untransformed_x():
     assert np.abs(y - yfull).max() < 1e-12
     yr = crandn(rng, y.shape)
     assert abs(np.vdot(yr, y) - np.vdot(O.decode(yr, Sm, samp, T, n, nmotion=2), x)) < 1e-10 * abs(np.vdot(yr, y))
+
+
+def test_lm_singular_system_takes_no_step_and_wrap_is_bounded():
+    """A singular damped system gives no step (the reference's mldivide
+    would return Inf/NaN, and its rotation wrap would then never end); far-out
+    angles are wrapped in bounded time."""
+    step = O.lm_step(np.zeros(21), np.ones(6), 1e-3, 1e-3)
+    assert np.array_equal(step, np.zeros(6))
+    T = np.array([[0, 0, 0, 1e30, -7.0, 4.0]])
+    R = O.restrict_transform(T)
+    assert np.all(np.abs(R[:, 3:]) <= np.pi)
+    assert abs(R[0, 4] - (-7.0 + 2 * np.pi)) < 1e-15 and abs(R[0, 5] - (4.0 - 2 * np.pi)) < 1e-15
