@@ -56,6 +56,7 @@ SIGNATURES = {
                        c_int32, c_int32, ctypes.c_double, c_double_p, c_int32_p, c_int32_p]),
     "dis_autotune": (c_int, [c_void_p, c_int32_p, c_int32_p, c_int32, c_int32_p, c_double_p]),
     "dis_autotune_clear": (c_int, [c_void_p]),
+    "dis_set_coil_schedule": (c_int, [c_void_p, c_int32]),
     "dis_set_pass_shape": (c_int, [c_void_p, c_int32_p, c_int32, c_int32, c_int32, c_int32]),
     "dis_cg_batched": (c_int, [c_void_p, c_int32, ctypes.POINTER(c_void_p), ctypes.POINTER(c_double_p),
                                ctypes.POINTER(c_void_p), ctypes.POINTER(c_float_p), ctypes.POINTER(c_void_p), c_int32,

@@ -33,6 +33,7 @@ struct dis_ctx {
   // buffer; while replaying the same sequence of uploads returns those
   // buffers without copies or synchronisation (for graph capture)
   std::map<std::array<int, 4>, PassTune> tune;  // tuned pass shapes per (n1, n2, n3, axis)
+  int coil_schedule = 0;  // 0 auto, 1 static grid, 2 persistent work queue
   int tape_mode = 0;  // 0 off, 1 record, 2 replay
   std::vector<std::pair<void*, size_t>> tape;
   size_t tape_pos = 0;
@@ -205,6 +206,8 @@ bool ctx_profiling(dis_ctx* c, int on) {
 }
 
 int ctx_device(dis_ctx* c) { return c->device; }
+
+int ctx_coil_schedule(dis_ctx* c) { return c->coil_schedule; }
 
 int prof_begin(dis_ctx* c, int kclass) {
   if (!c->prof) return -1;
@@ -579,6 +582,13 @@ extern "C" int dis_set_pass_shape(dis_ctx* c, const int32_t n[3], int32_t axis, 
   t.maxradix = maxradix;
   const int nn[3] = {n[0], n[1], n[2]};
   ctx_tune_set(c, nn, axis, t);
+  return DIS_OK;
+}
+
+extern "C" int dis_set_coil_schedule(dis_ctx* c, int32_t mode) {
+  if (!c) return DIS_E_INVALID;
+  if (mode < 0 || mode > 2) return fail(c, DIS_E_INVALID, "dis_set_coil_schedule: mode must be 0, 1 or 2");
+  c->coil_schedule = mode;
   return DIS_OK;
 }
 

@@ -11,6 +11,8 @@
 // LDS float atomics, inverse-FFTs along dim 1, multiplies by conj(S_c) and
 // sums the coils in registers.  For E^H E both halves run in the same
 // workgroup, so the state's k-space never leaves the chip.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "regfft.hpp"
 
@@ -290,7 +292,19 @@ hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStre
 // coil maps through the XCD's L2.
 constexpr int kRJ = 3;  // samples per thread per chunk on the register path
 
-template <int R1, int MODE>
+// Next (state, plane) item for a persistent workgroup whose home queue is
+// `home`: items of queue k are planes i3 = k (mod 8), all states of a plane
+// consecutive.  Returns k * per_queue + local, or -1 when every queue is dry.
+__device__ __forceinline__ int queue_next(int* queue, int home, int per_queue) {
+  for (int q = 0; q < 8; ++q) {
+    const int k = (home + q) & 7;
+    const int v = atomicAdd(queue + k, 1);
+    if (v < per_queue) return k * per_queue + v;
+  }
+  return -1;
+}
+
+template <int R1, int MODE, bool PERSIST = false>
 __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nstates) {
   constexpr bool FWD = MODE == COIL_ENCODE || MODE == COIL_EHE || MODE == COIL_ENERGY;
   constexpr bool ADJ = MODE == COIL_DECODE || MODE == COIL_EHE;
@@ -314,13 +328,11 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   for (int k = tid; k < n2; k += kThreads) st2(tw2 + k, a.tw2[k]);
   __syncthreads();
 
-  // XCD-aware decode of the 1D grid: XCD x takes planes x, x+8, ... with all
-  // states of a plane consecutive.
-  const int id = blockIdx.x;
-  const int xcd = id & 7, local = id >> 3;
-  const int i3 = (local / nstates) * 8 + xcd;
-  const int sl = local - (local / nstates) * nstates;
-  if (i3 >= a.n3) return;
+  // XCD-aware items: XCD x takes planes x, x+8, ... with all states of a
+  // plane consecutive (static: one item per workgroup from blockIdx;
+  // persistent: items pulled from the per-XCD queues).
+  const int per_queue = ((a.n3 + 7) / 8) * nstates;
+  auto item_body = [&](const int sl, const int i3) {
   const int s = a.slist ? a.slist[sl] : a.state0 + sl;
   const long long nplane = (long long)N * n2;
   const long long nvox = nplane * a.n3;
@@ -565,6 +577,31 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
     }
     if (sc1 >= o1) break;
   }
+  };  // item_body
+  if (PERSIST) {
+    __shared__ int item_sh;
+    for (;;) {
+      if (tid == 0) item_sh = queue_next(a.queue, blockIdx.x & 7, per_queue);
+      __syncthreads();
+      const int item = item_sh;
+      __syncthreads();
+      if (item < 0) break;  // uniform: every thread read the same item
+      const int xcd = item / per_queue, local = item - xcd * per_queue;
+      const int i3 = (local / nstates) * 8 + xcd;
+      if (i3 < a.n3) {
+        item_body(local - (local / nstates) * nstates, i3);
+        __syncthreads();  // the next item's LDS writes follow every read of this one
+      }
+    }
+  } else {
+    const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int i3 = (local / nstates) * 8 + xcd;
+    if (i3 < a.n3) item_body(local - (local / nstates) * nstates, i3);
+  }
+}
+
+__global__ void zero_queue_kernel(int* q) {
+  if (threadIdx.x < 8) q[threadIdx.x] = 0;
 }
 
 template <int R1>
@@ -580,6 +617,55 @@ hipError_t launch_rcoil(const CoilArgs& a, int mode, int nstates, hipStream_t st
   }
   return hipGetLastError();
 }
+
+template <int R1>
+hipError_t launch_rcoil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  static int resident[5] = {0, 0, 0, 0, 0};
+  const int mi = mode - 1;
+  if (mi < 0 || mi > 3) return hipErrorInvalidValue;
+  if (!resident[mi]) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) {
+      switch (mode) {
+        case COIL_ENCODE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_ENCODE, true>, kThreads, 0); break;
+        case COIL_DECODE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_DECODE, true>, kThreads, 0); break;
+        case COIL_EHE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_EHE, true>, kThreads, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_ENERGY, true>, kThreads, 0); break;
+      }
+    }
+    if (e != hipSuccess) return e;
+    resident[mi] = std::max(8, (ncu * std::max(1, per_cu)) / 8 * 8);
+  }
+  const int items = ((a.n3 + 7) / 8) * 8 * nstates;
+  const int nblk = std::max(8, std::min(resident[mi], (items + 7) / 8 * 8));
+  hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, st, a.queue);
+  dim3 grid(nblk);
+  switch (mode) {
+    case COIL_ENCODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_ENCODE, true>), grid, dim3(kThreads), 0, st, a, nstates); break;
+    case COIL_DECODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_DECODE, true>), grid, dim3(kThreads), 0, st, a, nstates); break;
+    case COIL_EHE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_EHE, true>), grid, dim3(kThreads), 0, st, a, nstates); break;
+    default: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_ENERGY, true>), grid, dim3(kThreads), 0, st, a, nstates); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  if (!a.queue || a.n2 > 1024 || a.force_generic) return hipErrorInvalidValue;
+  switch (reg_radix(a.n1)) {
+    case 16: return launch_rcoil_persistent<16>(a, mode, nstates, st);
+    case 14: return launch_rcoil_persistent<14>(a, mode, nstates, st);
+    case 12: return launch_rcoil_persistent<12>(a, mode, nstates, st);
+    case 10: return launch_rcoil_persistent<10>(a, mode, nstates, st);
+    case 8: return launch_rcoil_persistent<8>(a, mode, nstates, st);
+    case 4: return launch_rcoil_persistent<4>(a, mode, nstates, st);
+    case 2: return launch_rcoil_persistent<2>(a, mode, nstates, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+bool coil_queue_available(int n1, int n2) { return n2 <= 1024 && reg_radix(n1) != 0; }
 
 int coil_chunk(int n1, int n2) {
   return (n2 <= 1024 && reg_radix(n1)) ? kRJ * kThreads : kMaxJ * kThreads;

@@ -90,11 +90,19 @@ struct CoilArgs {
   const int* rowptr;      // [S][n1+1] per-state row pointers of the samples sorted by k1
   const int* csr;         // [Ns] (k2 << 16) | (sample index within its state), sorted by k1 per state
   const int* slist;       // optional [nstates] list of the states handled (U/V indexed by list position)
+  int* queue;             // persistent launch: 8 per-XCD work counters (zeroed before the launch), else null
 };
 
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st);
+// Persistent register-path coil stage: resident workgroups pull (state,
+// readout-plane) items from per-XCD queues (stealing from the others when
+// their own runs dry) until every item is done.  a.queue must point to 8
+// ints; they are zeroed by a kernel on the stream first.  Returns
+// hipErrorInvalidValue where the register path does not apply.
+hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st);
+bool coil_queue_available(int n1, int n2);
 // Samples per chunk of the coil-stage kernel that launch_coil picks for a
 // (n1, n2) plane: a state with more samples is processed in several chunks.
 int coil_chunk(int n1, int n2);

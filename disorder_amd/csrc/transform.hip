@@ -353,6 +353,21 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
 
   const int G = std::max(1, std::min(g.n[1], kMaxTile / g.n[0]));
   bool first_out = true;
+  // uneven shots (auto: largest state above 1.25x the smallest) go through
+  // the persistent work-queue launch on the register path
+  int* queue = nullptr;
+  {
+    int minnec = maxnec;
+    for (int s = 0; s < NM; ++s) minnec = std::min(minnec, v.hoff[s + 1] - v.hoff[s]);
+    const int sched = ctx_coil_schedule(c);
+    const bool want = sched == 2 || (sched == 0 && NM > 1 && maxnec * 4 > minnec * 5);
+    if (want && coil_queue_available(g.n[0], g.n[1])) {
+      void* qb = nullptr;
+      rc = ctx_buffer(c, 4, 64, &qb);
+      if (rc) return rc;
+      queue = (int*)qb;
+    }
+  }
   auto coil_args = [&](int s0, const float2* Usrc, long long us) {
     CoilArgs a{};
     a.U = Usrc;
@@ -408,7 +423,12 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
     a.tadj = adj ? (const double*)(d + ota) - s0 : nullptr;
     {
       const int ps = prof_begin(c, DIS_K_COIL);
-      DCHK(c, launch_coil(a, mode, sb, stream));
+      if (queue) {
+        a.queue = queue;
+        DCHK(c, launch_coil_persistent(a, mode, sb, stream));
+      } else {
+        DCHK(c, launch_coil(a, mode, sb, stream));
+      }
       // pass model of the coil stage per state: (6C+2) volumes (E^H E), (3C+1) forward or adjoint only
       const double cv = mode == COIL_EHE ? 6.0 * g.ncoils + 2 : 3.0 * g.ncoils + 1;
       prof_end(c, ps, cv * 8.0 * (double)nv * sb);

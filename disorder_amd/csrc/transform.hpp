@@ -75,6 +75,9 @@ void ctx_tape_free(dis_ctx* c);
 // Switch event profiling on (1) / off (0) / leave it (-1); returns the old state.
 bool ctx_profiling(dis_ctx* c, int on);
 int ctx_device(dis_ctx* c);
+// Coil-stage schedule: 0 auto (work queue for uneven shots), 1 static grid,
+// 2 persistent work queue.
+int ctx_coil_schedule(dis_ctx* c);
 
 // Event timing around a launch when profiling is enabled (no-ops otherwise).
 int prof_begin(dis_ctx* c, int kclass);  // returns a slot or -1
