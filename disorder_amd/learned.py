@@ -168,17 +168,29 @@ def level_scale(L):
 class Proposer:
     """Per-iteration proposal step for pyramid.solveXT: returns the motion
     after the proposal and the mask of states that took it (their energy
-    decreased); LMsolver then handles the other states."""
+    decreased); LMsolver then handles the other states.  The network's update
+    is tried at the step fractions `alphas` in turn (one energy evaluation
+    each, a shot keeps the first fraction that lowers its energy), on the
+    first `max_outer` outer iterations of every level only (where the motion
+    errors the network was trained on are largest)."""
 
-    def __init__(self, net: MotionNet):
+    def __init__(self, net: MotionNet, alphas=(1.0,), max_outer=2):
         self.net = net
+        self.alphas = tuple(alphas)
+        self.max_outer = max_outer
         self.accepted = 0
         self.proposed = 0
+        self._level = None
+        self._calls = 0
 
     def __call__(self, L, x, T, cT):
+        if self._level is not L:
+            self._level, self._calls = L, 0
+        self._calls += 1
         active = ~cT
-        if not active.any():
-            return T, np.zeros(len(T), bool)
+        none = np.zeros(len(T), bool)
+        if not active.any() or self._calls > self.max_outer:
+            return T, none
         sums = lm_sums(L.y, L.E, x, T)
         # translations in voxels of the level grid
         scale = level_scale(L)
@@ -187,13 +199,21 @@ class Proposer:
         F = torch.from_numpy(Fh).to(x.device)
         Y = self.net.forward(F, [0, F.shape[0]]).cpu().numpy()
         Y[:, 0:3] *= scale
-        Tp = T + to_update(Y)
-        Tp[:, 3:] = (Tp[:, 3:] + math.pi) % (2 * math.pi) - math.pi  # restrictTransform.m:28-29
-        e_new = state_energy(L.E, x, L.y, Tp, L.off)
-        take = active & (e_new < sums[:, 0])
+        d = to_update(Y)
+        Tn = T.copy()
+        take = none.copy()
+        for a in self.alphas:
+            todo = active & ~take
+            if not todo.any():
+                break
+            Tp = T + a * d
+            Tp[:, 3:] = (Tp[:, 3:] + math.pi) % (2 * math.pi) - math.pi  # restrictTransform.m:28-29
+            e_new = state_energy(L.E, x, L.y, Tp, L.off)
+            ok = todo & (e_new < sums[:, 0])
+            Tn[ok] = Tp[ok]
+            take |= ok
         self.proposed += int(active.sum())
         self.accepted += int(take.sum())
-        Tn = np.where(take[:, None], Tp, T)
         return Tn, take
 
 
