@@ -136,32 +136,41 @@ struct CgVol {
   void* base;
 };
 
-int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* const* T, const float2* const* y,
-                   const float* const* Se, float2* const* x, int nIt, bool use_graph, double* res, double* ms) {
-  hipStream_t st = ctx_stream(c);
-  std::vector<CgVol> vs(nvol);
-  auto release = [&]() {
+namespace {
+
+// Everything the batched solver holds on the device or in the context,
+// released on every return path (also after a failed capture).
+struct BatchedScope {
+  dis_ctx* c;
+  hipStream_t st;
+  std::vector<CgVol> vs;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  int prof_restore = -1;
+  BatchedScope(dis_ctx* c_, int nvol) : c(c_), st(ctx_stream(c_)), vs(nvol) {}
+  ~BatchedScope() {
     (void)hipStreamSynchronize(st);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
     for (auto& v : vs)
       if (v.base) (void)hipFree(v.base);
     ctx_tape_free(c);
-  };
-  hipEvent_t e0 = nullptr, e1 = nullptr, e1b = nullptr, e2 = nullptr;
-  CCHK(c, hipEventCreate(&e0));
-  CCHK(c, hipEventCreate(&e1));
-  CCHK(c, hipEventCreate(&e1b));
-  CCHK(c, hipEventCreate(&e2));
-  auto destroy_events = [&]() {
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e1b);
-    (void)hipEventDestroy(e2);
-  };
-  auto fail_all = [&](int rc) {
-    release();
-    destroy_events();
-    return rc;
-  };
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (prof_restore >= 0) ctx_profiling(c, prof_restore);
+  }
+};
+
+}  // namespace
+
+int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* const* T, const float2* const* y,
+                   const float* const* Se, float2* const* x, int nIt, bool use_graph, double* res, double* ms) {
+  BatchedScope sc(c, nvol);
+  hipStream_t st = sc.st;
+  std::vector<CgVol>& vs = sc.vs;
+  for (auto& e : sc.ev) CCHK(c, hipEventCreate(&e));
+  hipEvent_t e0 = sc.ev[0], e1 = sc.ev[1], e1b = sc.ev[2], e2 = sc.ev[3];
   CCHK(c, hipEventRecord(e0, st));
   // setup: b = decode(y), x = 0, r = b, z = Se r, p = z (CGsolver.m:46-68)
   for (int i = 0; i < nvol; ++i) {
@@ -173,7 +182,7 @@ int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* cons
     if (hipMalloc(&v.base, bytes) != hipSuccess) {
       v.base = nullptr;
       (void)hipGetLastError();
-      return fail_all(ctx_fail(c, DIS_E_NOMEM, "batched CG: state allocation failed"));
+      return ctx_fail(c, DIS_E_NOMEM, "batched CG: state allocation failed");
     }
     v.r = (float2*)v.base;
     v.z = v.r + v.n;
@@ -185,7 +194,7 @@ int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* cons
     v.hist = (double*)((char*)v.base + hist);
     v.cnt = (int*)((char*)v.base + cnt);
     int rc = op_apply(c, views[i], T[i], nullptr, y[i], nullptr, v.r, COIL_DECODE, nullptr);
-    if (rc) return fail_all(rc);
+    if (rc) return rc;
     CCHK(c, hipMemsetAsync(x[i], 0, vb, st));
     CCHK(c, hipMemsetAsync(v.cnt, 0, sizeof(int), st));
     CCHK(c, cg_residual(v.r, nullptr, v.r, v.z, v.p, Se ? Se[i] : nullptr, v.n, v.part, v.part2, &v.sc->rsold,
@@ -219,52 +228,29 @@ int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* cons
     // iteration 1 records every coefficient upload on the tape
     ctx_tape_mode(c, 1);
     int rc = iteration();
-    if (rc) return fail_all(rc);
+    if (rc) return rc;
     CCHK(c, hipStreamSynchronize(st));
     CCHK(c, hipEventRecord(e1b, st));
     if (nIt > 1) {
-      const bool prof = ctx_profiling(c, 0);
+      sc.prof_restore = ctx_profiling(c, 0) ? 1 : 0;
+      ctx_tape_mode(c, 2);
       if (use_graph) {
-        ctx_tape_mode(c, 2);
-        hipGraph_t g = nullptr;
-        hipGraphExec_t ge = nullptr;
         CCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         rc = iteration();
-        hipGraph_t gc = nullptr;
-        const hipError_t ec = hipStreamEndCapture(st, &gc);
-        g = gc;
-        if (rc || ec != hipSuccess) {
-          if (g) (void)hipGraphDestroy(g);
-          ctx_profiling(c, prof ? 1 : 0);
-          return fail_all(rc ? rc : ctx_fail(c, DIS_E_HIP, "batched CG: graph capture failed"));
-        }
-        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
-          (void)hipGraphDestroy(g);
-          ctx_profiling(c, prof ? 1 : 0);
-          return fail_all(ctx_fail(c, DIS_E_HIP, "batched CG: graph instantiation failed"));
-        }
-        for (int k = 2; k <= nIt; ++k) {
-          if (hipGraphLaunch(ge, st) != hipSuccess) {
-            (void)hipGraphExecDestroy(ge);
-            (void)hipGraphDestroy(g);
-            ctx_profiling(c, prof ? 1 : 0);
-            return fail_all(ctx_fail(c, DIS_E_HIP, "batched CG: graph launch failed"));
-          }
-        }
-        CCHK(c, hipStreamSynchronize(st));
-        (void)hipGraphExecDestroy(ge);
-        (void)hipGraphDestroy(g);
+        const hipError_t ec = hipStreamEndCapture(st, &sc.g);
+        if (rc) return rc;
+        if (ec != hipSuccess) return ctx_fail(c, DIS_E_HIP, "batched CG: graph capture failed");
+        if (hipGraphInstantiate(&sc.ge, sc.g, nullptr, nullptr, 0) != hipSuccess)
+          return ctx_fail(c, DIS_E_HIP, "batched CG: graph instantiation failed");
+        for (int k = 2; k <= nIt; ++k)
+          if (hipGraphLaunch(sc.ge, st) != hipSuccess) return ctx_fail(c, DIS_E_HIP, "batched CG: graph launch failed");
       } else {
         for (int k = 2; k <= nIt; ++k) {
           ctx_tape_mode(c, 2);
           rc = iteration();
-          if (rc) {
-            ctx_profiling(c, prof ? 1 : 0);
-            return fail_all(rc);
-          }
+          if (rc) return rc;
         }
       }
-      ctx_profiling(c, prof ? 1 : 0);
     }
   } else {
     CCHK(c, hipEventRecord(e1b, st));
@@ -290,8 +276,6 @@ int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* cons
       for (int k = 0; k < nIt; ++k) res[(size_t)i * nIt + k] = (k < cnt && bn > 0) ? std::sqrt(h[k] / bn) : 0.0;
     }
   }
-  release();
-  destroy_events();
   return DIS_OK;
 }
 

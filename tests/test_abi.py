@@ -42,3 +42,25 @@ def test_null_arguments_are_rejected_without_a_device():
     assert lib.dis_encode(None, None, None, None) == _lib.DIS_E_INVALID
     assert lib.dis_plan_destroy(None) == _lib.DIS_E_INVALID
     assert lib.dis_last_error(None) == b"null context"
+
+
+def test_new_entry_points_reject_bad_arguments_without_a_device():
+    import ctypes
+    lib = _lib.load()
+    assert lib.dis_mlp_param_count() == 256 * 13 + 256 + 256 * 256 + 256 + 6 * 256 + 6 + 6 * 6 * 5 + 6
+    assert lib.dis_set_coil_schedule(None, 0) == _lib.DIS_E_INVALID
+    assert lib.dis_autotune(None, None, None, 1, None, None) == _lib.DIS_E_INVALID
+    assert lib.dis_autotune_clear(None) == _lib.DIS_E_INVALID
+    assert lib.dis_set_pass_shape(None, None, 0, 0, 0, 16) == _lib.DIS_E_INVALID
+    assert lib.dis_cg_batched(None, 0, None, None, None, None, None, 1, 1, None, None) == _lib.DIS_E_INVALID
+    assert lib.dis_coil_gram(None, None, 0, 0, None) == _lib.DIS_E_INVALID
+    assert lib.dis_coil_basis(None, 4, None, 0, None, None, None) == _lib.DIS_E_INVALID
+    assert lib.dis_resample(None, None, None, 1, None, None) == _lib.DIS_E_INVALID
+    assert lib.dis_mlp_forward(None, None, None, 1, None, 1, None) == _lib.DIS_E_INVALID
+    # the host eigensolver needs no device
+    P = (ctypes.c_double * 8)(2.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0)
+    perc = (ctypes.c_double * 1)(0.5)
+    cc = (ctypes.c_int32 * 1)()
+    D = (ctypes.c_double * 2)()
+    assert lib.dis_coil_basis(P, 2, perc, 1, cc, D, None) == _lib.DIS_OK
+    assert cc[0] == 1 and D[0] == 2.0 and D[1] == 1.0
