@@ -16,11 +16,16 @@ repeat), driving the C-ABI operators level by level:
   each outer iteration: motion-convergence reset schedule (:376-381), CG with
   'RelativeResidual2Error', tol 0 and n_cg iterations (:431), one LMsolver
   call (:507); when every state has converged a final 'Energy' CG closes the
-  level (:425-431, :523).
+  level (:425-431, :523);
+  at the full-resolution level (AlignedRec = 2, leff = 0) the per-sample
+  residuals of computeEnergy after each CG give robust per-shot weights
+  (disorder_amd.robust, :441-481) that decode and the later CG calls of the
+  level apply as EH.We (cleared when a level starts, :352); LMsolver and
+  computeEnergy do not use them.
 
 Omitted from solveXT (documented deviations): the Gibbs-ringing apodisation
-of y (downsampleOperators.m:54-57), the robust per-shot weights and
-within-shot subdivision (:446-499), compressMotion binning (:392).
+of y (downsampleOperators.m:54-57), within-shot subdivision (:482-499),
+compressMotion binning (:392).
 """
 from __future__ import annotations
 
@@ -29,7 +34,7 @@ import time
 
 import numpy as np
 
-from . import ops
+from . import ops, robust as RB
 from .sampling import level_sampling
 
 
@@ -59,7 +64,8 @@ class Level:
 
 
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
-            echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None):
+            echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
+            robust=True):
     """Returns (x [N] on the device, T [S x 6], info).  callback(level, it, x,
     T, t_solve) runs after every outer iteration (and after a level's final
     CG, with outer = last + 1) outside the timed region
@@ -68,7 +74,8 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     proposer(level_ops, x, T, cT) -> (T', take) may propose motion before
     each LM call: states in `take` kept a proposal that lowered their energy
     (computeEnergy) and skip this LM call; LMsolver runs on the others, so
-    the plain method stays the fallback."""
+    the plain method stays the fallback.  robust: robust per-shot weights
+    at the full-resolution level (info['We'] holds the last ones)."""
     import torch
     ctx = ctx or ops.default_context(y.device.index or 0)
     S_states = len(off) - 1
@@ -79,6 +86,8 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     cb_time = 0.0
     for li, (n, resIso) in enumerate(pyramid_levels(N, nlevels)):
         L = Level(y, S, k1, k2, off, N, n, ctx)
+        full_res = list(n) == list(N)
+        st_of_sample = np.repeat(np.arange(S_states), np.diff(L.off))
         if x is None:
             x = ops.empty_colmajor(n, y.device)
             x.zero_()
@@ -97,6 +106,11 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                 cT = cT0.copy()
             cont += 1
             x, _, _ = ops.CGsolver(L.y, L.E, T, x=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0, Se=L.Se)
+            if robust and full_res:
+                ps = ops.computeEnergy(L.y, x, L.E, T, per_sample=True)
+                We, _ = RB.shot_weights(ps, st_of_sample, S_states, finer=False)
+                L.E.set_weights(RB.sample_weights(We, st_of_sample))
+                info["We"] = We
             if proposer is not None:
                 T_before = T.copy()
                 T, take = proposer(L, x, T, cT)

@@ -846,6 +846,59 @@ def level_size(N, l):
 
 
 # ----------------------------------------------------------------------------
+# Robust per-shot weights (Reconstruction/solveXT.m:455-468)
+# ----------------------------------------------------------------------------
+def prctile(x, p):
+    """MATLAB prctile of a vector: the i-th smallest of n values sits at
+    percentile 100 (i - 0.5) / n; linear interpolation in between, the
+    minimum / maximum outside [100 * 0.5 / n, 100 (n - 0.5) / n]."""
+    xs = np.sort(np.asarray(x, np.float64).ravel())
+    n = len(xs)
+    out = []
+    for q in np.atleast_1d(p):
+        pos = q / 100.0 * n + 0.5  # 1-based fractional index
+        if pos <= 1:
+            out.append(xs[0])
+        elif pos >= n:
+            out.append(xs[-1])
+        else:
+            i = int(math.floor(pos))
+            f = pos - i
+            out.append(xs[i - 1] + f * (xs[i] - xs[i - 1]))
+    return np.array(out)
+
+
+def robust_shot_weights(per_sample, state_of_sample, nshots, finer=True, perc=(0.125, 0.25), ener=0.95):
+    """solveXT.m:455-468: per shot the percentiles (15:5:85 % when finer,
+    else 30:5:70 %) of the log residual energies of its samples; across
+    shots a robust std from the perc percentiles scaled by the normal
+    quantiles and a robust mean from their mid percentile; per shot the mean
+    z-score, and We = min((1 - normcdf(z)) / ((1 - ener) / nshots), 1).
+    A shot without samples has NaN percentiles, which prctile across shots
+    skips; its weight is 1 (it weights no sample)."""
+    from scipy.special import erfcinv, ndtr, ndtri
+    pu = np.arange(15, 86, 5) if finer else np.arange(30, 71, 5)
+    logr = np.log(np.asarray(per_sample, np.float64))
+    st = np.asarray(state_of_sample)
+    rows, live = [], []
+    for s in range(nshots):
+        v = logr[st == s]
+        if len(v):
+            rows.append(prctile(v, pu))
+            live.append(s)
+    WeO = np.array(rows)
+    lo = np.array([prctile(WeO[:, j], [perc[0] * 100, perc[1] * 100]) for j in range(len(pu))])  # [nperc, 2]
+    Westd = (lo[:, 1] - lo[:, 0]) / (ndtri(perc[1]) - ndtri(perc[0]))
+    pm = float(np.mean(perc))
+    Wmea = np.array([prctile(WeO[:, j], [pm * 100])[0] for j in range(len(pu))])
+    Wmea = Wmea + Westd * math.sqrt(2.0) * erfcinv(2 * pm)
+    z = ((WeO - Wmea[None, :]) / Westd[None, :]).mean(axis=1)
+    We = np.ones(nshots)
+    We[live] = np.minimum((1.0 - ndtr(z)) / ((1.0 - ener) / nshots), 1.0)
+    return We
+
+
+# ----------------------------------------------------------------------------
 # Pyramid driver (Reconstruction/solveXT.m:266-531, joint CG + LM per level)
 # ----------------------------------------------------------------------------
 def _level_kspace_rows(k1, k2, off, N12, n12):
@@ -876,7 +929,8 @@ def pyramid_levels(N, nlevels):
 
 
 def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
-                  traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None):
+                  traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
+                  info=None):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -887,7 +941,11 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     then one LMsolver call (:507); when every state has converged, a final
     CG of final_cg iterations with 'Energy' tol 1e-3 closes the level when
     an outer iteration is left for it (:425-431 with estT(l) = 0 and nIt(1),
-    then the break of :523).  Returns (x at full size, T)."""
+    then the break of :523).  With robust (AlignedRec = 2), at the
+    full-resolution level the per-sample residuals after each CG give the
+    robust per-shot weights EH.We used by the later CG calls (:441-481);
+    info (a dict, optional) receives the last shot weights as info['We'].
+    Returns (x at full size, T)."""
     S = len(off) - 1
     T = np.zeros((S, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
     x = None
@@ -897,6 +955,7 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
         yl = resample_fourier(yc[rows] * scale, (len(rows), n[2], yc.shape[2]))
         Sl = resample_fourier(Smaps, tuple(n) + (Smaps.shape[3],))
         samp = Sampling(l1, l2, loff, n[0], n[1])
+        full_res = list(n) == list(N)
         Se = 1.0 / (np.sum(np.abs(Sl) ** 2, axis=-1) + 1e-9)
         x = np.zeros(n, np.complex128) if x is None else resample_fourier(x, n)
         w = np.full(S, winit)
@@ -913,6 +972,12 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
             cont += 1
             x, _, _ = cg_solver(yl, Sl, samp, T, N, x0=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0,
                                 Se=Se)
+            if robust and full_res:
+                ps = compute_energy(yl, x, Sl, Sampling(l1, l2, loff, n[0], n[1]), T, N, per_sample=True)
+                Wshot = robust_shot_weights(ps, samp.state_of_sample(), S, finer=False)
+                samp.We = Wshot[samp.state_of_sample()]
+                if info is not None:
+                    info["We"] = Wshot
             T, w, cT, _ = lm_solver(yl, x, Sl, samp, T, N, w, cT, winit, tL)
             if callback is not None:
                 callback(li, it, x, T)
