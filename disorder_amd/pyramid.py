@@ -21,11 +21,13 @@ repeat), driving the C-ABI operators level by level:
   residuals of computeEnergy after each CG give robust per-shot weights
   (disorder_amd.robust, :441-481) that decode and the later CG calls of the
   level apply as EH.We (cleared when a level starts, :352); LMsolver and
-  computeEnergy do not use them.
+  computeEnergy do not use them;
+  the image solves encode with the motion binned by compressMotion (:392,
+  disorder_amd.motion), LMsolver with the per-shot motion (:496).
 
 Omitted from solveXT (documented deviations): the Gibbs-ringing apodisation
-of y (downsampleOperators.m:54-57), within-shot subdivision (:482-499),
-compressMotion binning (:392).
+of y (downsampleOperators.m:54-57), within-shot subdivision (:270-287,
+:472-478).
 """
 from __future__ import annotations
 
@@ -34,6 +36,7 @@ import time
 
 import numpy as np
 
+from . import motion as MB
 from . import ops, robust as RB
 from .sampling import level_sampling
 
@@ -48,24 +51,87 @@ def pyramid_levels(N, nlevels):
 
 
 class Level:
-    """Operators of one pyramid level, resident on the device."""
+    """Operators of one pyramid level, resident on the device.
 
-    def __init__(self, y, S, k1, k2, off, N, n, ctx):
+    Coil subsets are prefixes of the (compressed, energy-ordered) coil axis,
+    so each is a view of the level's maps and data.  E and y are the LMsolver
+    pair (lm_coils); image solves ask cg(ncoil, T) for their encoding,
+    motion, data and preconditioner."""
+
+    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None):
         self.n = list(n)
         rows, l1, l2, loff = level_sampling(k1, k2, off, N[:2], n[:2])
         self.rows, self.off = rows, loff
         scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
         C = int(S.shape[3])
         g = ops.gather_samples(y, rows, scale, ctx)
-        self.y = ops.resampling(g, (len(rows), n[2], C), ctx) if n[2] != N[2] else g
-        self.S = ops.resampling(S, n, ctx) if list(n) != list(N) else S
-        self.Se = ops.coil_precond(self.S, 1e-9, ctx)
-        self.E = ops.Encoding(self.S, l1, l2, loff, Nfull=N, ctx=ctx)
+        self.yall = ops.resampling(g, (len(rows), n[2], C), ctx) if n[2] != N[2] else g
+        self.Sall = ops.resampling(S, n, ctx) if list(n) != list(N) else S
+        self.k1, self.k2, self.N, self.ctx = l1, l2, N, ctx
+        self.C = C
+        self.We = None
+        self._enc = {}     # ncoil -> per-shot encoding
+        self._binned = {}  # ncoil -> (offsets, encoding) of the last binning
+        self._Se = {}
+        lm = C if lm_coils is None else lm_coils
+        self.E, self.y = self.encoding(lm), self.data(lm)
+
+    def _check(self, ncoil):
+        if not 1 <= ncoil <= self.C:
+            raise ValueError(f"coil count {ncoil} outside 1..{self.C}")
+
+    def data(self, ncoil):
+        self._check(ncoil)
+        return self.yall[..., :ncoil]
+
+    def encoding(self, ncoil):
+        self._check(ncoil)
+        if ncoil not in self._enc:
+            self._enc[ncoil] = ops.Encoding(self.Sall[..., :ncoil], self.k1, self.k2, self.off, Nfull=self.N,
+                                            We=self.We, ctx=self.ctx)
+        return self._enc[ncoil]
+
+    def precond(self, ncoil):
+        """A.Se = 1 / (sum_c |S_c|^2 + 1e-9) over the first ncoil coils."""
+        self._check(ncoil)
+        if ncoil not in self._Se:
+            self._Se[ncoil] = ops.coil_precond(self.Sall[..., :ncoil], 1e-9, self.ctx)
+        return self._Se[ncoil]
+
+    def set_weights(self, We):
+        """EH.We per sample, for every encoding of the level."""
+        self.We = We
+        for E in list(self._enc.values()) + [e for _, e in self._binned.values()]:
+            E.set_weights(We)
+
+    def cg(self, ncoil, T, res=None, traLimX=None):
+        """(encoding, motion, data, preconditioner) for an image solve over
+        the first ncoil coils; with res, shots binned by bin_motion (the
+        per-shot encoding when every bin holds one shot)."""
+        if res is None:
+            return self.encoding(ncoil), T, self.data(ncoil), self.precond(ncoil)
+        Tb, offb, _ = MB.bin_motion(T, self.off, res, traLimX)
+        if len(offb) == len(self.off):
+            return self.encoding(ncoil), Tb, self.data(ncoil), self.precond(ncoil)
+        prev = self._binned.get(ncoil)
+        if prev is None or not np.array_equal(prev[0], offb):
+            if prev is not None:
+                prev[1].close()
+            self._binned[ncoil] = (offb, ops.Encoding(self.Sall[..., :ncoil], self.k1, self.k2, offb,
+                                                      Nfull=self.N, We=self.We, ctx=self.ctx))
+        return self._binned[ncoil][1], Tb, self.data(ncoil), self.precond(ncoil)
+
+    def close(self):
+        for _, E in self._binned.values():
+            E.close()
+        for E in self._enc.values():
+            E.close()
+        self._binned, self._enc = {}, {}
 
 
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
-            robust=True):
+            robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None):
     """Returns (x [N] on the device, T [S x 6], info).  callback(level, it, x,
     T, t_solve) runs after every outer iteration (and after a level's final
     CG, with outer = last + 1) outside the timed region
@@ -75,7 +141,15 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     each LM call: states in `take` kept a proposal that lowered their energy
     (computeEnergy) and skip this LM call; LMsolver runs on the others, so
     the plain method stays the fallback.  robust: robust per-shot weights
-    at the full-resolution level (info['We'] holds the last ones)."""
+    at the full-resolution level (info['We'] holds the last ones).
+    bin_motion: the image solves and the energies for the weights use the
+    binned motion (disorder_amd.motion.bin_motion at the resolution of the
+    finest level); LMsolver keeps the per-shot motion.
+    eivaS: coil counts (final image, motion estimation, image solves
+    while estimating motion) as compressCoils returns them for three
+    energy fractions (solveXT.m:401,501): CG calls use the first eivaS[2]
+    coils, the closing CG of a level the first eivaS[0], LMsolver (and a
+    proposer) the first eivaS[1]; None uses every coil everywhere."""
     import torch
     ctx = ctx or ops.default_context(y.device.index or 0)
     S_states = len(off) - 1
@@ -84,8 +158,17 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     info = {"levels": [], "callback_s": 0.0, "lm_trials": 0, "proposals_accepted": 0}
     t0 = time.perf_counter()
     cb_time = 0.0
+    res_fine = pyramid_levels(N, nlevels)[-1][1]
+
+    C = int(S.shape[3])
+    ev = [C, C, C] if eivaS is None else [int(v) for v in eivaS]
+    if len(ev) != 3:
+        raise ValueError("eivaS holds three coil counts")
+
+    def cg_ops(L, T, final):
+        return L.cg(ev[0] if final else ev[2], T, *((res_fine, traLimX) if bin_motion else ()))
     for li, (n, resIso) in enumerate(pyramid_levels(N, nlevels)):
-        L = Level(y, S, k1, k2, off, N, n, ctx)
+        L = Level(y, S, k1, k2, off, N, n, ctx, lm_coils=ev[1])
         full_res = list(n) == list(N)
         st_of_sample = np.repeat(np.arange(S_states), np.diff(L.off))
         if x is None:
@@ -105,11 +188,12 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                 cont = 0
                 cT = cT0.copy()
             cont += 1
-            x, _, _ = ops.CGsolver(L.y, L.E, T, x=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0, Se=L.Se)
+            Ec, Tc, yc, Se = cg_ops(L, T, False)
+            x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0, Se=Se)
             if robust and full_res:
-                ps = ops.computeEnergy(L.y, x, L.E, T, per_sample=True)
+                ps = ops.computeEnergy(yc, x, Ec, Tc, per_sample=True)
                 We, _ = RB.shot_weights(ps, st_of_sample, S_states, finer=False)
-                L.E.set_weights(RB.sample_weights(We, st_of_sample))
+                L.set_weights(RB.sample_weights(We, st_of_sample))
                 info["We"] = We
             if proposer is not None:
                 T_before = T.copy()
@@ -130,7 +214,8 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                 cb_time += time.perf_counter() - tc
             if cT.all():
                 if final_cg and it < n_outer:
-                    x, _, _ = ops.CGsolver(L.y, L.E, T, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=L.Se)
+                    Ec, Tc, yc, Se = cg_ops(L, T, True)
+                    x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
                     if callback is not None:
                         torch.cuda.synchronize()
                         tc = time.perf_counter()
@@ -138,7 +223,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                         cb_time += time.perf_counter() - tc
                 break
         info["levels"].append({"n": n, "outer": its, "states_converged": int(cT.sum())})
-        L.E.close()
+        L.close()
     if list(x.shape) != list(N):
         x = ops.resampling(x, N, ctx)
     torch.cuda.synchronize()

@@ -899,6 +899,105 @@ def robust_shot_weights(per_sample, state_of_sample, nshots, finer=True, perc=(0
 
 
 # ----------------------------------------------------------------------------
+# Motion binning (Methods/compressMotion.m:27-80)
+# ----------------------------------------------------------------------------
+def haar_dwt(x):
+    """One level of MATLAB dwt(x, 'haar') in the default 'sym' extension
+    mode: x is extended by one mirrored sample at each end, filtered with
+    Lo_D = [1 1]/sqrt2 and Hi_D = [-1 1]/sqrt2 ('valid' convolution, length
+    n + 1) and the even-indexed outputs are kept: a_k = (x_{2k-1} + x_{2k})
+    / sqrt2, d_k = (x_{2k-1} - x_{2k}) / sqrt2 (1-based), with x_{n+1} = x_n
+    for odd n."""
+    x = [float(v) for v in x]
+    n = len(x)
+    ext = [x[0]] + x + [x[-1]]
+    r = 1.0 / math.sqrt(2.0)
+    a, d = [], []
+    for k in range(2, n + 2, 2):  # 1-based indices of the 'valid' output kept
+        lo = r * ext[k - 1] + r * ext[k]       # y(k) f(2) + y(k+1) f(1), f = Lo_D
+        hi = r * ext[k - 1] - r * ext[k]       # f = Hi_D = [-r, r]
+        a.append(lo)
+        d.append(hi)
+    return np.array(a), np.array(d)
+
+
+def haar_idwt(a, d, n):
+    """MATLAB idwt(a, d, 'haar', n): zero-interleave (dyadup), full
+    convolution with Lo_R = [1 1]/sqrt2 and Hi_R = [1 -1]/sqrt2, keep the
+    central n samples (for 2m -> 2m - 1 the last one is dropped)."""
+    r = 1.0 / math.sqrt(2.0)
+    out = []
+    for ak, dk in zip(a, d):
+        out.append(r * ak + r * dk)
+        out.append(r * ak - r * dk)
+    extra = len(out) - n
+    first = extra // 2
+    return np.array(out[first:first + n])
+
+
+def wavedec_haar(x, NL):
+    """MATLAB [C, L] = wavedec(x, NL, 'haar'): C = [a_NL, d_NL, ..., d_1],
+    L = [len(a_NL), len(d_NL), ..., len(d_1), len(x)]."""
+    a = np.asarray(x, np.float64)
+    ds, lens = [], [len(a)]
+    for _ in range(NL):
+        a, d = haar_dwt(a)
+        ds.append(d)
+        lens.append(len(d))
+    C = np.concatenate([a] + ds[::-1])
+    L = [len(a)] + lens[1:][::-1] + [lens[0]]
+    return C, L
+
+
+def waverec_haar(C, L):
+    """MATLAB waverec(C, L, 'haar')."""
+    NL = len(L) - 2
+    a = np.asarray(C[:L[0]], np.float64)
+    pos = L[0]
+    for j in range(NL):
+        d = C[pos:pos + L[1 + j]]
+        pos += L[1 + j]
+        a = haar_idwt(a, d, L[2 + j])
+    return a
+
+
+def compress_motion(T, off, res, traLimX=(0.05, 0.02)):
+    """compressMotion.m:27-80 for states that all carry motion (no outer
+    segments): each of the 6 parameter series over the S states is
+    decomposed to NL = floor(log2 S) Haar levels, coefficients below
+    traLimX(1) res (translations, voxels) or traLimX(2) res degrees in
+    radians (rotations) are zeroed and the series rebuilt (:30-36); runs of
+    consecutive states whose rebuilt parameters agree to 1e-6 with the run's
+    first state form one bin (:40-52) with the mean rebuilt parameters (:57-61)
+    and the run's samples (their lists concatenated in state order).
+    Returns (Tb [B x 6], off_b [B + 1], bin_of_state [S])."""
+    T = np.atleast_2d(np.asarray(T, np.float64))
+    S = T.shape[0]
+    off = np.asarray(off)
+    if S <= 1:
+        return T.copy(), off.copy(), np.zeros(S, np.int64)
+    NL = int(math.floor(math.log2(S)))
+    Test = np.empty_like(T)
+    for m in range(6):
+        th = traLimX[0] * res if m < 3 else traLimX[1] * res * math.pi / 180.0
+        C, L = wavedec_haar(T[:, m], NL)
+        C = np.where(np.abs(C) < th, 0.0, C)
+        Test[:, m] = waverec_haar(C, L)
+    starts = []
+    n = 0
+    while n < S:
+        m = 1
+        while n + m < S and not np.any(np.abs(Test[n + m] - Test[n]) > 1e-6):
+            m += 1
+        starts.append((n, m))
+        n += m
+    Tb = np.array([Test[a:a + m].mean(axis=0) for a, m in starts])
+    off_b = np.array([off[a] for a, _ in starts] + [off[S]])
+    bin_of_state = np.concatenate([np.full(m, b) for b, (_, m) in enumerate(starts)])
+    return Tb, off_b, bin_of_state
+
+
+# ----------------------------------------------------------------------------
 # Pyramid driver (Reconstruction/solveXT.m:266-531, joint CG + LM per level)
 # ----------------------------------------------------------------------------
 def _level_kspace_rows(k1, k2, off, N12, n12):
@@ -930,7 +1029,7 @@ def pyramid_levels(N, nlevels):
 
 def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
                   traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
-                  info=None):
+                  info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -945,10 +1044,18 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     full-resolution level the per-sample residuals after each CG give the
     robust per-shot weights EH.We used by the later CG calls (:441-481);
     info (a dict, optional) receives the last shot weights as info['We'].
+    With bin_motion, every CG and the energies for the weights use the
+    motion binned by compress_motion (res = resIso of the finest level,
+    solveXT.m:392) while LMsolver keeps the per-shot motion (:496).  eivaS = (final,
+    motion, image-while-estimating) coil counts: CG uses the first eivaS[2]
+    coils (its preconditioner too), the closing CG eivaS[0], LMsolver
+    eivaS[1] (:401-407, :501-503).
     Returns (x at full size, T)."""
     S = len(off) - 1
     T = np.zeros((S, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
     x = None
+    res_fine = pyramid_levels(N, nlevels)[-1][1]
+    ev = [Smaps.shape[3]] * 3 if eivaS is None else list(eivaS)
     for li, (n, resIso) in enumerate(pyramid_levels(N, nlevels)):
         rows, l1, l2, loff = _level_kspace_rows(k1, k2, off, N[:2], n[:2])
         scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
@@ -956,7 +1063,14 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
         Sl = resample_fourier(Smaps, tuple(n) + (Smaps.shape[3],))
         samp = Sampling(l1, l2, loff, n[0], n[1])
         full_res = list(n) == list(N)
-        Se = 1.0 / (np.sum(np.abs(Sl) ** 2, axis=-1) + 1e-9)
+
+        def binned(T, We):
+            if not bin_motion:
+                return Sampling(l1, l2, loff, n[0], n[1], We), T
+            Tb, offb, _ = compress_motion(T, loff, res_fine, traLimX)
+            return Sampling(l1, l2, offb, n[0], n[1], We), Tb
+        Se = 1.0 / (np.sum(np.abs(Sl[..., :ev[2]]) ** 2, axis=-1) + 1e-9)
+        Sef = 1.0 / (np.sum(np.abs(Sl[..., :ev[0]]) ** 2, axis=-1) + 1e-9)
         x = np.zeros(n, np.complex128) if x is None else resample_fourier(x, n)
         w = np.full(S, winit)
         nsamp = np.diff(loff)
@@ -970,19 +1084,25 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
                 cont = 0
                 cT = cT0.copy()
             cont += 1
-            x, _, _ = cg_solver(yl, Sl, samp, T, N, x0=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0,
-                                Se=Se)
+            samp_b, Tb = binned(T, samp.We)
+            c = ev[2]
+            x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=n_cg,
+                                tolType="RelativeResidual2Error", tol=0.0, Se=Se)
             if robust and full_res:
-                ps = compute_energy(yl, x, Sl, Sampling(l1, l2, loff, n[0], n[1]), T, N, per_sample=True)
+                ps = compute_energy(yl[..., :c], x, Sl[..., :c], Sampling(l1, l2, samp_b.off, n[0], n[1]), Tb, N,
+                                    per_sample=True)
                 Wshot = robust_shot_weights(ps, samp.state_of_sample(), S, finer=False)
                 samp.We = Wshot[samp.state_of_sample()]
                 if info is not None:
                     info["We"] = Wshot
-            T, w, cT, _ = lm_solver(yl, x, Sl, samp, T, N, w, cT, winit, tL)
+            T, w, cT, _ = lm_solver(yl[..., :ev[1]], x, Sl[..., :ev[1]], samp, T, N, w, cT, winit, tL)
             if callback is not None:
                 callback(li, it, x, T)
             if cT.all():
                 if final_cg and it < n_outer:
-                    x, _, _ = cg_solver(yl, Sl, samp, T, N, x0=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
+                    samp_b, Tb = binned(T, samp.We)
+                    c = ev[0]
+                    x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=final_cg,
+                                        tolType="Energy", tol=1e-3, Se=Sef)
                 break
     return resample_fourier(x, N), T
