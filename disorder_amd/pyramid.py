@@ -25,9 +25,11 @@ repeat), driving the C-ABI operators level by level:
   the image solves encode with the motion binned by compressMotion (:392,
   disorder_amd.motion), LMsolver with the per-shot motion (:496).
 
+  after the spatial levels, n_within full-resolution levels split the
+  states of outlier shots in two (:270-287, :470-478).
+
 Omitted from solveXT (documented deviations): the Gibbs-ringing apodisation
-of y (downsampleOperators.m:54-57), within-shot subdivision (:270-287,
-:472-478).
+of y (downsampleOperators.m:54-57).
 """
 from __future__ import annotations
 
@@ -58,13 +60,13 @@ class Level:
     pair (lm_coils); image solves ask cg(ncoil, T) for their encoding,
     motion, data and preconditioner."""
 
-    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None):
+    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None):
         self.n = list(n)
         rows, l1, l2, loff = level_sampling(k1, k2, off, N[:2], n[:2])
         self.rows, self.off = rows, loff
         scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
         C = int(S.shape[3])
-        g = ops.gather_samples(y, rows, scale, ctx)
+        g = ops.gather_samples(y, rows if rows0 is None else np.asarray(rows0)[rows], scale, ctx)
         self.yall = ops.resampling(g, (len(rows), n[2], C), ctx) if n[2] != N[2] else g
         self.Sall = ops.resampling(S, n, ctx) if list(n) != list(N) else S
         self.k1, self.k2, self.N, self.ctx = l1, l2, N, ctx
@@ -131,17 +133,17 @@ class Level:
 
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
-            robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None):
-    """Returns (x [N] on the device, T [S x 6], info).  callback(level, it, x,
-    T, t_solve) runs after every outer iteration (and after a level's final
-    CG, with outer = last + 1) outside the timed region
+            robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None):
+    """Returns (x [N] on the device, T [states x 6], info).  callback(level,
+    it, x, T, t_solve) runs after every outer iteration (and after a level's
+    final CG, with outer = last + 1) outside the timed region
     (t_solve: solve seconds so far; callback time is reported in
     info['callback_s'] and excluded from info['solve_s']).
     proposer(level_ops, x, T, cT) -> (T', take) may propose motion before
     each LM call: states in `take` kept a proposal that lowered their energy
     (computeEnergy) and skip this LM call; LMsolver runs on the others, so
     the plain method stays the fallback.  robust: robust per-shot weights
-    at the full-resolution level (info['We'] holds the last ones).
+    at the full-resolution levels (info['We'] holds the last ones).
     bin_motion: the image solves and the energies for the weights use the
     binned motion (disorder_amd.motion.bin_motion at the resolution of the
     finest level); LMsolver keeps the per-shot motion.
@@ -149,52 +151,103 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     while estimating motion) as compressCoils returns them for three
     energy fractions (solveXT.m:401,501): CG calls use the first eivaS[2]
     coils, the closing CG of a level the first eivaS[0], LMsolver (and a
-    proposer) the first eivaS[1]; None uses every coil everywhere."""
+    proposer) the first eivaS[1]; None uses every coil everywhere.
+    n_within: full-resolution levels after the spatial pyramid
+    (round(log2 NWend), pyramidPlan.m:21-22).  At each of them a level that
+    converges closes with the final CG, then robust weights over the
+    15..85 % percentiles flag the shots with weight below one and the next
+    level splits each of their states at its middle profile in acquisition
+    order (solveXT.m:270-287, :470-478); after a split only the new states
+    are estimated on the first outer iteration (every iteration before the
+    last level, :383-385).  Needs traj, the dict of
+    disorder_amd.trajectory.trajectory_states for the scan (its state map
+    must give k1, k2, off); the returned T then has one row per final state
+    and info['sweep_of_state'] maps states to shots."""
     import torch
+    from . import trajectory as TJ
+    from .sampling import harmonic_sampling
     ctx = ctx or ops.default_context(y.device.index or 0)
-    S_states = len(off) - 1
-    T = np.zeros((S_states, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
+    nsweeps = len(off) - 1
+    T = np.zeros((nsweeps, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
     x = x0
     info = {"levels": [], "callback_s": 0.0, "lm_trials": 0, "proposals_accepted": 0}
     t0 = time.perf_counter()
     cb_time = 0.0
     res_fine = pyramid_levels(N, nlevels)[-1][1]
-
     C = int(S.shape[3])
     ev = [C, C, C] if eivaS is None else [int(v) for v in eivaS]
     if len(ev) != 3:
         raise ValueError("eivaS holds three coil counts")
+    if n_within and traj is None:
+        raise ValueError("within-shot levels need the trajectory (traj)")
+    # state bookkeeping for within-shot subdivision (solveXT.m:270-287)
+    sweep_of_state = np.arange(nsweeps)
+    k1c, k2c, offc, rows0 = k1, k2, np.asarray(off), None
+    if traj is not None:
+        kk1, kk2, koff = harmonic_sampling(traj["H"], traj["sweeps"])
+        if not (np.array_equal(kk1, k1) and np.array_equal(kk2, k2) and np.array_equal(koff, off)):
+            raise ValueError("traj does not describe the sampling k1, k2, off")
+        state_of_profile = np.array(traj["sweep_of_profile"], np.int64)
+        pos_row = np.full(traj["H"].shape, -1, np.int64)
+        pos_row[k1, k2] = np.arange(len(k1))
+    subT = np.zeros(nsweeps, bool)
+    subTT = np.zeros(nsweeps, bool)
 
     def cg_ops(L, T, final):
         return L.cg(ev[0] if final else ev[2], T, *((res_fine, traLimX) if bin_motion else ()))
-    for li, (n, resIso) in enumerate(pyramid_levels(N, nlevels)):
-        L = Level(y, S, k1, k2, off, N, n, ctx, lm_coils=ev[1])
+
+    levels = pyramid_levels(N, nlevels) + [(list(N), res_fine)] * int(n_within)
+    last = len(levels) - 1  # L of :383, lowered to a level that flags no shot (:474)
+    for li, (n, resIso) in enumerate(levels):
+        leff = li - (nlevels - 1)
+        if subT.any():  # split the flagged states (:270-287)
+            T = np.repeat(T, np.where(subT, 2, 1), axis=0)
+            subTT = np.repeat(subT, np.where(subT, 2, 1))
+            sweep_of_state = np.repeat(sweep_of_state, np.where(subT, 2, 1))
+            state_of_profile, _ = TJ.subdivide_states(state_of_profile, subT)
+            Ti = traj["time_index"]
+            H = np.where(Ti > 0, state_of_profile[np.maximum(Ti, 1) - 1], 0)
+            k1c, k2c, offc = harmonic_sampling(H, len(sweep_of_state))
+            if (np.diff(offc) == 0).any():
+                raise ValueError("a split left a state without samples")
+            rows0 = pos_row[k1c, k2c]
+        subT = np.zeros(len(sweep_of_state), bool)
+        nstates = len(sweep_of_state)
+        L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0)
         full_res = list(n) == list(N)
-        st_of_sample = np.repeat(np.arange(S_states), np.diff(L.off))
+        sweep_of_sample = sweep_of_state[np.repeat(np.arange(nstates), np.diff(L.off))]
         if x is None:
             x = ops.empty_colmajor(n, y.device)
             x.zero_()
         elif list(x.shape) != list(n):
             x = ops.resampling(x, n, ctx)
-        w = np.full(S_states, winit)
+        w = np.full(nstates, winit)
         cT0 = np.diff(L.off) < echoesMin
         cT = cT0.copy()
         tL = (traLimXT[0] * resIso, traLimXT[1] * resIso)
         nReset, cont = 0, 0
         its = 0
+
+        def weights(x, final):
+            Ec, Tc, yc, _ = cg_ops(L, T, final)
+            ps = ops.computeEnergy(yc, x, Ec, Tc, per_sample=True)
+            We, _ = RB.shot_weights(ps, sweep_of_sample, nsweeps, finer=leff > 0)
+            L.set_weights(RB.sample_weights(We, sweep_of_sample))
+            info["We"] = We
+            return We
+
         for it in range(1, n_outer + 1):
             if (nReset == 0 and cont == 0) or (nReset > 0 and cont % nReset == 0) or cT.all():
                 nReset += 1
                 cont = 0
                 cT = cT0.copy()
+            if subTT.any() and (it == 1 or li < last):
+                cT = cT | ~subTT
             cont += 1
             Ec, Tc, yc, Se = cg_ops(L, T, False)
             x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0, Se=Se)
             if robust and full_res:
-                ps = ops.computeEnergy(yc, x, Ec, Tc, per_sample=True)
-                We, _ = RB.shot_weights(ps, st_of_sample, S_states, finer=False)
-                L.set_weights(RB.sample_weights(We, st_of_sample))
-                info["We"] = We
+                weights(x, False)
             if proposer is not None:
                 T_before = T.copy()
                 T, take = proposer(L, x, T, cT)
@@ -213,20 +266,30 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                 callback(li, it, x, T, tc - t0 - cb_time)
                 cb_time += time.perf_counter() - tc
             if cT.all():
-                if final_cg and it < n_outer:
-                    Ec, Tc, yc, Se = cg_ops(L, T, True)
-                    x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
-                    if callback is not None:
+                if it < n_outer:  # the estT(l) = 0 iteration of :425-481
+                    if final_cg:
+                        Ec, Tc, yc, Se = cg_ops(L, T, True)
+                        x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
+                    if robust and leff > 0:
+                        We = weights(x, True)
+                        subT = (We < 1.0)[sweep_of_state]
+                        if not subT.any():
+                            last = li
+                    if final_cg and callback is not None:
                         torch.cuda.synchronize()
                         tc = time.perf_counter()
                         callback(li, it + 1, x, T, tc - t0 - cb_time)
                         cb_time += time.perf_counter() - tc
                 break
-        info["levels"].append({"n": n, "outer": its, "states_converged": int(cT.sum())})
+        if li == len(levels) - 1:
+            subT[:] = False
+        info["levels"].append({"n": n, "outer": its, "states_converged": int(cT.sum()), "states": nstates,
+                               "split": int(subT.sum())})
         L.close()
     if list(x.shape) != list(N):
         x = ops.resampling(x, N, ctx)
     torch.cuda.synchronize()
+    info["sweep_of_state"] = sweep_of_state
     info["callback_s"] = cb_time
     info["solve_s"] = time.perf_counter() - t0 - cb_time
     return x, T, info

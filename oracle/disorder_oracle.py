@@ -1029,7 +1029,7 @@ def pyramid_levels(N, nlevels):
 
 def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
                   traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
-                  info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None):
+                  info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None, n_within=0, traj=None):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -1049,26 +1049,74 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     solveXT.m:392) while LMsolver keeps the per-shot motion (:496).  eivaS = (final,
     motion, image-while-estimating) coil counts: CG uses the first eivaS[2]
     coils (its preconditioner too), the closing CG eivaS[0], LMsolver
-    eivaS[1] (:401-407, :501-503).
+    eivaS[1] (:401-407, :501-503).  n_within full-resolution levels follow
+    the spatial ones (pyramidPlan.m:21-22); at them, once a level has
+    converged its estT(l) = 0 iteration (the closing CG, if final_cg) ends
+    with the 15..85 % robust weights, every state of a shot weighted below
+    one is flagged (:470-478), and the next level splits the flagged states
+    at their middle profile (:270-287, traj from trajectory_states) and
+    estimates only the new states on its first iteration (on all of them
+    before the last level, :383-385).  info['sweep_of_state'] receives the
+    shot of every final state.
     Returns (x at full size, T)."""
-    S = len(off) - 1
-    T = np.zeros((S, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
+    from oracle import trajectory_oracle as TO
+    NSw = len(off) - 1
+    T = np.zeros((NSw, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
     x = None
     res_fine = pyramid_levels(N, nlevels)[-1][1]
     ev = [Smaps.shape[3]] * 3 if eivaS is None else list(eivaS)
-    for li, (n, resIso) in enumerate(pyramid_levels(N, nlevels)):
-        rows, l1, l2, loff = _level_kspace_rows(k1, k2, off, N[:2], n[:2])
+    sweep_of_state = list(range(NSw))
+    kc1, kc2, offc = np.asarray(k1), np.asarray(k2), np.asarray(off)
+    ysrc = yc
+    if traj is not None:
+        stateSample = np.array(traj["sweep_of_profile"])
+        row_of = {(int(a), int(b)): r for r, (a, b) in enumerate(zip(k1, k2))}
+    subT = [1] * NSw
+    subTT = [1] * NSw
+    levels = pyramid_levels(N, nlevels) + [(list(N), res_fine)] * int(n_within)
+    Lidx = len(levels) - 1
+    for li, (n, resIso) in enumerate(levels):
+        leff = li - (nlevels - 1)
+        if any(v == 2 for v in subT):
+            T = np.concatenate([np.repeat(T[i:i + 1], subT[i], axis=0) for i in range(len(subT))])
+            subTT = [v for v in subT for _ in range(v)]
+            sweep_of_state = [sw for sw, v in zip(sweep_of_state, subT) for _ in range(v)]
+            stateSample = TO.subdivide_states(stateSample, subT)
+            Ti = traj["time_index"]
+            H = np.zeros_like(Ti)
+            for i in range(Ti.shape[0]):
+                for j in range(Ti.shape[1]):
+                    if Ti[i, j]:
+                        H[i, j] = stateSample[Ti[i, j] - 1]
+            kc1, kc2, offh, _ = build_harmonic_sampling(H, len(sweep_of_state))
+            offc = offh[:-1]
+            kc1, kc2 = kc1[:offc[-1]], kc2[:offc[-1]]
+            ysrc = yc[[row_of[(int(a), int(b))] for a, b in zip(kc1, kc2)]]
+        subT = [1] * len(sweep_of_state)
+        S = len(sweep_of_state)
+        rows, l1, l2, loff = _level_kspace_rows(kc1, kc2, offc, N[:2], n[:2])
         scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
-        yl = resample_fourier(yc[rows] * scale, (len(rows), n[2], yc.shape[2]))
+        yl = resample_fourier(ysrc[rows] * scale, (len(rows), n[2], yc.shape[2]))
         Sl = resample_fourier(Smaps, tuple(n) + (Smaps.shape[3],))
         samp = Sampling(l1, l2, loff, n[0], n[1])
         full_res = list(n) == list(N)
+        sweep_of_sample = np.array(sweep_of_state)[samp.state_of_sample()]
 
         def binned(T, We):
             if not bin_motion:
                 return Sampling(l1, l2, loff, n[0], n[1], We), T
             Tb, offb, _ = compress_motion(T, loff, res_fine, traLimX)
             return Sampling(l1, l2, offb, n[0], n[1], We), Tb
+
+        def weights(c, finer):
+            samp_b, Tb = binned(T, samp.We)
+            ps = compute_energy(yl[..., :c], x, Sl[..., :c], Sampling(l1, l2, samp_b.off, n[0], n[1]), Tb, N,
+                                per_sample=True)
+            Wshot = robust_shot_weights(ps, sweep_of_sample, NSw, finer=finer)
+            samp.We = Wshot[sweep_of_sample]
+            if info is not None:
+                info["We"] = Wshot
+            return Wshot
         Se = 1.0 / (np.sum(np.abs(Sl[..., :ev[2]]) ** 2, axis=-1) + 1e-9)
         Sef = 1.0 / (np.sum(np.abs(Sl[..., :ev[0]]) ** 2, axis=-1) + 1e-9)
         x = np.zeros(n, np.complex128) if x is None else resample_fourier(x, n)
@@ -1083,26 +1131,33 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
                 nReset += 1
                 cont = 0
                 cT = cT0.copy()
+            if (it == 1 or li < Lidx) and any(v == 2 for v in subTT):
+                cT = cT | (np.array(subTT) == 1)
             cont += 1
             samp_b, Tb = binned(T, samp.We)
             c = ev[2]
             x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=n_cg,
                                 tolType="RelativeResidual2Error", tol=0.0, Se=Se)
             if robust and full_res:
-                ps = compute_energy(yl[..., :c], x, Sl[..., :c], Sampling(l1, l2, samp_b.off, n[0], n[1]), Tb, N,
-                                    per_sample=True)
-                Wshot = robust_shot_weights(ps, samp.state_of_sample(), S, finer=False)
-                samp.We = Wshot[samp.state_of_sample()]
-                if info is not None:
-                    info["We"] = Wshot
+                weights(c, leff > 0)
             T, w, cT, _ = lm_solver(yl[..., :ev[1]], x, Sl[..., :ev[1]], samp, T, N, w, cT, winit, tL)
             if callback is not None:
                 callback(li, it, x, T)
             if cT.all():
-                if final_cg and it < n_outer:
-                    samp_b, Tb = binned(T, samp.We)
-                    c = ev[0]
-                    x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=final_cg,
-                                        tolType="Energy", tol=1e-3, Se=Sef)
+                if it < n_outer:
+                    if final_cg:
+                        samp_b, Tb = binned(T, samp.We)
+                        c = ev[0]
+                        x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=final_cg,
+                                            tolType="Energy", tol=1e-3, Se=Sef)
+                    if robust and leff > 0:
+                        Wshot = weights(ev[0], True)
+                        subT = [2 if Wshot[sw] < 1 else 1 for sw in sweep_of_state]
+                        if all(v == 1 for v in subT):
+                            Lidx = li
                 break
+        if li == len(levels) - 1:
+            subT = [1] * len(sweep_of_state)
+    if info is not None:
+        info["sweep_of_state"] = np.array(sweep_of_state)
     return resample_fourier(x, N), T
