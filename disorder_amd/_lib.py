@@ -55,6 +55,15 @@ SIGNATURES = {
     "dis_cg": (c_int, [c_void_p, c_double_p, c_void_p, c_float_p, c_float_p, ctypes.c_double, c_void_p, c_int32,
                        c_int32, c_int32, ctypes.c_double, c_double_p, c_int32_p, c_int32_p]),
     "dis_autotune": (c_int, [c_void_p, c_int32_p, c_int32_p, c_int32, c_int32_p, c_double_p]),
+    "dis_shearlet_create": (c_int, [c_void_p, c_int32_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32_p, c_int32,
+                                    ctypes.POINTER(c_void_p)]),
+    "dis_shearlet_destroy": (c_int, [c_void_p]),
+    "dis_shearlet_weights": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_double, ctypes.c_double]),
+    "dis_shearlet_regularize": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_double, c_void_p]),
+    "dis_shearlet_energy": (c_int, [c_void_p, c_void_p, c_void_p, c_double_p]),
+    "dis_fidelity_trace": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_float_p, c_void_p, c_double_p]),
+    "dis_cg_cs": (c_int, [c_void_p, c_double_p, c_void_p, c_float_p, c_float_p, ctypes.c_double, c_void_p,
+                          ctypes.c_double, c_void_p, c_int32, c_int32, c_int32, ctypes.c_double, c_int32_p]),
     "dis_autotune_clear": (c_int, [c_void_p]),
     "dis_set_coil_schedule": (c_int, [c_void_p, c_int32]),
     "dis_set_pass_shape": (c_int, [c_void_p, c_int32_p, c_int32, c_int32, c_int32, c_int32]),

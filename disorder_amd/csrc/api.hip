@@ -560,6 +560,22 @@ extern "C" int dis_cg(dis_plan* p, const double* T, const void* y, const float* 
   return DIS_OK;
 }
 
+extern "C" int dis_cg_cs(dis_plan* p, const double* T, const void* y, const float* Se, const float* Ma, double lambda,
+                         dis_shearlet* sh, double sh_scale, void* x, int32_t x_init, int32_t nIt, int32_t tolType,
+                         double tol, int32_t* nn) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  if (!x || !y || !sh || nIt < 0 || (p->g.nmotion > 0 && !T)) return fail(c, DIS_E_INVALID, "dis_cg_cs: invalid argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  int it = 0;
+  int st = run_cg(c, plan_view(p), T, (const float2*)y, Se, Ma, lambda, (float2*)x, x_init != 0, nIt, tolType, tol,
+                  nullptr, nullptr, &it, sh, sh_scale);
+  if (st) return st;
+  if (nn) *nn = it;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DIS_OK;
+}
+
 extern "C" int dis_autotune(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], int32_t S, int32_t* choice,
                             double* ms) {
   if (!c) return DIS_E_INVALID;

@@ -19,9 +19,11 @@ namespace dis {
   } while (0)
 
 int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const float* Se, const float* Ma,
-           double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn) {
+           double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn,
+           dis_shearlet* sh, double sh_scale) {
   if (tolType != 1 && tolType != 6 && tolType != 7)
     return ctx_fail(c, DIS_E_UNSUPPORTED, "CG tolType must be RelativeResidual2Error(1), Energy(6) or None(7)");
+  if (sh && tolType == 6) return ctx_fail(c, DIS_E_UNSUPPORTED, "CG with the shearlet term: tolType 'Energy' not supported");
   hipStream_t st = ctx_stream(c);
   const long long n = (long long)v.g.n[0] * v.g.n[1] * v.g.n[2];
   const size_t vb = sizeof(float2) * n;
@@ -66,6 +68,7 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
   if (x_init) {
     // EHE = C(encodedecode(x) + R x) (:55)
     rc = op_apply(c, v, T, x, nullptr, nullptr, q, COIL_EHE, nullptr);
+    if (!rc && sh) rc = shearlet_apply(c, sh, x, sh_scale, q);
     if (rc) return rc;
     CCHK(c, cg_constrain_dot(q, x, Ma, flam, n, part, &sc->dot, st));
     CCHK(c, cg_residual(r, q, r, z, p, Se, n, part, need_b ? part2 : nullptr, &sc->rsold, &sc->bnorm2, st));
@@ -87,6 +90,7 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
   std::vector<double> Ehist;  // [fid, reg] of odd iterations
   for (it = 1; it <= nIt; ++it) {
     rc = op_apply(c, v, T, p, nullptr, nullptr, q, COIL_EHE, nullptr);
+    if (!rc && sh) rc = shearlet_apply(c, sh, p, sh_scale, q);
     if (rc) return rc;
     CCHK(c, cg_constrain_dot(q, p, Ma, flam, n, part, &sc->pq, st));
     CCHK(c, cg_step(sc, st));

@@ -137,7 +137,14 @@ int coil_apply(dis_ctx* c, const float* Bm_dev, int Cc, int C, const float2* in,
 int run_cg_batched(dis_ctx* c, int nvol, const OpView* views, const double* const* T, const float2* const* y,
                    const float* const* Se, float2* const* x, int nIt, bool use_graph, double* res, double* ms);
 
+// Shearlet regulariser (cs.hip): out += scale * R_sh(x), R_sh the IRLS-weighted
+// shearlet term of regularize.m:27-50 with the weights last computed.
+int shearlet_apply(dis_ctx* c, dis_shearlet* sh, const float2* x, double scale, float2* out);
+
+// sh (or null): adds (la_sh p / 2) R_sh to the CG operator (CSsolver.m:76);
+// 'Energy' stopping is not available with it.
 int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const float* Se, const float* Ma,
-           double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn);
+           double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn,
+           dis_shearlet* sh = nullptr, double sh_scale = 0.0);
 
 }  // namespace dis

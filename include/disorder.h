@@ -40,6 +40,7 @@ extern "C" {
 
 typedef struct dis_ctx dis_ctx;
 typedef struct dis_plan dis_plan;
+typedef struct dis_shearlet dis_shearlet;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int dis_version(void);
@@ -264,6 +265,41 @@ int dis_mlp_backward(dis_ctx* ctx, const float* params, const float* X, int32_t 
 int dis_mlp_train_step(dis_ctx* ctx, float* params, float* adam_m, float* adam_v, int32_t step, float lr,
                        const float* X, const float* target, int32_t M, const int32_t* scan_off, int32_t nscan,
                        float* loss);
+
+/* ---- Shearlet-regularised final reconstruction (Reconstruction/CSsolver.m) ----
+ * The 3-D shearlet system of Shearlets/buildShearlet.m (CSsolver.m:52-54:
+ * one scale, K = 1, 'kos' directional and 'maxflat2' scaling filters) is
+ * passed as its 2-D plane filters (combineShearlets, buildShearlet.m:94-106):
+ * p12 [n1 x n2 x nf], p13 [n1 x n3 x nf], p32 [n3 x n2 x nf] complex64 host
+ * arrays in FFT order (DC first), and bands [nw x 4] = (plane a, filter a,
+ * plane b, filter b) with planes 0 = (k1,k2), 1 = (k1,k3), 2 = (k3,k2):
+ * band w is S_w(k1,k2,k3) = plane_a[filter a] * plane_b[filter b].  The
+ * handle owns the IRLS weights (nw float volumes) and three work volumes. */
+int dis_shearlet_create(dis_ctx* ctx, const int32_t n[3], const void* p12, const void* p13, const void* p32,
+                        int32_t nf, const int32_t* bands, int32_t nw, dis_shearlet** out);
+int dis_shearlet_destroy(dis_shearlet* sh);
+/* R.Sh.We = (|IFFT3(conj(S_w) FFT3(x))| + epsil)^(p - 2) for every band
+ * (CSsolver.m:55-59 computeShearlet, :84-96). */
+int dis_shearlet_weights(dis_ctx* ctx, dis_shearlet* sh, const void* x, double epsil, double p);
+/* out += scale * IFFT3(sum_w S_w FFT3(We_w IFFT3(conj(S_w) FFT3(x)))): the
+ * shearlet term y{5} of Reconstruction/regularize.m:27-50 with scale =
+ * R.Sh.la * R.Sh.p / 2.  x, out: DEVICE [n1 x n2 x n3] complex64. */
+int dis_shearlet_regularize(dis_ctx* ctx, dis_shearlet* sh, const void* x, double scale, void* out);
+/* sum_w || sqrt(We_w) IFFT3(conj(S_w) FFT3(x)) ||^2 (one probe of the
+ * regulariser trace, Reconstruction/traceEstimation.m:82-97); host result. */
+int dis_shearlet_energy(dis_ctx* ctx, dis_shearlet* sh, const void* x, double* out);
+/* sum_c sum_k Wgrid(k1,k2) |F2(S_c x)(k) / sqrt(n1 n2)|^2 (one probe of the
+ * fidelity trace, traceEstimation.m:61-79 with EH.We laid on the k-space
+ * grid, DC first): S DEVICE [n1 x n2 x n3 x C], Wgrid DEVICE float
+ * [n1 x n2], x DEVICE complex64 volume; uses sh for its work buffers. */
+int dis_fidelity_trace(dis_ctx* ctx, dis_shearlet* sh, const void* S, int32_t C, const float* Wgrid, const void* x,
+                       double* out);
+/* Reconstruction/CGsolver.m:1 with the shearlet term (CSsolver.m:76):
+ * dis_cg with R x = lambda x + (la_sh p / 2) R_sh x.  tolType 'Energy' is
+ * not supported with the shearlet term. */
+int dis_cg_cs(dis_plan* plan, const double* T, const void* y, const float* Se, const float* Ma, double lambda,
+              dis_shearlet* sh, double sh_scale, void* x, int32_t x_init, int32_t nIt, int32_t tolType, double tol,
+              int32_t* nn);
 
 /* Kernel classes for dis_profile_read. */
 #define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */

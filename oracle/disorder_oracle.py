@@ -566,9 +566,11 @@ def compute_energy(y, x, Smaps, samp, T, Nfull, nmotion=None, per_sample=False, 
 
 
 def cg_solver(y, Smaps, samp, T, Nfull, x0=None, nIt=300, tolType="Energy", tol=1e-5,
-              Se=None, Ma=None, lam=None, nmotion=None):
+              Se=None, Ma=None, lam=None, nmotion=None, reg=None):
     """Reconstruction/CGsolver.m:45-109 with A.Se (precondition.m:19-23),
-    C.Ma (constrain.m:12) and an optional Tikhonov term R.Ti.la (regularize.m:24).
+    C.Ma (constrain.m:12), an optional Tikhonov term R.Ti.la (regularize.m:24)
+    and an optional further regulariser reg(v) added to R x (the shearlet term
+    of regularize.m:27-50 for CSsolver).
 
     Returns (x, En, nn) where En holds [fidelity; regulariser] energies of the
     odd iterations for tolType 'Energy' (computeEnergy.m:72-80)."""
@@ -579,7 +581,8 @@ def cg_solver(y, Smaps, samp, T, Nfull, x0=None, nIt=300, tolType="Energy", tol=
         return v if Se is None else v * Se
 
     def R_(v):
-        return 0 if lam is None else lam * v
+        r = 0 if lam is None else lam * v
+        return r if reg is None else r + reg(v)
 
     def EHE(v):
         return C_(encodedecode(v, Smaps, samp, T, Nfull, nmotion) + R_(v))
