@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched"], default="ehe",
+    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched", "cs"], default="ehe",
                     help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid; "
                          "compress: 64 -> 8 coil compression of maps and data; batched: six uneven volumes in one "
                          "batched CG with hipGraph-captured iterations")
@@ -317,6 +317,91 @@ def bench_batched(args):
     return line
 
 
+def bench_cs(args):
+    """Shearlet regulariser of the CS final reconstruction at 256x192x256
+    (CSsolver.m / regularize.m:27-50): 50 bands, each IFFT3 -> IRLS weight ->
+    FFT3 with the band evaluated from its two plane filters inside the
+    passes; 306 FFT-axis passes per application.  Also the IRLS weights and
+    one regulariser trace probe."""
+    import numpy as np
+    import torch
+    from disorder_amd import shearlet as SH
+    from disorder_amd import synth
+
+    dev = torch.device("cuda", 0)
+    n = (256, 192, 256)
+    x = synth.phantom(n, 12, seed=8, device=dev)
+    t0 = time.perf_counter()
+    sysh = SH.ShearletSystem(n)
+    build_s = time.perf_counter() - t0
+    ctx = sysh.ctx
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sysh.weights(x)
+    torch.cuda.synchronize()
+    weights_s = time.perf_counter() - t0
+    out = torch.zeros_like(x)
+    for _ in range(args.warmup):
+        sysh.regularize(x, 0.5, out)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sysh.regularize(x, 0.5, out)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    nl, kms, kb = prof["shearlet_pass"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e = sysh.energy(x)
+    torch.cuda.synchronize()
+    energy_s = time.perf_counter() - t0
+    sysh.close()
+    ms = wall / args.steps * 1e3
+    line = {"metric": "shearlet regulariser applications/s at 256x192x256 (50 bands)", "value": 1e3 / ms,
+            "unit": "apps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "complex64",
+            "data": "synthetic (seeded phantom)",
+            "config": {"workload": "CSsolver shearlet term: IFFT3(sum_w S_w FFT3(We_w IFFT3(conj(S_w) FFT3 x))) at "
+                                   "256x192x256, buildShearlet(N,1,.,1,'kos'), 50 bands"},
+            "roofline": {"bound": "hbm", "kernel": "shearlet_pass", "achieved": kb / (kms * 1e6), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": kb / (kms * 1e6) / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes_per_launch": kb / nl, "avg_launch_ms": kms / nl,
+                         "launches_per_step": nl / args.steps},
+            "weights_s": weights_s, "energy_probe_s": energy_s, "host_plane_build_s": build_s,
+            "energy_check": e}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = _cpu_baseline_cs(n)
+    return line
+
+
+def _cpu_baseline_cs(n):
+    """FP64 oracle (scipy.fft, 16 threads) on one of the 50 bands at the full
+    grid: t_band; apps/s = 1 / (50 t_band + t_fft3 pair)."""
+    import numpy as np
+    import scipy.fft as sf
+    from oracle import shearlet_oracle as SO
+    n1, n2, n3 = n
+    A = SO.shearlets_2d(n1, n2)[:, :, 2]
+    B = np.transpose(SO.shearlets_2d(n3, n2), (1, 0, 2))[:, :, 2]
+    S = A[:, :, None] * B[None, :, :]
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    We = rng.uniform(0.5, 2.0, n)
+    t0 = time.perf_counter()
+    xF = sf.fftn(x, workers=16)
+    t1 = time.perf_counter()
+    t = sf.ifftn(np.conj(S) * xF, workers=16) * We
+    acc = sf.fftn(t, workers=16) * S
+    t2 = time.perf_counter()
+    tb, tf = t2 - t1, t1 - t0
+    return {"value": 1.0 / (50 * tb + 2 * tf), "unit": "apps/s", "cores": 16, "kind": "port",
+            "sample": f"FP64 oracle band body (IFFT3, weight, FFT3, band multiply) of 1 of 50 bands at "
+                      f"256x192x256: {tb:.2f} s, FFT3 {tf:.2f} s; apps/s = 1/(50 band + 2 FFT3)"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -329,6 +414,9 @@ def main():
     if args.workload == "compress":
         print(json.dumps(bench_compress(args)), flush=True)
         return
+    if args.workload == "cs":
+        print(json.dumps(bench_cs(args)), flush=True)
+        return 0
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)), flush=True)
         return

@@ -94,9 +94,9 @@ SIGNATURES = {
     "dis_profile_read": (c_int, [c_void_p, c_int32, ctypes.POINTER(ctypes.c_int64), c_double_p, c_double_p]),
 }
 
-K_PASS, K_BCAST, K_ACCUM, K_COIL, K_FLIP, K_GRAM, K_GEMM = range(7)
+K_PASS, K_BCAST, K_ACCUM, K_COIL, K_FLIP, K_GRAM, K_GEMM, K_SHEAR = range(8)
 K_NAMES = {K_PASS: "pass", K_BCAST: "bcast_pass", K_ACCUM: "accum_pass", K_COIL: "coil_stage", K_FLIP: "flip_pass",
-           K_GRAM: "coil_gram", K_GEMM: "coil_apply"}
+           K_GRAM: "coil_gram", K_GEMM: "coil_apply", K_SHEAR: "shearlet_pass"}
 
 _lib = None
 

@@ -14,8 +14,12 @@
 // IRLS weights (|v| + eps)^(p-2), weighted energies, += scale v).  Tiles of P
 // pencils are read whole into LDS before they are written, so the passes run
 // in place.
+// rsh_kernel: the same passes on the register path for lengths 16 R1
+// (two-level FFT, sixteen lanes per pencil, one LDS exchange per transform),
+// plus the fused middle pass of a band along the last axis: inverse FFT,
+// times the weights, forward FFT, without a round trip through HBM.
 //   regulariser  (la p/2) IFFT3( sum_w S_w FFT3( We_w IFFT3( conj(S_w) FFT3 x )))
-//   = 3 + 6 NW + 3 passes; weights and energies 3 + 3 NW passes.
+//   = 3 + 5 NW + 3 passes; weights and energies 3 + 3 NW passes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,6 +28,7 @@
 
 #include "../../include/disorder.h"
 #include "kernels.hpp"
+#include "regfft.hpp"
 #include "transform.hpp"
 
 struct dis_shearlet {
@@ -49,6 +54,7 @@ typedef dis_shearlet Shearlet;
 
 enum LoadOp { LD_PLAIN = 0, LD_CONJ_BAND = 1, LD_MUL_W = 2, LD_MUL_COIL = 3 };
 enum StoreOp { ST_PLAIN = 0, ST_ACC_BAND = 1, ST_WEIGHTS = 2, ST_ENERGY = 3, ST_ACC_SCALED = 4, ST_ENERGY_GRID = 5 };
+enum PassKind { PK_FWD = 0, PK_INV = 1, PK_MID = 2 };  // MID: inverse, x W, forward
 
 namespace {
 
@@ -85,6 +91,102 @@ __device__ __forceinline__ float2 band_value(const AxisArgs& a, long long v) {
   const int k2 = (int)(t % a.n2);
   const int k3 = (int)(t / a.n2);
   return cmul(plane_value(a, a.band.x, a.band.y, k1, k2, k3), plane_value(a, a.band.z, a.band.w, k1, k2, k3));
+}
+
+template <int LOP>
+__device__ __forceinline__ float2 load_op(const AxisArgs& a, long long g) {
+  float2 v = a.in[g];
+  if (LOP == LD_CONJ_BAND) v = cmulc(v, band_value(a, g));
+  if (LOP == LD_MUL_W) v = cscale(v, a.W[g]);
+  if (LOP == LD_MUL_COIL) v = cmul(v, a.coil[g]);
+  return v;
+}
+
+template <int SOP>
+__device__ __forceinline__ void store_op(const AxisArgs& a, long long g, float2 v, double& acc) {
+  if (SOP == ST_PLAIN) a.out[g] = v;
+  if (SOP == ST_ACC_BAND) a.out[g] = cadd(a.out[g], cmul(band_value(a, g), v));
+  if (SOP == ST_ACC_SCALED) a.out[g] = cadd(a.out[g], v);
+  if (SOP == ST_WEIGHTS) a.Wout[g] = powf(sqrtf(v.x * v.x + v.y * v.y) + a.eps, a.pexp);
+  if (SOP == ST_ENERGY) acc += (double)a.W[g] * ((double)v.x * v.x + (double)v.y * v.y);
+  if (SOP == ST_ENERGY_GRID) acc += (double)a.W[g % ((long long)a.n1 * a.n2)] * ((double)v.x * v.x + (double)v.y * v.y);
+}
+
+template <int SOP>
+__device__ __forceinline__ void block_energy(const AxisArgs& a, double acc, double* red, int tid) {
+  if (SOP == ST_ENERGY || SOP == ST_ENERGY_GRID) {
+    red[tid] = acc;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    if (tid == 0) a.part[blockIdx.x] = red[0];
+  }
+}
+
+// Register path, N = 16 R1.  CONTIG (axis 0): pencil q = 16 blockIdx + p,
+// lanes 16 p + t.  Strided: sixteen consecutive positions lo of the
+// contiguous index space per tile, lanes p + 16 t.
+template <int R1, bool CONTIG, int KIND, int LOP, int SOP>
+__global__ __launch_bounds__(kThreads) void rsh_kernel(AxisArgs a) {
+  constexpr int N = R1 * 16;
+  constexpr int XB = CONTIG ? RegFftSize<R1>::XB : RegFftSize<R1>::XB_STRIDED;
+  __shared__ fvec2 xraw[16 * XB];
+  __shared__ fvec2 traw[N];
+  __shared__ double red[kThreads];
+  const int tid = threadIdx.x;
+  const int p = CONTIG ? (tid >> 4) : (tid & 15);
+  const int t = CONTIG ? (tid & 15) : (tid >> 4);
+  lds_float2* tw = (lds_float2*)traw;
+  lds_float2* xb = (lds_float2*)xraw + p * XB;
+  for (int k = tid; k < N; k += kThreads) st2(tw + k, a.tw[k]);
+  __syncthreads();
+  long long hi0, lo0;
+  int Pv;
+  if (CONTIG) {
+    hi0 = (long long)blockIdx.x * 16;
+    lo0 = 0;
+    Pv = (int)(a.H - hi0 < 16 ? a.H - hi0 : 16);
+  } else {
+    hi0 = blockIdx.x / a.tiles_lo;
+    lo0 = (blockIdx.x - hi0 * a.tiles_lo) * 16LL;
+    Pv = (int)(a.L - lo0 < 16 ? a.L - lo0 : 16);
+  }
+  const bool live = p < Pv;
+  auto addr = [&](int j) -> long long {
+    return CONTIG ? (long long)j + (long long)N * (hi0 + p) : (lo0 + p) + a.L * ((long long)j + (long long)N * hi0);
+  };
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < R1; ++r) {
+    float2 x = live ? load_op<LOP>(a, addr(t + 16 * r)) : make_float2(0.f, 0.f);
+    v[r] = KIND == PK_FWD ? x : cswap(x);
+  }
+  reg_fft<R1, 16>(v, t, xb, tw);
+  double acc = 0.0;
+  if (KIND == PK_MID) {
+    const float inv = 1.0f / (float)N;
+    if (t < R1) {
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) {
+        const float w = live ? a.W[addr(t + R1 * k2)] : 0.f;
+        v[k2] = cscale(cswap(v[k2]), inv * w);
+      }
+    }
+    reg_fft<16, R1>(v, t, xb, tw);
+    if (live) {
+#pragma unroll
+      for (int r = 0; r < R1; ++r) store_op<SOP>(a, addr(t + 16 * r), cscale(v[r], a.scale), acc);
+    }
+  } else if (t < R1 && live) {
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) {
+      float2 y = KIND == PK_INV ? cswap(v[k2]) : v[k2];
+      store_op<SOP>(a, addr(t + R1 * k2), cscale(y, a.scale), acc);
+    }
+  }
+  block_energy<SOP>(a, acc, red, tid);
 }
 
 template <bool CONTIG, bool INV, int LOP, int SOP>
@@ -176,6 +278,7 @@ struct Pass {
   float2* out;
   int axis;
   bool inv;
+  bool mid;  // inverse, x W, forward in one register-path pass
   int lop, sop;
   float scale;
   int4 band;
@@ -184,6 +287,84 @@ struct Pass {
   const float2* coil;
   float eps, pexp;
 };
+
+double pass_bytes(const Pass& ps) {
+  // algorithmic HBM bytes per voxel: the input, the load operand, the weights of a fused pass, the store
+  double bpv = 8.0 + (ps.lop == LD_MUL_W ? 4.0 : ps.lop == LD_MUL_COIL ? 8.0 : 0.0) + (ps.mid ? 4.0 : 0.0);
+  bpv += ps.sop == ST_PLAIN ? 8.0 : (ps.sop == ST_ACC_BAND || ps.sop == ST_ACC_SCALED) ? 16.0
+         : ps.sop == ST_WEIGHTS ? 4.0 : ps.sop == ST_ENERGY ? 4.0 : 0.0;
+  return bpv;
+}
+
+template <int R1, bool CONTIG>
+int launch_rsh(const Pass& ps, AxisArgs& a, long long tiles, hipStream_t s) {
+  const int kind = ps.mid ? PK_MID : ps.inv ? PK_INV : PK_FWD;
+  const int key = kind | (ps.lop << 2) | (ps.sop << 4);
+#define DIS_RK(K, LO, SO) \
+  case (K | (LO << 2) | (SO << 4)): \
+    hipLaunchKernelGGL((rsh_kernel<R1, CONTIG, K, LO, SO>), dim3((unsigned)tiles), dim3(kThreads), 0, s, a); \
+    return DIS_OK;
+  if (CONTIG) {
+    switch (key) {
+      DIS_RK(PK_INV, LD_CONJ_BAND, ST_PLAIN)
+      DIS_RK(PK_FWD, LD_PLAIN, ST_ACC_BAND)
+      DIS_RK(PK_FWD, LD_PLAIN, ST_PLAIN)
+      DIS_RK(PK_INV, LD_PLAIN, ST_PLAIN)
+      DIS_RK(PK_FWD, LD_MUL_COIL, ST_PLAIN)
+      DIS_RK(PK_FWD, LD_MUL_W, ST_PLAIN)
+      default: return DIS_E_UNSUPPORTED;
+    }
+  } else {
+    switch (key) {
+      DIS_RK(PK_INV, LD_PLAIN, ST_PLAIN)
+      DIS_RK(PK_FWD, LD_PLAIN, ST_PLAIN)
+      DIS_RK(PK_FWD, LD_PLAIN, ST_ENERGY_GRID)
+      DIS_RK(PK_MID, LD_PLAIN, ST_PLAIN)
+      DIS_RK(PK_INV, LD_PLAIN, ST_WEIGHTS)
+      DIS_RK(PK_INV, LD_PLAIN, ST_ENERGY)
+      DIS_RK(PK_INV, LD_PLAIN, ST_ACC_SCALED)
+      DIS_RK(PK_FWD, LD_PLAIN, ST_ACC_BAND)
+      default: return DIS_E_UNSUPPORTED;
+    }
+  }
+#undef DIS_RK
+}
+
+int run_rsh(dis_ctx* c, Shearlet* sh, const Pass& ps, AxisArgs& a, int R1, bool contig, long long* ntiles_out) {
+  long long tiles;
+  if (contig) {
+    tiles = (a.H + 15) / 16;
+    a.tiles_lo = 1;
+  } else {
+    a.tiles_lo = (a.L + 15) / 16;
+    tiles = a.tiles_lo * a.H;
+  }
+  a.P = 16;
+  if (tiles > sh->npart) return ctx_fail(c, DIS_E_UNSUPPORTED, "shearlet: partial buffer too small");
+  if (ntiles_out) *ntiles_out = tiles;
+  hipStream_t s = ctx_stream(c);
+  const int prof = prof_begin(c, DIS_K_SHEAR);
+  int st = DIS_E_UNSUPPORTED;
+#define DIS_RR(R)                                                                     \
+  case R:                                                                             \
+    st = contig ? launch_rsh<R, true>(ps, a, tiles, s) : launch_rsh<R, false>(ps, a, tiles, s); \
+    break;
+  switch (R1) {
+    DIS_RR(2)
+    DIS_RR(4)
+    DIS_RR(8)
+    DIS_RR(10)
+    DIS_RR(12)
+    DIS_RR(14)
+    DIS_RR(16)
+    default: break;
+  }
+#undef DIS_RR
+  prof_end(c, prof, pass_bytes(ps) * (double)sh->nvox);
+  if (st) return ctx_fail(c, DIS_E_UNSUPPORTED, "shearlet: unsupported register pass combination");
+  if (hipGetLastError() != hipSuccess) return ctx_fail(c, DIS_E_HIP, "shearlet register pass launch failed");
+  return DIS_OK;
+}
 
 #define DIS_AX(C, I, LO, SO) hipLaunchKernelGGL((fft_axis_kernel<C, I, LO, SO>), dim3((unsigned)tiles), dim3(kThreads), 0, s, a)
 
@@ -213,8 +394,11 @@ int run_pass(dis_ctx* c, Shearlet* sh, const Pass& ps, long long* ntiles_out) {
   a.eps = ps.eps;
   a.pexp = ps.pexp;
   a.part = sh->part;
-  int P = std::max(1, kMaxTile / a.n);
   const bool contig = ax == 0;
+  const int R1 = reg_radix(a.n);
+  if (R1) return run_rsh(c, sh, ps, a, R1, contig, ntiles_out);
+  if (ps.mid) return ctx_fail(c, DIS_E_UNSUPPORTED, "shearlet: fused pass needs a register-path length");
+  int P = std::max(1, kMaxTile / a.n);
   long long tiles;
   if (contig) {
     P = (int)std::min<long long>(P, std::max<long long>(1, a.H));
@@ -230,6 +414,8 @@ int run_pass(dis_ctx* c, Shearlet* sh, const Pass& ps, long long* ntiles_out) {
   if (ntiles_out) *ntiles_out = tiles;
   hipStream_t s = ctx_stream(c);
   const int key = (ps.inv ? 1 : 0) | (ps.lop << 1) | (ps.sop << 3);
+  const double bpv = pass_bytes(ps);
+  const int prof = prof_begin(c, DIS_K_SHEAR);
   // the (direction, load, store) combinations the solver uses
 #define DIS_AXC(I, LO, SO) \
   do {                     \
@@ -252,6 +438,7 @@ int run_pass(dis_ctx* c, Shearlet* sh, const Pass& ps, long long* ntiles_out) {
     default: return ctx_fail(c, DIS_E_UNSUPPORTED, "shearlet: unsupported pass combination");
   }
 #undef DIS_AXC
+  prof_end(c, prof, bpv * (double)sh->nvox);
   if (hipGetLastError() != hipSuccess) return ctx_fail(c, DIS_E_HIP, "shearlet pass launch failed");
   return DIS_OK;
 }
@@ -323,13 +510,31 @@ int shearlet_apply(dis_ctx* c, Shearlet* sh, const float2* x, double scale, floa
   if (st) return st;
   hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, s, sh->Yf, sh->nvox);
   for (int w = 0; w < sh->nw && !st; ++w) {
+    const float* Ww = sh->We + (long long)w * sh->nvox;
+    if (reg_radix(sh->n[2])) {
+      // IFFT axis 0 (x conj(S_w)), IFFT axis 1, [IFFT axis 2, x We_w, FFT axis 2], FFT axis 1
+      Pass p0 = mk(sh->Xf, sh->t, 0, true, LD_CONJ_BAND, ST_PLAIN, 1.f / sh->n[0]);
+      p0.band = sh->bands_h[w];
+      st = run_pass(c, sh, p0, nullptr);
+      if (!st) st = run_pass(c, sh, mk(sh->t, sh->t, 1, true, LD_PLAIN, ST_PLAIN, 1.f / sh->n[1]), nullptr);
+      Pass pm = mk(sh->t, sh->t, 2, true, LD_PLAIN, ST_PLAIN, 1.f);
+      pm.mid = true;
+      pm.W = Ww;
+      if (!st) st = run_pass(c, sh, pm, nullptr);
+      if (!st) st = run_pass(c, sh, mk(sh->t, sh->t, 1, false), nullptr);
+      if (st) return st;
+      Pass f0 = mk(sh->t, sh->Yf, 0, false, LD_PLAIN, ST_ACC_BAND, 1.f);
+      f0.band = sh->bands_h[w];
+      st = run_pass(c, sh, f0, nullptr);
+      continue;
+    }
     Pass last{};
     last.out = sh->t;
     last.sop = ST_PLAIN;
     st = band_coefficients(c, sh, w, last, nullptr);
     if (st) return st;
     Pass f0 = mk(sh->t, sh->t, 0, false, LD_MUL_W, ST_PLAIN, 1.f);
-    f0.W = sh->We + (long long)w * sh->nvox;
+    f0.W = Ww;
     st = run_pass(c, sh, f0, nullptr);
     if (!st) st = run_pass(c, sh, mk(sh->t, sh->t, 1, false), nullptr);
     if (st) return st;
@@ -427,10 +632,11 @@ int dis_shearlet_create(dis_ctx* c, const int32_t n[3], const void* p12, const v
   for (int ax = 0; ax < 3; ++ax) {
     const long long L = ax == 0 ? 1 : ax == 1 ? n[0] : (long long)n[0] * n[1];
     const long long H = ax == 0 ? (long long)n[1] * n[2] : ax == 1 ? n[2] : 1;
-    const int P0 = std::max(1, kMaxTile / n[ax]);
-    long long t = ax == 0 ? (H + std::min<long long>(P0, H) - 1) / std::min<long long>(P0, H)
-                          : ((L + std::min<long long>(P0, L) - 1) / std::min<long long>(P0, L)) * H;
-    maxt = std::max(maxt, t);
+    for (int P0 : {std::max(1, kMaxTile / n[ax]), 16}) {  // LDS-engine and register-path tilings
+      long long t = ax == 0 ? (H + std::min<long long>(P0, H) - 1) / std::min<long long>(P0, H)
+                            : ((L + std::min<long long>(P0, L) - 1) / std::min<long long>(P0, L)) * H;
+      maxt = std::max(maxt, t);
+    }
   }
   sh->npart = maxt;
   const size_t ps[3] = {(size_t)n[0] * n[1] * nf, (size_t)n[0] * n[2] * nf, (size_t)n[2] * n[1] * nf};

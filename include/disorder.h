@@ -309,7 +309,8 @@ int dis_cg_cs(dis_plan* plan, const double* T, const void* y, const float* Se, c
 #define DIS_K_FLIP 4   /* >90 degree flip pass */
 #define DIS_K_GRAM 5   /* coil-compression Gram matrix */
 #define DIS_K_GEMM 6   /* coil-compression apply (MFMA) */
-#define DIS_K_COUNT 7
+#define DIS_K_SHEAR 7  /* shearlet FFT-axis pass (CS reconstruction) */
+#define DIS_K_COUNT 8
 
 /* Per-kernel-class timing with HIP events recorded around every launch on the
  * context's stream (no counterpart in the reference, which only prints

@@ -109,17 +109,21 @@ def _to_dev(a, dev):
 
 
 @pytest.mark.gpu
-def test_gpu_weights_regulariser_and_traces_match_oracle(dev):
+@pytest.mark.parametrize("grid", [N3, (64, 40, 64), (64, 64, 32 + 32)])
+def test_gpu_weights_regulariser_and_traces_match_oracle(dev, grid):
+    """(40,36,35): every axis on the LDS engine, regulariser without the fused
+    middle pass; (64,40,64): axes 0 and 2 on the register path, axis 1 on the
+    LDS engine, fused middle pass; (64,64,64): register path throughout."""
     import torch
     rng = np.random.default_rng(3)
-    x = crandn(rng, N3)
+    x = crandn(rng, grid)
     x32 = x.astype(np.complex64).astype(np.complex128)
-    sys_ = SH.ShearletSystem(N3)
-    S = SO.shearlets_3d(N3)
+    sys_ = SH.ShearletSystem(grid)
+    S = SO.shearlets_3d(grid)
     sys_.weights(_to_dev(x, dev))
     We_ref = SO.shearlet_weights(x32, S)
-    # the weights come back through the regulariser with unit scale on a delta-free probe
-    z = crandn(rng, N3)
+    # the weights come back through the regulariser
+    z = crandn(rng, grid)
     z32 = z.astype(np.complex64).astype(np.complex128)
     Rg = sys_.regularize(_to_dev(z, dev), 0.7).cpu().numpy()
     Rr = SO.regularize_shearlet(z32, S, We_ref, 1.4, 1.0)  # (la p / 2) with la = 1.4, p = 1
@@ -128,8 +132,8 @@ def test_gpu_weights_regulariser_and_traces_match_oracle(dev):
     zr = _to_dev(z.real.astype(np.complex64), dev)
     e_g = sys_.energy(zr)
     assert abs(e_g / e_r - 1) < 2e-5, (e_g, e_r)
-    Sm = crandn(rng, N3 + (3,))
-    W = rng.uniform(0.2, 1.0, N3[:2])
+    Sm = crandn(rng, grid + (3,))
+    W = rng.uniform(0.2, 1.0, grid[:2])
     Wd = torch.from_numpy(W.astype(np.float32).reshape(-1, order="F")).to(dev)
     f_g = sys_.fidelity(_to_dev(Sm, dev), Wd, zr)
     f_r = SO.trace_samples_fidelity([z32.real], Sm.astype(np.complex64).astype(np.complex128), W)[0]
