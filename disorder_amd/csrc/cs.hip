@@ -77,6 +77,7 @@ struct AxisArgs {
   float eps, pexp;
   const float2* coil;
   double* part;
+  int axis;
 };
 
 __device__ __forceinline__ float2 plane_value(const AxisArgs& a, int p, int i, int k1, int k2, int k3) {
@@ -157,10 +158,60 @@ __global__ __launch_bounds__(kThreads) void rsh_kernel(AxisArgs a) {
   auto addr = [&](int j) -> long long {
     return CONTIG ? (long long)j + (long long)N * (hi0 + p) : (lo0 + p) + a.L * ((long long)j + (long long)N * hi0);
   };
+  // this lane's fixed coordinates; the pencil index j supplies the third
+  int ca = 0, cb = 0;
+  if (a.axis == 0) {
+    const int q = (int)(hi0 + p);
+    ca = q % a.n2;   // k2
+    cb = q / a.n2;   // k3
+  } else if (a.axis == 1) {
+    ca = (int)(lo0 + p);  // k1
+    cb = (int)hi0;        // k3
+  } else {
+    const int lo = (int)(lo0 + p);
+    ca = lo % a.n1;  // k1
+    cb = lo / a.n1;  // k2
+  }
+  auto kc = [&](int j, int& k1, int& k2, int& k3) {
+    if (a.axis == 0) { k1 = j; k2 = ca; k3 = cb; }
+    else if (a.axis == 1) { k1 = ca; k2 = j; k3 = cb; }
+    else { k1 = ca; k2 = cb; k3 = j; }
+  };
+  auto pv = [&](int pl, int i, int k1, int k2, int k3) -> float2 {
+    if (pl == 0) return a.pl0[k1 + a.n1 * (k2 + a.n2 * i)];
+    if (pl == 1) return a.pl1[k1 + a.n1 * (k3 + a.n3 * i)];
+    return a.pl2[k3 + a.n3 * (k2 + a.n2 * i)];
+  };
+  auto bandv = [&](int j) -> float2 {
+    int k1, k2, k3;
+    kc(j, k1, k2, k3);
+    return cmul(pv(a.band.x, a.band.y, k1, k2, k3), pv(a.band.z, a.band.w, k1, k2, k3));
+  };
+  auto ld = [&](int j) -> float2 {
+    const long long g = addr(j);
+    float2 x = a.in[g];
+    if (LOP == LD_CONJ_BAND) x = cmulc(x, bandv(j));
+    if (LOP == LD_MUL_W) x = cscale(x, a.W[g]);
+    if (LOP == LD_MUL_COIL) x = cmul(x, a.coil[g]);
+    return x;
+  };
+  auto st = [&](int j, float2 y, double& acc) {
+    const long long g = addr(j);
+    if (SOP == ST_PLAIN) a.out[g] = y;
+    if (SOP == ST_ACC_BAND) a.out[g] = cadd(a.out[g], cmul(bandv(j), y));
+    if (SOP == ST_ACC_SCALED) a.out[g] = cadd(a.out[g], y);
+    if (SOP == ST_WEIGHTS) a.Wout[g] = powf(sqrtf(y.x * y.x + y.y * y.y) + a.eps, a.pexp);
+    if (SOP == ST_ENERGY) acc += (double)a.W[g] * ((double)y.x * y.x + (double)y.y * y.y);
+    if (SOP == ST_ENERGY_GRID) {
+      int k1, k2, k3;
+      kc(j, k1, k2, k3);
+      acc += (double)a.W[k1 + a.n1 * k2] * ((double)y.x * y.x + (double)y.y * y.y);
+    }
+  };
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < R1; ++r) {
-    float2 x = live ? load_op<LOP>(a, addr(t + 16 * r)) : make_float2(0.f, 0.f);
+    float2 x = live ? ld(t + 16 * r) : make_float2(0.f, 0.f);
     v[r] = KIND == PK_FWD ? x : cswap(x);
   }
   reg_fft<R1, 16>(v, t, xb, tw);
@@ -177,13 +228,13 @@ __global__ __launch_bounds__(kThreads) void rsh_kernel(AxisArgs a) {
     reg_fft<16, R1>(v, t, xb, tw);
     if (live) {
 #pragma unroll
-      for (int r = 0; r < R1; ++r) store_op<SOP>(a, addr(t + 16 * r), cscale(v[r], a.scale), acc);
+      for (int r = 0; r < R1; ++r) st(t + 16 * r, cscale(v[r], a.scale), acc);
     }
   } else if (t < R1 && live) {
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) {
       float2 y = KIND == PK_INV ? cswap(v[k2]) : v[k2];
-      store_op<SOP>(a, addr(t + R1 * k2), cscale(y, a.scale), acc);
+      st(t + R1 * k2, cscale(y, a.scale), acc);
     }
   }
   block_energy<SOP>(a, acc, red, tid);
@@ -387,6 +438,7 @@ int run_pass(dis_ctx* c, Shearlet* sh, const Pass& ps, long long* ntiles_out) {
   a.pl0 = sh->planes[0];
   a.pl1 = sh->planes[1];
   a.pl2 = sh->planes[2];
+  a.axis = ax;
   a.band = ps.band;
   a.W = ps.W;
   a.Wout = ps.Wout;
