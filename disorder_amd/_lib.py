@@ -84,6 +84,8 @@ SIGNATURES = {
     "dis_gather_samples": (c_int, [c_void_p, c_void_p, c_int32, ctypes.c_int64, c_int32_p, c_int32, ctypes.c_float,
                                    c_void_p]),
     "dis_coil_precond": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, ctypes.c_float, c_float_p]),
+    "dis_filter_rows": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_int32, c_int32_p, c_int32_p, c_int32,
+                                c_int32, ctypes.c_float]),
     "dis_mlp_param_count": (c_int, []),
     "dis_mlp_forward": (c_int, [c_void_p, c_float_p, c_float_p, c_int32, c_int32_p, c_int32, c_float_p]),
     "dis_mlp_backward": (c_int, [c_void_p, c_float_p, c_float_p, c_int32, c_int32_p, c_int32, c_float_p, c_float_p,

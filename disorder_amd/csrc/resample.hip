@@ -97,6 +97,72 @@ __global__ __launch_bounds__(kThreads) void resample_kernel(ResampleArgs a) {
   }
 }
 
+// Radial Tukey window of buildFilter(N, 'tukeyIso', [], gpu, gibbsRing)
+// (Methods/buildFilter.m:69-92, ifftshifted): |k| = sqrt(sum_d (2 pi f_d / N_d)^2)
+// over the signed frequencies f_d of the DC-first grid; 1 below pi (1 - g),
+// a raised-cosine taper to 0 at pi, 0 beyond.
+__device__ __forceinline__ float tukey_iso(int f1, int f2, int f3, int n1, int n2, int n3, float g) {
+  const float k1 = 6.28318530717958647692f * (float)f1 / (float)n1;
+  const float k2 = 6.28318530717958647692f * (float)f2 / (float)n2;
+  const float k3 = 6.28318530717958647692f * (float)f3 / (float)n3;
+  const float kk = sqrtf(k1 * k1 + k2 * k2 + k3 * k3);
+  const float pi = 3.14159265358979323846f;
+  const float alpha = 1.0f - g;
+  if (kk >= pi) return 0.f;
+  if (kk >= pi * alpha) return 0.5f * (1.0f + cosf(pi * ((kk - pi * alpha) / ((1.0f - alpha) * pi))));
+  return 1.f;
+}
+
+struct RowFilterArgs {
+  float2* y;       // [Ns x n3 x C], filtered in place
+  long long Ns, C;
+  int n3, P;
+  long long tiles_lo;
+  FftPlan pl;
+  const float2* tw;
+  const int2* k12;  // level FFT-order indices of every row
+  int n1, n2;
+  float g;
+};
+
+// filtering(y, H) (Methods/filtering.m:40-67) for compact rows whose phase
+// encodes are already in k-space: per row, FFT along the readout, times the
+// window at (k1, k2, k3), inverse FFT.  Tiles of P consecutive rows, strided
+// LDS layout; every tile reads all its pencils before writing them.
+__global__ __launch_bounds__(kThreads) void tukey_rows_kernel(RowFilterArgs a) {
+  __shared__ float2 buf[kMaxTile];
+  __shared__ float2 tw[1024];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < a.n3; k += kThreads) tw[k] = a.tw[k];
+  const int P = a.P;
+  const long long c = blockIdx.x / a.tiles_lo;
+  const long long r0 = (blockIdx.x - c * a.tiles_lo) * (long long)P;
+  const int Pv = (int)(a.Ns - r0 < P ? a.Ns - r0 : P);
+  const int tot = P * a.n3;
+  for (int e = tid; e < tot; e += kThreads) {
+    const int j = e / P, p = e - j * P;
+    buf[e] = p < Pv ? a.y[(r0 + p) + a.Ns * ((long long)j + (long long)a.n3 * c)] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_tile<false>(buf, tw, a.pl, P, tid);
+  for (int e = tid; e < tot; e += kThreads) {
+    const int j = e / P, p = e - j * P;
+    float h = 0.f;
+    if (p < Pv) {
+      const int2 k = a.k12[r0 + p];
+      h = tukey_iso(signed_freq(k.x, a.n1), signed_freq(k.y, a.n2), signed_freq(j, a.n3), a.n1, a.n2, a.n3, a.g);
+    }
+    buf[e] = cswap(cscale(buf[e], h));
+  }
+  __syncthreads();
+  fft_tile<false>(buf, tw, a.pl, P, tid);
+  const float inv = 1.0f / (float)a.n3;
+  for (int e = tid; e < tot; e += kThreads) {
+    const int j = e / P, p = e - j * P;
+    if (p < Pv) a.y[(r0 + p) + a.Ns * ((long long)j + (long long)a.n3 * c)] = cscale(cswap(buf[e]), inv);
+  }
+}
+
 // out[r + Ns_out (i3 + n3 c)] = scale * in[rows[r] + Ns_in (i3 + n3 c)]
 __global__ void gather_rows_kernel(const float2* __restrict__ in, long long Ns_in, const int* __restrict__ rows,
                                    long long Ns_out, long long ncol, float scale, float2* __restrict__ out) {
@@ -238,6 +304,46 @@ int dis_gather_samples(dis_ctx* c, const void* y, int32_t Ns_in, int64_t ncol, c
                      (const int*)d, (long long)Ns_out, (long long)ncol, scale, (float2*)out);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx_stream(c)) != hipSuccess)
     return ctx_fail(c, DIS_E_HIP, "dis_gather_samples failed");
+  return DIS_OK;
+}
+
+int dis_filter_rows(dis_ctx* c, void* y, int64_t Ns, int32_t n3, int32_t C, const int32_t* k1, const int32_t* k2,
+                    int32_t n1, int32_t n2, float gibbs) {
+  if (!c) return DIS_E_INVALID;
+  if (!y || !k1 || !k2 || Ns < 0 || n3 <= 0 || C <= 0 || n1 <= 0 || n2 <= 0 || !(gibbs > 0.f && gibbs <= 1.f))
+    return ctx_fail(c, DIS_E_INVALID, "dis_filter_rows: invalid argument");
+  if (Ns == 0) return DIS_OK;
+  if (ctx_set_device(c) != hipSuccess) return ctx_fail(c, DIS_E_HIP, "hipSetDevice failed");
+  for (long long r = 0; r < Ns; ++r)
+    if (k1[r] < 0 || k1[r] >= n1 || k2[r] < 0 || k2[r] >= n2)
+      return ctx_fail(c, DIS_E_INVALID, "dis_filter_rows: sample index outside the grid");
+  RowFilterArgs a{};
+  if (make_plan(n3, &a.pl)) return ctx_fail(c, DIS_E_UNSUPPORTED, "dis_filter_rows: unsupported readout length");
+  int st = ctx_twiddles(c, n3, &a.tw);
+  if (st) return st;
+  std::vector<int2> h((size_t)Ns);
+  for (long long r = 0; r < Ns; ++r) h[r] = make_int2(k1[r], k2[r]);
+  void* dk = nullptr;
+  st = ctx_buffer(c, 5, sizeof(int2) * (size_t)Ns, &dk);
+  if (st) return st;
+  hipStream_t s = ctx_stream(c);
+  if (hipMemcpyAsync(dk, h.data(), sizeof(int2) * (size_t)Ns, hipMemcpyHostToDevice, s) != hipSuccess)
+    return ctx_fail(c, DIS_E_HIP, "dis_filter_rows: upload failed");
+  a.y = (float2*)y;
+  a.Ns = Ns;
+  a.C = C;
+  a.n3 = n3;
+  a.k12 = (const int2*)dk;
+  a.n1 = n1;
+  a.n2 = n2;
+  a.g = gibbs;
+  a.P = (int)std::min<long long>(std::max(1, kMaxTile / n3), Ns);
+  a.tiles_lo = (Ns + a.P - 1) / a.P;
+  const long long tiles = a.tiles_lo * C;
+  if (tiles > 0x7fffffffLL) return ctx_fail(c, DIS_E_UNSUPPORTED, "dis_filter_rows: too many tiles");
+  hipLaunchKernelGGL(tukey_rows_kernel, dim3((unsigned)tiles), dim3(kThreads), 0, s, a);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return ctx_fail(c, DIS_E_HIP, "dis_filter_rows failed");
   return DIS_OK;
 }
 

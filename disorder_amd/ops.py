@@ -479,6 +479,24 @@ def gather_samples(y, rows, scale=1.0, ctx: Context = None):
     return out
 
 
+def filter_rows(y, k1, k2, n, gibbs, ctx: Context = None):
+    """Methods/downsampleOperators.m:54-57: the radial Tukey window
+    buildFilter(n, 'tukeyIso', [], gpu, gibbs) applied in place to compact
+    rows y [Ns, n3, C] with phase-encode indices k1, k2 (FFT order) of the
+    n1 x n2 x n3 grid (FFT, window, inverse FFT along the readout)."""
+    ctx = ctx or default_context(y.device.index or 0)
+    _ptr(y)
+    k1 = np.ascontiguousarray(k1, dtype=np.int32)
+    k2 = np.ascontiguousarray(k2, dtype=np.int32)
+    if y.dim() != 3 or len(k1) != y.shape[0] or len(k2) != y.shape[0] or y.shape[1] != n[2]:
+        raise ValueError("filter_rows: y must be [Ns, n3, C] with one (k1, k2) per row")
+    ctx._prep()
+    check(ctx.lib.dis_filter_rows(ctx.h, _ptr(y), int(y.shape[0]), int(n[2]), int(y.shape[2]),
+                                  k1.ctypes.data_as(_lib.c_int32_p), k2.ctypes.data_as(_lib.c_int32_p), int(n[0]),
+                                  int(n[1]), float(gibbs)), ctx.h)
+    return y
+
+
 def coil_precond(S, eps=1e-9, ctx: Context = None):
     """Reconstruction/solveXT.m:231-232: Se = 1/(sum_c |S_c|^2 + eps), float32 [n1,n2,n3]."""
     ctx = ctx or default_context(S.device.index or 0)

@@ -28,8 +28,9 @@ repeat), driving the C-ABI operators level by level:
   after the spatial levels, n_within full-resolution levels split the
   states of outlier shots in two (:270-287, :470-478).
 
-Omitted from solveXT (documented deviations): the Gibbs-ringing apodisation
-of y (downsampleOperators.m:54-57).
+Every level's data pass the Gibbs-ringing Tukey window
+(downsampleOperators.m:54-57); the last `accel` levels estimate no motion
+(pyramidPlan.m:23).
 """
 from __future__ import annotations
 
@@ -60,7 +61,7 @@ class Level:
     pair (lm_coils); image solves ask cg(ncoil, T) for their encoding,
     motion, data and preconditioner."""
 
-    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None):
+    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None, gibbs=0.0):
         self.n = list(n)
         rows, l1, l2, loff = level_sampling(k1, k2, off, N[:2], n[:2])
         self.rows, self.off = rows, loff
@@ -68,6 +69,8 @@ class Level:
         C = int(S.shape[3])
         g = ops.gather_samples(y, rows if rows0 is None else np.asarray(rows0)[rows], scale, ctx)
         self.yall = ops.resampling(g, (len(rows), n[2], C), ctx) if n[2] != N[2] else g
+        if gibbs:  # Gibbs-ringing filter at the level's size (downsampleOperators.m:54-57)
+            ops.filter_rows(self.yall, l1, l2, n, gibbs, ctx)
         self.Sall = ops.resampling(S, n, ctx) if list(n) != list(N) else S
         self.k1, self.k2, self.N, self.ctx = l1, l2, N, ctx
         self.C = C
@@ -133,7 +136,8 @@ class Level:
 
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
-            robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None):
+            robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None,
+            gibbs=0.05, accel=0):
     """Returns (x [N] on the device, T [states x 6], info).  callback(level,
     it, x, T, t_solve) runs after every outer iteration (and after a level's
     final CG, with outer = last + 1) outside the timed region
@@ -162,7 +166,15 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     last level, :383-385).  Needs traj, the dict of
     disorder_amd.trajectory.trajectory_states for the scan (its state map
     must give k1, k2, off); the returned T then has one row per final state
-    and info['sweep_of_state'] maps states to shots."""
+    and info['sweep_of_state'] maps states to shots.
+    gibbs: parXT.UseGiRingi * parXT.GibbsRingi (default 0.05): every
+    level's data pass through the radial Tukey window of that width
+    (downsampleOperators.m:54-57); 0 switches the filter off.
+    accel: the last `accel` levels estimate no motion (pyramidPlan.m:23,
+    parXT.accel(1)): such a level runs only its closing CG (final_cg
+    iterations, 'Energy' 1e-3), with the last robust weights when leff > 0
+    (solveXT.m:412-414), and flags shots for a further level as converged
+    levels do."""
     import torch
     from . import trajectory as TJ
     from .sampling import harmonic_sampling
@@ -192,6 +204,9 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
         pos_row[k1, k2] = np.arange(len(k1))
     subT = np.zeros(nsweeps, bool)
     subTT = np.zeros(nsweeps, bool)
+    last_We = None
+    if not 0 <= int(accel) <= nlevels + int(n_within):
+        raise ValueError("accel must be between 0 and the number of levels")
 
     def cg_ops(L, T, final):
         return L.cg(ev[0] if final else ev[2], T, *((res_fine, traLimX) if bin_motion else ()))
@@ -213,7 +228,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             rows0 = pos_row[k1c, k2c]
         subT = np.zeros(len(sweep_of_state), bool)
         nstates = len(sweep_of_state)
-        L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0)
+        L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0, gibbs=gibbs)
         full_res = list(n) == list(N)
         sweep_of_sample = sweep_of_state[np.repeat(np.arange(nstates), np.diff(L.off))]
         if x is None:
@@ -236,6 +251,29 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             info["We"] = We
             return We
 
+        if li >= len(levels) - int(accel):  # estT(l) = 0 from the start: closing CG only
+            if robust and leff > 0 and last_We is not None:
+                L.set_weights(RB.sample_weights(last_We, sweep_of_sample))
+            if final_cg:
+                Ec, Tc, yc, Se = cg_ops(L, T, True)
+                x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
+            if robust and leff > 0:
+                We = weights(x, True)
+                subT = (We < 1.0)[sweep_of_state]
+                if not subT.any():
+                    last = li
+            if callback is not None:
+                torch.cuda.synchronize()
+                tc = time.perf_counter()
+                callback(li, 1, x, T, tc - t0 - cb_time)
+                cb_time += time.perf_counter() - tc
+            if li == len(levels) - 1:
+                subT[:] = False
+            info["levels"].append({"n": n, "outer": 0, "states_converged": nstates, "states": nstates,
+                                   "split": int(subT.sum())})
+            last_We = info.get("We", last_We)
+            L.close()
+            continue
         for it in range(1, n_outer + 1):
             if (nReset == 0 and cont == 0) or (nReset > 0 and cont % nReset == 0) or cT.all():
                 nReset += 1
@@ -285,11 +323,15 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             subT[:] = False
         info["levels"].append({"n": n, "outer": its, "states_converged": int(cT.sum()), "states": nstates,
                                "split": int(subT.sum())})
+        last_We = info.get("We", last_We)
         L.close()
     if list(x.shape) != list(N):
         x = ops.resampling(x, N, ctx)
     torch.cuda.synchronize()
     info["sweep_of_state"] = sweep_of_state
+    # sampling of the final states: sample order, and rows of y for each sample
+    info["sampling"] = {"k1": k1c, "k2": k2c, "off": offc,
+                        "rows": np.arange(len(k1c)) if rows0 is None else np.asarray(rows0)}
     info["callback_s"] = cb_time
     info["solve_s"] = time.perf_counter() - t0 - cb_time
     return x, T, info

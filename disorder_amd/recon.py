@@ -1,0 +1,80 @@
+"""Aligned reconstruction end to end (Reconstruction/solveXT.m as called
+from IO/disorderAlgorithm.m, with IO/writeData.m): coil compression, the
+scan's trajectory to motion states, the joint motion + image solve over the
+pyramid and the within-shot levels, the optional shearlet-regularised final
+image, and the NIfTI/MAT outputs.
+
+Input data are the scan's k-space lines on the full phase-encode grid
+(y [NY1, NY2, NY3, C], column-major complex64 on the device, phase encodes
+in k-space with DC first, readout in image space, zero where not acquired)
+with coil maps S [N1, N2, N3, C] and the acquired phase-encode coordinates
+in acquisition order (rec.Assign.z, solveXT.m:82-85).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import nifti, ops
+from . import pyramid as PY
+from . import trajectory as TJ
+
+
+def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), nlevels=4, n_outer=10, n_cg=8,
+                           final_cg=300, n_within=0, accel=0, gibbs=0.05, cs=False, uncorrected_cg=0,
+                           write=None, ctx=None, **solve_kw):
+    """Returns a dict: x (corrected image, device), T [states x 6],
+    sweep_of_state, eivaS, info (of pyramid.solveXT), x_cs (with cs),
+    x_uncorrected (with uncorrected_cg > 0, CG without motion, solveXT.m:242),
+    files (with write = (path_out, name[, A]))."""
+    import torch
+    ctx = ctx or ops.default_context(y.device.index or 0)
+    N = tuple(int(v) for v in S.shape[:3])
+    if tuple(y.shape[:2]) != N[:2] or y.dim() != 4:
+        raise ValueError("y must be [NY1, NY2, NY3, C] on the phase-encode grid of S")
+    # coil compression (solveXT.m:61): maps and data on the first max(eivaS) components
+    if perc is not None:
+        S, y, eivaS, _, _ = ops.compressCoils(S, list(perc), y, 3, ctx)
+        eivaS = [int(v) for v in np.atleast_1d(eivaS)]
+        eivaS = (eivaS * 3)[:3] if len(eivaS) < 3 else eivaS[:3]
+    else:
+        eivaS = [int(S.shape[3])] * 3
+    # trajectory -> states -> harmonic sampling (solveXT.m:75-141, :349-352)
+    k1, k2, off, st = TJ.sampling_from_trajectory(kTraj, kRange, n_echoes)
+    NY12 = N[0] * N[1]
+    flat = y.as_strided((NY12, y.shape[2], y.shape[3]), (1, NY12, NY12 * y.shape[2]))
+    rows = np.asarray(k1) + N[0] * np.asarray(k2)
+    yc = ops.gather_samples(flat, rows, 1.0, ctx)
+    out = {"eivaS": eivaS}
+    if uncorrected_cg:
+        L = PY.Level(yc, S, k1, k2, off, N, N, ctx, lm_coils=eivaS[0], gibbs=gibbs)
+        Ec, _, ycc, Se = L.cg(eivaS[0], np.zeros((len(off) - 1, 6)))
+        out["x_uncorrected"], _, _ = ops.CGsolver(ycc, Ec, np.zeros((len(off) - 1, 6)), nIt=uncorrected_cg,
+                                                  tolType="Energy", tol=1e-3, Se=Se)
+        L.close()
+    x, T, info = PY.solveXT(yc, S, k1, k2, off, N, nlevels=nlevels, n_outer=n_outer, n_cg=n_cg, final_cg=final_cg,
+                            eivaS=eivaS, n_within=n_within, traj=st, gibbs=gibbs, accel=accel, ctx=ctx, **solve_kw)
+    out.update({"x": x, "T": T, "sweep_of_state": info["sweep_of_state"], "info": info})
+    if cs:
+        from . import shearlet as SH
+        sm = info["sampling"]
+        yf = ops.gather_samples(yc, sm["rows"], 1.0, ctx)
+        if gibbs:
+            ops.filter_rows(yf, sm["k1"], sm["k2"], N, gibbs, ctx)
+        sweep_of_sample = info["sweep_of_state"][np.repeat(np.arange(len(sm["off"]) - 1), np.diff(sm["off"]))]
+        We = None if "We" not in info else np.asarray(info["We"])[sweep_of_sample]
+        c0 = eivaS[0]
+        Sf = S[..., :c0]
+        E = ops.Encoding(Sf, sm["k1"], sm["k2"], sm["off"], We=We, ctx=ctx)
+        out["x_cs"], out["lambdas"] = SH.CSsolver(yf[..., :c0], E, T, x, Sf, sm["k1"], sm["k2"], We)
+        E.close()
+    if write is not None:
+        path_out, name = write[0], write[1]
+        A = write[2] if len(write) > 2 else None
+        vols = {"d": x.cpu().numpy()}
+        if "x_uncorrected" in out:
+            vols["x"] = out["x_uncorrected"].cpu().numpy()
+        if "x_cs" in out:
+            vols["r"] = out["x_cs"].cpu().numpy()
+        out["files"] = nifti.write_data(path_out, name, vols, T=T, A=A)
+    torch.cuda.synchronize()
+    return out

@@ -242,6 +242,14 @@ int dis_gather_samples(dis_ctx* ctx, const void* y, int32_t Ns_in, int64_t ncol,
 /* CG preconditioner Se = 1/(sum_c |S_c|^2 + eps) per voxel
  * (Reconstruction/solveXT.m:231-232); S [nv x C] device, Se [nv] device float. */
 int dis_coil_precond(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, float eps, float* Se);
+/* Gibbs-ringing filter of the level data (Methods/downsampleOperators.m:54-57,
+ * Reconstruction/solveXT.m:237-238): filtering(y, buildFilter(NY, 'tukeyIso',
+ * [], gpu, gibbs)) for compact rows y [Ns x n3 x C] (DEVICE, in place) whose
+ * phase encodes are in k-space: per row an FFT along the readout, the radial
+ * Tukey window at (k1[r], k2[r], k3) of the n1 x n2 x n3 grid (host int32
+ * FFT-order indices), and the inverse FFT.  0 < gibbs <= 1. */
+int dis_filter_rows(dis_ctx* ctx, void* y, int64_t Ns, int32_t n3, int32_t C, const int32_t* k1, const int32_t* k2,
+                    int32_t n1, int32_t n2, float gibbs);
 
 /* ---- Learned motion proposal (no counterpart in the reference; proposes
  * per-shot motion updates that Reconstruction/LMsolver.m then accepts or

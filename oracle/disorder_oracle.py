@@ -1003,6 +1003,33 @@ def compress_motion(T, off, res, traLimX=(0.05, 0.02)):
 # ----------------------------------------------------------------------------
 # Pyramid driver (Reconstruction/solveXT.m:266-531, joint CG + LM per level)
 # ----------------------------------------------------------------------------
+def tukey_iso(N, gibbs):
+    """Methods/buildFilter.m:69-92,122 'tukeyIso': on the centred grid
+    k_d = (2 pi / N_d)(j - ceil((N_d + 1)/2)), |k| = sqrt(sum k_d^2), window 1
+    below pi alpha (alpha = 1 - gibbs), 0.5 (1 + cos(pi (|k| - pi alpha) /
+    ((1 - alpha) pi))) up to pi, 0 from pi on; ifftshifted to DC first."""
+    grids = [(2 * np.pi / n) * (np.arange(1, n + 1) - math.ceil((n + 1) / 2)) for n in N]
+    kk = np.sqrt(sum(np.abs(g.reshape([-1 if d == m else 1 for d in range(len(N))])) ** 2
+                     for m, g in enumerate(grids)))
+    x = np.ones(tuple(N))
+    alpha = 1.0 - gibbs
+    if gibbs != 0:
+        fkk = 0.5 * (1 + np.cos(np.pi * ((kk - np.pi * alpha) / ((1 - alpha) * np.pi))))
+        x = np.where(kk >= np.pi * alpha, fkk, x)
+    x = np.where(kk >= np.pi, 0.0, x)
+    return np.fft.ifftshift(x)
+
+
+def filter_level_rows(yl, l1, l2, n, gibbs):
+    """filtering(yRes, tukeyIso) of downsampleOperators.m:54-57 for compact
+    rows (phase encodes in k-space, readout in image space): FFT along the
+    readout, window at (k1, k2, k3), inverse FFT."""
+    H = tukey_iso(n, gibbs)
+    Y = np.fft.fft(yl, axis=1)
+    Y = Y * H[np.asarray(l1), np.asarray(l2), :][:, :, None]
+    return np.fft.ifft(Y, axis=1)
+
+
 def _level_kspace_rows(k1, k2, off, N12, n12):
     """Full-resolution sample rows that survive the k-space crop of a level
     and their level indices (resampling(A, NYRes(1:2), 1) followed by
@@ -1032,7 +1059,8 @@ def pyramid_levels(N, nlevels):
 
 def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
                   traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
-                  info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None, n_within=0, traj=None):
+                  info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None, n_within=0, traj=None,
+                  gibbs=0.05, accel=0):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -1060,7 +1088,11 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     at their middle profile (:270-287, traj from trajectory_states) and
     estimates only the new states on its first iteration (on all of them
     before the last level, :383-385).  info['sweep_of_state'] receives the
-    shot of every final state.
+    shot of every final state.  gibbs: every level's data filtered by the
+    radial Tukey window (downsampleOperators.m:54-57).  accel: the last
+    accel levels have estT = 0 from the start (pyramidPlan.m:23): the closing
+    CG only, with the last weights when leff > 0 (solveXT.m:412-414), then
+    the outlier flags.
     Returns (x at full size, T)."""
     from oracle import trajectory_oracle as TO
     NSw = len(off) - 1
@@ -1076,6 +1108,8 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
         row_of = {(int(a), int(b)): r for r, (a, b) in enumerate(zip(k1, k2))}
     subT = [1] * NSw
     subTT = [1] * NSw
+    last_W = None
+    info = {} if info is None else info
     levels = pyramid_levels(N, nlevels) + [(list(N), res_fine)] * int(n_within)
     Lidx = len(levels) - 1
     for li, (n, resIso) in enumerate(levels):
@@ -1100,6 +1134,8 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
         rows, l1, l2, loff = _level_kspace_rows(kc1, kc2, offc, N[:2], n[:2])
         scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
         yl = resample_fourier(ysrc[rows] * scale, (len(rows), n[2], yc.shape[2]))
+        if gibbs:
+            yl = filter_level_rows(yl, l1, l2, n, gibbs)
         Sl = resample_fourier(Smaps, tuple(n) + (Smaps.shape[3],))
         samp = Sampling(l1, l2, loff, n[0], n[1])
         full_res = list(n) == list(N)
@@ -1129,6 +1165,26 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
         cT = cT0.copy()
         tL = (traLimXT[0] * resIso, traLimXT[1] * resIso)
         nReset, cont = 0, 0
+        if li >= len(levels) - int(accel):
+            if robust and leff > 0 and last_W is not None:
+                samp.We = last_W[sweep_of_sample]
+            if final_cg:
+                samp_b, Tb = binned(T, samp.We)
+                c = ev[0]
+                x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=final_cg,
+                                    tolType="Energy", tol=1e-3, Se=Sef)
+            if robust and leff > 0:
+                Wshot = weights(ev[0], True)
+                subT = [2 if Wshot[sw] < 1 else 1 for sw in sweep_of_state]
+                if all(v == 1 for v in subT):
+                    Lidx = li
+            if callback is not None:
+                callback(li, 1, x, T)
+            if li == len(levels) - 1:
+                subT = [1] * len(sweep_of_state)
+            if info is not None and "We" in info:
+                last_W = info["We"]
+            continue
         for it in range(1, n_outer + 1):
             if (nReset == 0 and cont == 0) or (nReset > 0 and cont % nReset == 0) or cT.all():
                 nReset += 1
@@ -1161,6 +1217,8 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
                 break
         if li == len(levels) - 1:
             subT = [1] * len(sweep_of_state)
+        if info is not None and "We" in info:
+            last_W = info["We"]
     if info is not None:
         info["sweep_of_state"] = np.array(sweep_of_state)
     return resample_fourier(x, N), T

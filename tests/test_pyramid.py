@@ -188,3 +188,66 @@ def test_pyramid_driver_coil_split_matches_oracle(dev):
     assert rel_l2(xg.cpu().numpy(), x_ref) < 1e-4, rel_l2(xg.cpu().numpy(), x_ref)
     assert np.abs(Tg[:, :3] - T_ref[:, :3]).max() < 0.01
     assert np.abs(Tg[:, 3:] - T_ref[:, 3:]).max() * 180 / np.pi < 0.01
+
+
+def test_tukey_window_closed_form():
+    """buildFilter 'tukeyIso' (DC first): 1 at DC and below pi (1 - g), the
+    raised cosine between, 0 from pi on (the k-space corners)."""
+    N, g = (16, 12, 10), 0.25
+    H = O.tukey_iso(N, g)
+    assert H[0, 0, 0] == 1.0
+    # Nyquist along dim 1: |k| = pi -> 0; the corner (pi, pi, pi) -> 0
+    assert H[8, 0, 0] == 0.0 and H[8, 6, 5] == 0.0
+    # a point at radius r on axis 1: k = 2 pi f / 16
+    for f in range(1, 8):
+        r = 2 * np.pi * f / 16
+        ref = 1.0 if r < np.pi * (1 - g) else 0.5 * (1 + np.cos(np.pi * (r - np.pi * (1 - g)) / (g * np.pi)))
+        assert abs(H[f, 0, 0] - ref) < 1e-12 and abs(H[16 - f, 0, 0] - ref) < 1e-12
+    # radial in radians: f = 4 of 16 and f = 3 of 12 are both at pi/2
+    assert H[4, 0, 0] == H[0, 3, 0] and H[12, 0, 0] == H[0, 9, 0]
+    # gibbs = 0: a hard spherical shutter at pi
+    H0 = O.tukey_iso(N, 0.0)
+    assert set(np.unique(H0)) <= {0.0, 1.0} and H0[7, 0, 0] == 1.0 and H0[8, 0, 0] == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,C", [((32, 24, 16), 3), ((20, 18, 15), 2), ((64, 48, 40), 4)])
+def test_filter_rows_matches_oracle(dev, n, C):
+    from disorder_amd import ops
+    rng = np.random.default_rng(30)
+    Ns = 300
+    l1 = rng.integers(0, n[0], Ns)
+    l2 = rng.integers(0, n[1], Ns)
+    y = crandn(rng, (Ns, n[2], C))
+    ref = O.filter_level_rows(y.astype(np.complex64).astype(np.complex128), l1, l2, n, 0.05)
+    yd = _to_dev(y, dev)
+    ops.filter_rows(yd, l1, l2, n, 0.05)
+    assert rel_l2(yd.cpu().numpy(), ref) < 1e-5
+
+
+@pytest.mark.gpu
+def test_pyramid_driver_accel_matches_oracle(dev):
+    """accel = 1 (pyramidPlan.m:23 default): the full-resolution level runs
+    only its closing CG; with the Gibbs window on (default 0.05) the GPU
+    driver and the oracle driver agree (image 1e-4, motion 0.01)."""
+    from disorder_amd import pyramid as PY
+    from tests.helpers import smooth_volume
+    rng = np.random.default_rng(23)
+    N = (32, 24, 16)
+    S, C = 16, 4
+    x_gt = smooth_volume(rng, N, nblob=8)
+    Sm = _coil_maps_np(N, C, 5)
+    H = checkered_sampling(rng, N[0], N[1], S, 4, 4)
+    k1, k2, off = SA.harmonic_sampling(H, S)
+    T_gt = np.zeros((S, 6))
+    T_gt[:, :3] = rng.uniform(-1, 1, (S, 3))
+    samp = O.Sampling(k1, k2, off, N[0], N[1])
+    y = O.encode(x_gt, Sm, samp, T_gt, N).astype(np.complex64).astype(np.complex128)
+    Sm32 = Sm.astype(np.complex64).astype(np.complex128)
+    kw = dict(nlevels=2, n_outer=3, n_cg=4, final_cg=5, accel=1)
+    x_ref, T_ref = O.pyramid_solve(y, Sm32, k1, k2, off, N, **kw)
+    xg, Tg, info = PY.solveXT(_to_dev(y, dev), _to_dev(Sm, dev), k1, k2, off, N, **kw)
+    assert info["levels"][1]["outer"] == 0
+    assert rel_l2(xg.cpu().numpy(), x_ref) < 1e-4, rel_l2(xg.cpu().numpy(), x_ref)
+    assert np.abs(Tg[:, :3] - T_ref[:, :3]).max() < 0.01
+    assert np.abs(Tg[:, 3:] - T_ref[:, 3:]).max() * 180 / np.pi < 0.01

@@ -128,8 +128,11 @@ def _corrupted_scan(seed=31, N=(32, 24, 16), S=16, C=4, bad=5, gain=3.0):
 def test_oracle_driver_rejects_corrupted_shot():
     x_gt, _, Sm32, y, k1, k2, off, N, bad = _corrupted_scan()
     info = {}
-    xr, _ = O.pyramid_solve(y, Sm32, k1, k2, off, N, nlevels=2, n_outer=3, n_cg=4, robust=True, info=info)
-    xp, _ = O.pyramid_solve(y, Sm32, k1, k2, off, N, nlevels=2, n_outer=3, n_cg=4, robust=False)
+    # gibbs = 0: the Tukey window zeroes the k-space corners of the data (not of
+    # the model), which raises the residuals of shots sampling the corners of
+    # this full rectangular grid; a scanner's elliptical shutter never samples them
+    xr, _ = O.pyramid_solve(y, Sm32, k1, k2, off, N, nlevels=2, n_outer=3, n_cg=4, robust=True, info=info, gibbs=0)
+    xp, _ = O.pyramid_solve(y, Sm32, k1, k2, off, N, nlevels=2, n_outer=3, n_cg=4, robust=False, gibbs=0)
     We = info["We"]
     assert We[bad] < 0.05 and (np.delete(We, bad) > 0.5).all(), We
     assert rel_l2(xr, x_gt) < 0.8 * rel_l2(xp, x_gt), (rel_l2(xr, x_gt), rel_l2(xp, x_gt))
@@ -141,9 +144,10 @@ def test_driver_robust_weights_match_oracle(dev):
     from tests.test_pyramid import _to_dev
     x_gt, Sm, Sm32, y, k1, k2, off, N, bad = _corrupted_scan()
     info_o = {}
-    x_ref, T_ref = O.pyramid_solve(y, Sm32, k1, k2, off, N, nlevels=2, n_outer=3, n_cg=4, robust=True, info=info_o)
+    x_ref, T_ref = O.pyramid_solve(y, Sm32, k1, k2, off, N, nlevels=2, n_outer=3, n_cg=4, robust=True, info=info_o,
+                                   gibbs=0)
     xg, Tg, info = PY.solveXT(_to_dev(y, dev), _to_dev(Sm, dev), k1, k2, off, N, nlevels=2, n_outer=3, n_cg=4,
-                              robust=True)
+                              robust=True, gibbs=0)
     assert np.abs(info["We"] - info_o["We"]).max() < 1e-3, (info["We"], info_o["We"])
     assert info["We"][bad] < 0.05
     assert rel_l2(xg.cpu().numpy(), x_ref) < 1e-4
