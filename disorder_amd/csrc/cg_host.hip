@@ -21,8 +21,11 @@ namespace dis {
 int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const float* Se, const float* Ma,
            double lam, float2* x, bool x_init, int nIt, int tolType, double tol, double* En, int* nEn, int* nn,
            dis_shearlet* sh, double sh_scale) {
-  if (tolType != 1 && tolType != 6 && tolType != 7)
-    return ctx_fail(c, DIS_E_UNSUPPORTED, "CG tolType must be RelativeResidual2Error(1), Energy(6) or None(7)");
+  // tolType 8: Simulation/Xsolver.m:53-75 -- stop when max |alpha p|^2 < tol, or when sqrt|rs| < 1e-10,
+  // or after nIt iterations (the caller turns Xsolver's operation budget into nIt)
+  if (tolType != 1 && tolType != 6 && tolType != 7 && tolType != 8)
+    return ctx_fail(c, DIS_E_UNSUPPORTED,
+                    "CG tolType must be RelativeResidual2Error(1), Energy(6), None(7) or MaxUpdate(8)");
   if (sh && tolType == 6) return ctx_fail(c, DIS_E_UNSUPPORTED, "CG with the shearlet term: tolType 'Energy' not supported");
   hipStream_t st = ctx_stream(c);
   const long long n = (long long)v.g.n[0] * v.g.n[1] * v.g.n[2];
@@ -79,7 +82,8 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
   CgScalars h;
   CCHK(c, hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st));
   CCHK(c, hipStreamSynchronize(st));
-  if (std::hypot(h.rsold.x, h.rsold.y) < 1e-6) {  // early termination (:70-75)
+  // early termination (:70-75); Xsolver only stops on sqrt|rs| < 1e-10
+  if (std::hypot(h.rsold.x, h.rsold.y) < (tolType == 8 ? 1e-20 : 1e-6)) {
     if (nn) *nn = 0;
     if (nEn) *nEn = 0;
     return DIS_OK;
@@ -95,6 +99,7 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
     CCHK(c, cg_constrain_dot(q, p, Ma, flam, n, part, &sc->pq, st));
     CCHK(c, cg_step(sc, st));
     CCHK(c, cg_update(x, r, z, p, q, Se, sc, n, part, part2, st));
+    if (tolType == 8) CCHK(c, cg_maxabs2(p, n, part, &sc->dot, st));  // before p changes
     CCHK(c, cg_direction(p, z, sc, n, st));
     if (tolType == 6 && it % 2 == 1) {
       double e2[2];
@@ -105,6 +110,12 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
     }
     CCHK(c, hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st));
     CCHK(c, hipStreamSynchronize(st));
+    if (tolType == 8) {
+      const double g2 = h.g.x * h.g.x + h.g.y * h.g.y;
+      if (g2 * h.dot.x < tol) break;                 // max |alpha p|^2 < toler
+      if (std::sqrt(std::hypot(h.rs.x, h.rs.y)) < 1e-10) break;
+      continue;
+    }
     if (tolType == 1) {
       err = std::sqrt(h.zr_abs.x / bnorm2);
     } else if (tolType == 6 && it % 2 == 1 && it > 1) {

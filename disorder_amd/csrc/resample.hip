@@ -113,6 +113,15 @@ __device__ __forceinline__ float tukey_iso(int f1, int f2, int f3, int n1, int n
   return 1.f;
 }
 
+// buildFilter(N, 'CubicBSpline') (Methods/buildFilter.m:102-106): per
+// dimension 6 sinc(f/N)^4 / (4 + 2 cos(2 pi f/N)), sinc(t) = sin(pi t)/(pi t).
+__device__ __forceinline__ float bspline1(int f, int n) {
+  const float t = (float)f / (float)n;
+  const float s = f == 0 ? 1.0f : sinpif(t) / (3.14159265358979323846f * t);
+  const float s2 = s * s;
+  return 6.0f * s2 * s2 / (4.0f + 2.0f * cospif(2.0f * t));
+}
+
 struct RowFilterArgs {
   float2* y;       // [Ns x n3 x C], filtered in place
   long long Ns, C;
@@ -150,7 +159,9 @@ __global__ __launch_bounds__(kThreads) void tukey_rows_kernel(RowFilterArgs a) {
     float h = 0.f;
     if (p < Pv) {
       const int2 k = a.k12[r0 + p];
-      h = tukey_iso(signed_freq(k.x, a.n1), signed_freq(k.y, a.n2), signed_freq(j, a.n3), a.n1, a.n2, a.n3, a.g);
+      const int f1 = signed_freq(k.x, a.n1), f2 = signed_freq(k.y, a.n2), f3 = signed_freq(j, a.n3);
+      h = a.g <= 1.0f ? tukey_iso(f1, f2, f3, a.n1, a.n2, a.n3, a.g)
+                      : bspline1(f1, a.n1) * bspline1(f2, a.n2) * bspline1(f3, a.n3);
     }
     buf[e] = cswap(cscale(buf[e], h));
   }
@@ -310,7 +321,7 @@ int dis_gather_samples(dis_ctx* c, const void* y, int32_t Ns_in, int64_t ncol, c
 int dis_filter_rows(dis_ctx* c, void* y, int64_t Ns, int32_t n3, int32_t C, const int32_t* k1, const int32_t* k2,
                     int32_t n1, int32_t n2, float gibbs) {
   if (!c) return DIS_E_INVALID;
-  if (!y || !k1 || !k2 || Ns < 0 || n3 <= 0 || C <= 0 || n1 <= 0 || n2 <= 0 || !(gibbs > 0.f && gibbs <= 1.f))
+  if (!y || !k1 || !k2 || Ns < 0 || n3 <= 0 || C <= 0 || n1 <= 0 || n2 <= 0 || !(gibbs > 0.f))
     return ctx_fail(c, DIS_E_INVALID, "dis_filter_rows: invalid argument");
   if (Ns == 0) return DIS_OK;
   if (ctx_set_device(c) != hipSuccess) return ctx_fail(c, DIS_E_HIP, "hipSetDevice failed");

@@ -73,6 +73,37 @@ __global__ __launch_bounds__(kThreads) void dot_kernel(const float2* __restrict_
   block_store(re, im, partial);
 }
 
+// per-block max |a|^2 into partial[].x (FP64)
+__global__ __launch_bounds__(kThreads) void maxabs2_kernel(const float2* __restrict__ a, long long n,
+                                                           double2* partial) {
+  __shared__ double s[kThreads];
+  double m = 0;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads) {
+    const float2 u = a[i];
+    m = fmax(m, (double)u.x * u.x + (double)u.y * u.y);
+  }
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = kThreads / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(s[0], 0.0);
+}
+
+__global__ __launch_bounds__(kThreads) void finalize_max_kernel(const double2* partial, int nb, double2* out) {
+  __shared__ double s[kThreads];
+  double m = 0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) m = fmax(m, partial[i].x);
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = kThreads / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = make_double2(s[0], 0.0);
+}
+
 // q = Ma .* (q + lam p); partial of sum conj(p) q
 __global__ __launch_bounds__(kThreads) void constrain_dot_kernel(float2* __restrict__ q, const float2* __restrict__ p,
                                                                  const float* __restrict__ Ma, float lam, long long n,
@@ -215,6 +246,13 @@ hipError_t cg_dot(const float2* a, const float2* b, long long n, double2* partia
   const int nb = grid_for(n);
   hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(kThreads), 0, st, a, b, n, partial);
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, st, partial, nb, out);
+  return hipGetLastError();
+}
+
+hipError_t cg_maxabs2(const float2* a, long long n, double2* partial, double2* out, hipStream_t st) {
+  const int nb = grid_for(n);
+  hipLaunchKernelGGL(maxabs2_kernel, dim3(nb), dim3(kThreads), 0, st, a, n, partial);
+  hipLaunchKernelGGL(finalize_max_kernel, dim3(1), dim3(kThreads), 0, st, partial, nb, out);
   return hipGetLastError();
 }
 

@@ -20,6 +20,8 @@ constexpr int kCgPartials = 1024;
 
 hipError_t cg_dot(const float2* a, const float2* b, long long n, double2* partial, double2* out, hipStream_t st);
 hipError_t cg_mask(float2* v, const float* Ma, long long n, hipStream_t st);
+// out->x = max_i |a_i|^2
+hipError_t cg_maxabs2(const float2* a, long long n, double2* partial, double2* out, hipStream_t st);
 hipError_t cg_constrain_dot(float2* q, const float2* p, const float* Ma, float lam, long long n, double2* partial,
                             double2* out, hipStream_t st);
 hipError_t cg_residual(const float2* b, const float2* q, float2* r, float2* z, float2* p, const float* Se, long long n,

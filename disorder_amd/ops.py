@@ -270,7 +270,7 @@ class Encoding:
         return ps[:self.Ns] if per_sample else tot.value
 
 
-TOLTYPES = {"RelativeResidual2Error": 1, "Energy": 6, "None": 7}
+TOLTYPES = {"RelativeResidual2Error": 1, "Energy": 6, "None": 7, "MaxUpdate": 8}
 
 
 def CGsolver(y, E: "Encoding", T, x=None, nIt=300, tolType="Energy", tol=1e-5, Se=None, Ma=None, lam=0.0):

@@ -116,7 +116,9 @@ int dis_energy(dis_plan* plan, const double* T, const void* x, const void* y, co
  * Se, Ma: DEVICE [n1,n2,n3] real preconditioner (A.Se) and mask (C.Ma), or
  * NULL.  x: DEVICE [n1,n2,n3], initial guess when x_init (else zeroed), the
  * solution on return.  tolType: 1 'RelativeResidual2Error', 6 'Energy',
- * 7 'None' (Reconstruction/stopCondition.m).  Inner products, the step
+ * 7 'None' (Reconstruction/stopCondition.m), or 8, the rule of
+ * Simulation/Xsolver.m:53-75: stop once max|alpha p|^2 < tol or
+ * sqrt|sum conj(z) r| < 1e-10 (nIt then caps the iterations).  Inner products, the step
  * scalars and the residual norm are FP64.  En (host, 2 doubles per odd
  * iteration: fidelity, regulariser) receives the 'Energy' history, nEn its
  * column count, nn the iterations run (0 on the early exit of
@@ -247,7 +249,9 @@ int dis_coil_precond(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, float e
  * [], gpu, gibbs)) for compact rows y [Ns x n3 x C] (DEVICE, in place) whose
  * phase encodes are in k-space: per row an FFT along the readout, the radial
  * Tukey window at (k1[r], k2[r], k3) of the n1 x n2 x n3 grid (host int32
- * FFT-order indices), and the inverse FFT.  0 < gibbs <= 1. */
+ * FFT-order indices), and the inverse FFT.  0 < gibbs <= 1; gibbs > 1
+ * selects the cubic B-spline filter buildFilter(N, 'CubicBSpline') that
+ * downsampleOperators.m:54-55 uses at coarse levels otherwise. */
 int dis_filter_rows(dis_ctx* ctx, void* y, int64_t Ns, int32_t n3, int32_t C, const int32_t* k1, const int32_t* k2,
                     int32_t n1, int32_t n2, float gibbs);
 

@@ -1020,11 +1020,22 @@ def tukey_iso(N, gibbs):
     return np.fft.ifftshift(x)
 
 
+def cubic_bspline(N):
+    """Methods/buildFilter.m:102-106 'CubicBSpline' (ifftshifted): per
+    dimension 6 sinc(k/(2 pi))^4 / (4 + 2 cos k), k = (2 pi/N)(j - ceil((N+1)/2))."""
+    x = np.ones(tuple(N))
+    for m, n in enumerate(N):
+        k = (2 * np.pi / n) * (np.arange(1, n + 1) - math.ceil((n + 1) / 2))
+        f = 6 * np.sinc(k / (2 * np.pi)) ** 4 / (4 + 2 * np.cos(k))
+        x = x * f.reshape([-1 if d == m else 1 for d in range(len(N))])
+    return np.fft.ifftshift(x)
+
+
 def filter_level_rows(yl, l1, l2, n, gibbs):
     """filtering(yRes, tukeyIso) of downsampleOperators.m:54-57 for compact
     rows (phase encodes in k-space, readout in image space): FFT along the
     readout, window at (k1, k2, k3), inverse FFT."""
-    H = tukey_iso(n, gibbs)
+    H = tukey_iso(n, gibbs) if gibbs <= 1 else cubic_bspline(n)
     Y = np.fft.fft(yl, axis=1)
     Y = Y * H[np.asarray(l1), np.asarray(l2), :][:, :, None]
     return np.fft.ifft(Y, axis=1)

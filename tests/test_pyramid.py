@@ -210,18 +210,30 @@ def test_tukey_window_closed_form():
     assert set(np.unique(H0)) <= {0.0, 1.0} and H0[7, 0, 0] == 1.0 and H0[8, 0, 0] == 0.0
 
 
+def test_cubic_bspline_closed_form():
+    """6 sinc(f/N)^4 / (4 + 2 cos(2 pi f/N)) per dimension, DC first: 1 at DC,
+    separable, even in f, and 6 sinc(1/2)^4/2 = 48/pi^4 at the Nyquist bin."""
+    N = (16, 12, 10)
+    H = O.cubic_bspline(N)
+    assert abs(H[0, 0, 0] - 1) < 1e-15
+    assert abs(H[8, 0, 0] - 48 / np.pi ** 4) < 1e-12
+    assert abs(H[3, 5, 2] - H[3, 0, 0] * H[0, 5, 0] * H[0, 0, 2]) < 1e-15
+    assert abs(H[3, 0, 0] - H[13, 0, 0]) < 1e-15
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,C", [((32, 24, 16), 3), ((20, 18, 15), 2), ((64, 48, 40), 4)])
-def test_filter_rows_matches_oracle(dev, n, C):
+@pytest.mark.parametrize("n,C,g", [((32, 24, 16), 3, 0.05), ((20, 18, 15), 2, 0.05), ((64, 48, 40), 4, 0.05),
+                                   ((32, 24, 16), 2, 2.0)])
+def test_filter_rows_matches_oracle(dev, n, C, g):
     from disorder_amd import ops
     rng = np.random.default_rng(30)
     Ns = 300
     l1 = rng.integers(0, n[0], Ns)
     l2 = rng.integers(0, n[1], Ns)
     y = crandn(rng, (Ns, n[2], C))
-    ref = O.filter_level_rows(y.astype(np.complex64).astype(np.complex128), l1, l2, n, 0.05)
+    ref = O.filter_level_rows(y.astype(np.complex64).astype(np.complex128), l1, l2, n, g)
     yd = _to_dev(y, dev)
-    ops.filter_rows(yd, l1, l2, n, 0.05)
+    ops.filter_rows(yd, l1, l2, n, g)
     assert rel_l2(yd.cpu().numpy(), ref) < 1e-5
 
 
