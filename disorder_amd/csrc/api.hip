@@ -59,6 +59,7 @@ struct dis_plan {
   int* rowptr = nullptr;  // [S][n1+1] per-state CSR by k1
   int* csr = nullptr;     // [Ns]
   std::vector<int> hoff;
+  int fold[2] = {0, 0};   // acquired phase-encode grid under SENSE folding
 };
 
 namespace {
@@ -455,6 +456,33 @@ int dis_plan_destroy(dis_plan* p) {
   return DIS_OK;
 }
 
+int dis_plan_set_fold(dis_plan* p, int32_t ny1, int32_t ny2) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  const dis_geom& g = p->g;
+  if (ny1 == g.n[0] && ny2 == g.n[1]) {
+    p->fold[0] = p->fold[1] = 0;
+    return DIS_OK;
+  }
+  if (ny1 <= 0 || ny2 <= 0 || ny1 > g.n[0] || ny2 > g.n[1])
+    return fail(c, DIS_E_INVALID, "dis_plan_set_fold: the acquired grid must be within the reconstruction grid");
+  FftPlan pl;
+  if (ny1 > 1024 || ny2 > 1024 || g.n[0] > 1024 || make_plan(ny1, &pl))
+    return fail(c, DIS_E_UNSUPPORTED, "dis_plan_set_fold: unsupported acquired length");
+  if (g.nsamples > 0) {
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int> h1(g.nsamples), h2(g.nsamples);
+    HIPCHK(c, hipMemcpy(h1.data(), p->k1, sizeof(int) * g.nsamples, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(h2.data(), p->k2, sizeof(int) * g.nsamples, hipMemcpyDeviceToHost));
+    for (int j = 0; j < g.nsamples; ++j)
+      if (h1[j] >= ny1 || h2[j] >= ny2)
+        return fail(c, DIS_E_INVALID, "dis_plan_set_fold: sample index outside the acquired grid");
+  }
+  p->fold[0] = ny1;
+  p->fold[1] = ny2;
+  return DIS_OK;
+}
+
 int dis_plan_set_weights(dis_plan* p, const float* We) {
   if (!p) return DIS_E_INVALID;
   dis_ctx* c = p->ctx;
@@ -485,6 +513,8 @@ OpView plan_view(dis_plan* p) {
   v.csr = p->csr;
   v.g = p->g;
   v.hoff = p->hoff.data();
+  v.fold[0] = p->fold[0];
+  v.fold[1] = p->fold[1];
   return v;
 }
 

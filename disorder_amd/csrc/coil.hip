@@ -274,6 +274,7 @@ __global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
 }
 
 hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  if (a.fy1 > 0) return launch_coil_fold(a, mode, nstates, st);
   dim3 grid(a.n3, nstates);
   switch (mode) {
     case COIL_ENCODE: hipLaunchKernelGGL(coil_kernel<COIL_ENCODE>, grid, dim3(kThreads), 0, st, a); break;
@@ -652,6 +653,7 @@ hipError_t launch_rcoil_persistent(const CoilArgs& a, int mode, int nstates, hip
 }
 
 hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  if (a.fy1 > 0) return launch_coil_fold(a, mode, nstates, st);
   if (!a.queue || a.n2 > 1024 || a.force_generic) return hipErrorInvalidValue;
   switch (reg_radix(a.n1)) {
     case 16: return launch_rcoil_persistent<16>(a, mode, nstates, st);
@@ -672,6 +674,7 @@ int coil_chunk(int n1, int n2) {
 }
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  if (a.fy1 > 0) return launch_coil_fold(a, mode, nstates, st);
   if (a.n2 <= 1024 && !a.force_generic) {
     switch (reg_radix(a.n1)) {
       case 16: return launch_rcoil<16>(a, mode, nstates, st);

@@ -91,6 +91,14 @@ struct CoilArgs {
   const int* csr;         // [Ns] (k2 << 16) | (sample index within its state), sorted by k1 per state
   const int* slist;       // optional [nstates] list of the states handled (U/V indexed by list position)
   int* queue;             // persistent launch: 8 per-XCD work counters (zeroed before the launch), else null
+  // SENSE folding: acquired phase-encode grid fy1 x fy2 smaller than n1 x n2
+  // (0 = none); k1/k2 then index the acquired grid
+  int fy1, fy2;
+  int fmaxnec;            // largest state (samples) handled
+  float2* fwork;          // scratch of coil_fold_work_bytes
+  FftPlan fy1pl;          // FFT plan and twiddles of length fy1; twiddles of fy2
+  const float2* fy1tw;
+  const float2* fy2tw;
 };
 
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };
@@ -107,6 +115,11 @@ bool coil_queue_available(int n1, int n2);
 // (n1, n2) plane: a state with more samples is processed in several chunks.
 int coil_chunk(int n1, int n2);
 hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStream_t st);
+// The coil stage with SENSE folding (CoilArgs::fy1 > 0): batched kernels
+// per state; fwork must hold coil_fold_work_bytes(...).  launch_coil and
+// launch_coil_persistent dispatch here when folding is on.
+hipError_t launch_coil_fold(const CoilArgs& a, int mode, int nstates, hipStream_t st);
+size_t coil_fold_work_bytes(int n1, int n2, int n3, int C, int ny1, int ny2, int maxnec);
 
 // ---------------------------------------------------------------- utilities
 hipError_t launch_bcast(const float2* x, float2* out, long long nvox, int S, hipStream_t st);

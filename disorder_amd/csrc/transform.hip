@@ -309,6 +309,23 @@ int rigid_inverse(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, c
   return DIS_OK;
 }
 
+int fold_setup(dis_ctx* c, const OpView& v, int maxnec, CoilArgs* a) {
+  if (v.fold[0] <= 0) return DIS_OK;
+  const dis_geom& g = v.g;
+  if (make_plan(v.fold[0], &a->fy1pl)) return ctx_fail(c, DIS_E_UNSUPPORTED, "SENSE folding: unsupported acquired length");
+  int rc = ctx_twiddles(c, v.fold[0], &a->fy1tw);
+  if (!rc) rc = ctx_twiddles(c, v.fold[1], &a->fy2tw);
+  if (rc) return rc;
+  void* w = nullptr;
+  rc = ctx_buffer(c, 6, coil_fold_work_bytes(g.n[0], g.n[1], g.n[2], g.ncoils, v.fold[0], v.fold[1], maxnec), &w);
+  if (rc) return rc;
+  a->fwork = (float2*)w;
+  a->fy1 = v.fold[0];
+  a->fy2 = v.fold[1];
+  a->fmaxnec = std::max(1, maxnec);
+  return DIS_OK;
+}
+
 int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* yin, float2* yout,
              float2* xout, int mode, EnergyOut* eo) {
   const dis_geom& g = v.g;
@@ -368,8 +385,11 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
       queue = (int*)qb;
     }
   }
+  CoilArgs fold_a{};
+  rc = fold_setup(c, v, maxnec, &fold_a);
+  if (rc) return rc;
   auto coil_args = [&](int s0, const float2* Usrc, long long us) {
-    CoilArgs a{};
+    CoilArgs a = fold_a;
     a.U = Usrc;
     a.us = us;
     a.us_rel = 1;
@@ -725,8 +745,15 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
   float2* X0 = G + 6 * nv * Sb;
   float2* Yv = X0 + 3 * nv * Sb;
   const int G16 = std::max(1, std::min(g.n[1], kMaxTile / g.n[0]));
+  CoilArgs fold_a{};
+  {
+    int maxnec = 0;
+    for (int s = 0; s < g.nstates; ++s) maxnec = std::max(maxnec, v.hoff[s + 1] - v.hoff[s]);
+    rc = fold_setup(c, v, maxnec, &fold_a);
+    if (rc) return rc;
+  }
   auto coil_args = [&](const int* dlist, const float2* U, int mode) {
-    CoilArgs a{};
+    CoilArgs a = fold_a;
     a.U = U;
     a.us = nv;
     a.us_rel = 1;

@@ -40,7 +40,13 @@ struct OpView {
   const int* rowptr;  // per-state CSR of the samples by k1
   const int* csr;
   const int* hoff;  // host copy of the offsets
+  int fold[2];      // acquired phase-encode grid when SENSE folding is on, else {0, 0}
 };
+
+// Fills the SENSE-folding fields of a coil-stage argument block (no-op
+// without folding): plan and twiddles of the acquired grid, scratch for
+// states of up to maxnec samples (context buffer slot 6).
+int fold_setup(dis_ctx* c, const OpView& v, int maxnec, CoilArgs* a);
 
 struct EnergyOut {
   const float* Mbe = nullptr;  // host [n3] or null

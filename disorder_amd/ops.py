@@ -171,10 +171,12 @@ def sincRigidTransformGradient(x: torch.Tensor, T, Nfull=None, ctx: Context = No
 class Encoding:
     """The E / EH structures of the reconstruction path as one plan
     (dis_plan): coil maps Sf, per-segment sampling (E.Fs rows as FFT-order
-    indices k1/k2, offsets E.nSt), weights EH.We and NMs motion states."""
+    indices k1/k2, offsets E.nSt), weights EH.We and NMs motion states;
+    NY (the acquired phase-encode grid, E.Uf{1:2}.NY) below the maps' grid
+    turns on SENSE folding, with k1/k2 indexing the NY grid."""
 
     def __init__(self, Smaps: torch.Tensor, k1, k2, state_off, Nfull=None, We=None, nmotion=None,
-                 ctx: Context = None):
+                 ctx: Context = None, NY=None):
         self.ctx = ctx or default_context(Smaps.device.index or 0)
         self.S = Smaps  # kept alive: the plan borrows the pointer
         n = tuple(Smaps.shape[:3])
@@ -208,6 +210,11 @@ class Encoding:
                                            self.off.ctypes.data_as(_lib.c_int32_p), wp, ctypes.byref(h)),
               self.ctx.h)
         self.h = h
+        # SENSE folding (E.Uf / EH.Ub): data acquired on an NY[0] x NY[1]
+        # phase-encode grid smaller than the reconstruction grid
+        self.NY = tuple(int(v) for v in NY[:2]) if NY is not None else n[:2]
+        if self.NY != n[:2]:
+            check(self.ctx.lib.dis_plan_set_fold(self.h, self.NY[0], self.NY[1]), self.ctx.h)
 
     def close(self):
         if getattr(self, "h", None):

@@ -96,6 +96,15 @@ int dis_plan_create(dis_ctx* ctx, const dis_geom* g, const void* S_maps, const i
 int dis_plan_destroy(dis_plan* plan);
 /* Replace EH.We ([Ns] host, or NULL for none). */
 int dis_plan_set_weights(dis_plan* plan, const float* We);
+/* SENSE folding of the phase encodes (E.Uf{1:2}, EH.Ub{1:2}; Methods/fold.m,
+ * ifold.m; Reconstruction/encode.m:100-104, decode.m:73-76): the data were
+ * acquired on an ny1 x ny2 phase-encode grid smaller than the plan's n1 x n2
+ * reconstruction grid, so E aliases S_c T_s x onto ny1 x ny2 (fold.m with
+ * FOV(2) = ceil((n - ny) / 2)) before the Fourier rows of size ny, and E^H
+ * replicates back (ifold.m).  k1/k2 of the plan must index the ny grid.
+ * ny = n turns folding off.  Every operator of the plan (encode, decode,
+ * encodedecode, energy, CG, LM) then runs folded. */
+int dis_plan_set_fold(dis_plan* plan, int32_t ny1, int32_t ny2);
 
 /* y = E x  (Reconstruction/encode.m:1).  T: host [nmotion x 6]. */
 int dis_encode(dis_plan* plan, const double* T, const void* x, void* y);

@@ -459,6 +459,90 @@ def build_harmonic_sampling(H, nstates):
 
 
 # ----------------------------------------------------------------------------
+# SENSE folding (Methods/fold.m, ifold.m, buildFoldM.m)
+# ----------------------------------------------------------------------------
+def _fold_consts(NS, NY):
+    odd = 2 * math.ceil((math.ceil(NS / NY) - 1) / 2) + 1      # oddFactSENSE
+    over = (NS - NY) / 2
+    return odd, (math.floor(over), math.ceil(over))
+
+
+def fold(x, m, NS, NY):
+    """Methods/fold.m:30-110 with A empty and fl = 0: the NS samples of axis
+    m aliased onto NY.  Non-multiples: the central window
+    x(FOV(2)+1:NS-FOV(1)) plus the windows shifted by +-s NY (:52-68) and the
+    two partial wraps at the ends (:85-100); multiples: reshape to
+    [NY, NS/NY], sum, circshift by -FOV(2) (:104-109)."""
+    if NS == NY:
+        return x
+    x = np.moveaxis(np.asarray(x), m, 0)
+    odd, FOV = _fold_consts(NS, NY)
+    oFRed = (odd - 3) // 2
+    oFRedT = oFRed * NY
+    if NS % NY != 0:
+        xb = x[FOV[1]:NS - FOV[0]].copy()
+        for s_ in range(1, oFRed + 1):
+            xb += x[FOV[1] - s_ * NY:NS - FOV[0] - s_ * NY] + x[FOV[1] + s_ * NY:NS - FOV[0] + s_ * NY]
+        xb[0:FOV[0] - oFRedT] += x[NS - FOV[0] + oFRedT:NS]
+        xb[NY - FOV[1] + oFRedT:NY] += x[0:FOV[1] - oFRedT]
+    else:
+        q = NS // NY
+        xb = x.reshape((q, NY) + x.shape[1:]).sum(axis=0)     # element i + NY k -> row i
+        xb = np.roll(xb, -FOV[1], axis=0)
+    return np.moveaxis(xb, 0, m)
+
+
+def ifold(x, m, NS, NY):
+    """Methods/ifold.m:28-72 with A empty and fl = 0: unfolding NY -> NS,
+    indUnf = [1+iFOV(1):NY, repmat(1:NY, oFRed), 1:NY-iFOV(2)] (:46) or, for
+    multiples, circshift by -iFOV(1) and repmat (:66-70)."""
+    if NS == NY:
+        return x
+    x = np.moveaxis(np.asarray(x), m, 0)
+    odd, _ = _fold_consts(NS, NY)
+    oFRed = odd - 2
+    disc = (odd * NY - NS) / 2
+    iFOV = (math.floor(disc), math.ceil(disc))
+    if NS % NY != 0:
+        ind = np.r_[np.arange(iFOV[0], NY), np.tile(np.arange(NY), oFRed), np.arange(0, NY - iFOV[1])]
+        out = x[ind]
+    else:
+        out = np.tile(np.roll(x, -iFOV[0], axis=0), (NS // NY,) + (1,) * (x.ndim - 1))
+    return np.moveaxis(out, 0, m)
+
+
+def build_fold_matrix(NS, NY):
+    """Methods/buildFoldM.m:37-48 (emre = 0): the [NY x NS] 0/1 matrix A
+    with fold(x) = A x; the central run of 2 oFRed + 1 windows (:44) and the
+    two end wraps (:45), set through MATLAB column-major linear indices."""
+    A = np.zeros(NY * NS)
+    if NS == NY:
+        return np.eye(NY)
+    odd, FOV = _fold_consts(NS, NY)
+    oFRed = (odd - 3) // 2
+    oFRedT = oFRed * NY
+    rows = np.tile(np.arange(1, NY + 1), 2 * oFRed + 1)
+    cols = np.arange(FOV[1] - oFRed * NY, NS - FOV[0] + oFRed * NY)        # 0-based column (MATLAB col - 1)
+    A[rows + cols * NY - 1] = 1
+    r2 = np.r_[np.arange(1, FOV[0] - oFRedT + 1), np.arange(NY - FOV[1] + 1 + oFRedT, NY + 1)]
+    c2 = np.r_[np.arange(NS - FOV[0] + 1 + oFRedT, NS + 1), np.arange(1, FOV[1] - oFRedT + 1)]
+    A[r2 + (c2 - 1) * NY - 1] = 1
+    return A.reshape((NY, NS), order="F")
+
+
+def _fold12(v, n, ny):
+    for m in range(2):
+        v = fold(v, m, n[m], ny[m])
+    return v
+
+
+def _ifold12(v, n, ny):
+    for m in range(2):
+        v = ifold(v, m, n[m], ny[m])
+    return v
+
+
+# ----------------------------------------------------------------------------
 # Operators (Reconstruction/encode.m, decode.m, encodedecode.m)
 # ----------------------------------------------------------------------------
 def _transform_states(x, T, n, Nfull, nmotion, S):
@@ -490,12 +574,13 @@ def encode(x, Smaps, samp: Sampling, T, Nfull, nmotion=None):
     xt = _transform_states(x, T, n, Nfull, nmotion, samp.S)
     C = Smaps.shape[3]
     y = np.zeros((samp.Ns, n[2], C), np.complex128)
+    ny = (samp.n1, samp.n2)
     for s in range(samp.S):
         sl, k1, k2 = samp.state(s)
         if len(k1) == 0:
             continue
         for c in range(C):
-            X = _fft2o(Smaps[..., c] * xt[..., s])
+            X = _fft2o(_fold12(Smaps[..., c] * xt[..., s], n, ny))   # SENSE folding, encode.m:100-104
             y[sl, :, c] = X[k1, k2, :]
     return y
 
@@ -525,7 +610,8 @@ def decode(y, Smaps, samp: Sampling, T, Nfull, nmotion=None):
     (:46), conj(F2 rows) and F1 rows' adjoint (:51-52), conj(S) and coil sum
     (:82-84), inverse transform (di=0) and state sum (:96-112)."""
     T = np.atleast_2d(np.asarray(T, np.float64))
-    n1, n2 = samp.n1, samp.n2
+    ny1, ny2 = samp.n1, samp.n2
+    n1, n2 = Smaps.shape[:2]
     n3 = y.shape[1]
     C = Smaps.shape[3]
     S = samp.S
@@ -537,9 +623,10 @@ def decode(y, Smaps, samp: Sampling, T, Nfull, nmotion=None):
         if samp.We is not None:
             ys = ys * samp.We[sl][:, None, None]
         for c in range(C):
-            K = np.zeros((n1, n2, n3), np.complex128)
+            K = np.zeros((ny1, ny2, n3), np.complex128)
             np.add.at(K, (k1, k2), ys[:, :, c])
-            v[..., s] += np.conj(Smaps[..., c]) * _ifft2o(K)
+            # SENSE unfolding (decode.m:73-76)
+            v[..., s] += np.conj(Smaps[..., c]) * _ifold12(_ifft2o(K), (n1, n2), (ny1, ny2))
     return _inverse_states(v, T, (n1, n2, n3), Nfull, nmotion)
 
 

@@ -1,0 +1,157 @@
+"""SENSE folding of the phase encodes (Methods/fold.m, ifold.m,
+buildFoldM.m; Reconstruction/encode.m:100-104, decode.m:73-76).
+
+CPU: the oracle's literal fold.m (window slicing, or reshape-sum-circshift
+for multiples) equals the literal buildFoldM.m matrix and the alias sum over
+r = i + ceil((NX - NY)/2) (mod NY); ifold.m is its adjoint and the
+transposed matrix; for NX = 2 NY the DFT of the folded signal is the
+NX-point DFT at the even frequencies times exp(2 pi i k o / NY); an object
+inside the central NY window folds to its crop, so the folded encode equals
+the unfolded encode on the cropped grid; the folded encode/decode pair passes
+the adjoint dot test.
+GPU: a plan with dis_plan_set_fold against the oracle for encode, decode,
+E^H E (weights, motion with dim-1 translation), per-sample energies, CG and
+the LM sums, on a non-multiple, an odd-difference and a multiple grid."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import disorder_oracle as O
+from tests.helpers import crandn, random_motion, rel_l2, smooth_volume
+
+SIZES = [(8, 8), (10, 8), (9, 8), (12, 8), (15, 8), (16, 8), (24, 8), (20, 16), (7, 4), (13, 10), (31, 24),
+         (40, 32), (36, 27)]
+
+
+@pytest.mark.parametrize("NX,NY", SIZES)
+def test_fold_matches_matrix_alias_sum_and_adjoint(NX, NY):
+    rng = np.random.default_rng(NX * 100 + NY)
+    x = crandn(rng, (NX, 3))
+    f = O.fold(x, 0, NX, NY)
+    A = O.build_fold_matrix(NX, NY)
+    assert np.array_equal(A.sum(axis=0), np.ones(NX))          # every sample lands once
+    assert np.abs(A @ x - f).max() < 1e-14
+    o = math.ceil((NX - NY) / 2)
+    ref = np.zeros((NY, 3), complex)
+    for r in range(NX):
+        ref[(r - o) % NY] += x[r]
+    assert np.abs(ref - f).max() < 1e-14
+    y = crandn(rng, (NY, 3))
+    u = O.ifold(y, 0, NX, NY)
+    assert np.abs(A.T @ y - u).max() < 1e-14
+    assert abs(np.vdot(y, f) - np.vdot(u, x)) < 1e-12
+    # along another axis
+    x3 = crandn(rng, (3, NX, 2))
+    assert np.abs(O.fold(x3, 1, NX, NY) - np.einsum("ij,ajb->aib", A, x3)).max() < 1e-14
+
+
+def test_fold_then_dft_is_subsampled_spectrum():
+    rng = np.random.default_rng(3)
+    NY = 12
+    x = crandn(rng, 2 * NY)
+    o = math.ceil(NY / 2)
+    k = np.arange(NY)
+    lhs = np.fft.fft(O.fold(x, 0, 2 * NY, NY))
+    rhs = np.fft.fft(x)[2 * k] * np.exp(2j * np.pi * k * o / NY)
+    assert np.abs(lhs - rhs).max() < 1e-12
+
+
+def _fold_problem(rng, n, ny, C=3, S=4, weights=False):
+    x = smooth_volume(rng, n) + 0.1 * crandn(rng, n)
+    Sm = crandn(rng, n + (C,)) * 0.5
+    H = rng.integers(1, S + 1, size=ny)
+    k1, k2, off, _ = O.build_harmonic_sampling(H, S)
+    We = rng.uniform(0.5, 1.5, off[S]) if weights else None
+    samp = O.Sampling(k1, k2, off[:S + 1], ny[0], ny[1], We)
+    return x, Sm, samp
+
+
+def test_object_inside_window_folds_to_its_crop():
+    rng = np.random.default_rng(4)
+    n, ny = (20, 18, 6), (16, 12)
+    x, Sm, samp = _fold_problem(rng, n, ny)
+    o = [math.ceil((n[d] - ny[d]) / 2) for d in range(2)]
+    mask = np.zeros(n)
+    mask[o[0]:o[0] + ny[0], o[1]:o[1] + ny[1], :] = 1
+    x = x * mask
+    T0 = np.zeros((samp.S, 6))
+    y = O.encode(x, Sm, samp, T0, n)
+    crop = (slice(o[0], o[0] + ny[0]), slice(o[1], o[1] + ny[1]))
+    yc = O.encode(x[crop], Sm[crop], samp, T0, (ny[0], ny[1], n[2]))
+    assert np.abs(y - yc).max() < 1e-12 * np.abs(y).max()
+
+
+def test_folded_encode_decode_adjoint():
+    rng = np.random.default_rng(5)
+    n, ny = (20, 15, 6), (16, 12)
+    x, Sm, samp = _fold_problem(rng, n, ny, weights=False)
+    T = random_motion(rng, samp.S)
+    y = crandn(rng, (samp.Ns, n[2], Sm.shape[3]))
+    lhs = np.vdot(y, O.encode(x, Sm, samp, T, n))
+    rhs = np.vdot(O.decode(y, Sm, samp, T, n), x)
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+
+
+def _to_dev(a, dev):
+    import torch
+    import disorder_amd as D
+    return D.colmajor(torch.from_numpy(np.asfortranarray(np.asarray(a, np.complex64))).to(dev))
+
+
+# (reconstruction grid, acquired phase-encode grid): non-multiple, odd
+# difference with a 3^3 acquired length, and multiples (register-path n1)
+GRIDS = [((40, 36, 16), (32, 30)), ((40, 36, 12), (35, 27)), ((64, 48, 16), (32, 24))]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,ny", GRIDS)
+def test_folded_operators_match_oracle(dev, n, ny):
+    import disorder_amd as D
+    rng = np.random.default_rng(sum(n) + sum(ny))
+    x, Sm, samp = _fold_problem(rng, n, ny, C=3, S=4, weights=True)
+    T = random_motion(rng, samp.S)
+    Sm = Sm.astype(np.complex64).astype(np.complex128)
+    x = x.astype(np.complex64).astype(np.complex128)
+    E = D.Encoding(_to_dev(Sm, dev), samp.k1, samp.k2, samp.off, We=samp.We, NY=ny)
+    xd = _to_dev(x, dev)
+    y_g = E.encode(xd, T).cpu().numpy()
+    y_r = O.encode(x, Sm, samp, T, n)
+    assert rel_l2(y_g, y_r) < 1e-5, rel_l2(y_g, y_r)
+    yn = (y_r + 0.1 * crandn(rng, y_r.shape)).astype(np.complex64).astype(np.complex128)
+    yd = _to_dev(yn, dev)
+    x_g = E.decode(yd, T).cpu().numpy()
+    assert rel_l2(x_g, O.decode(yn, Sm, samp, T, n)) < 1e-5
+    ehe = E.encodedecode(xd, T).cpu().numpy()
+    assert rel_l2(ehe, O.encodedecode(x, Sm, samp, T, n)) < 1e-5
+    e_g = E.energy(xd, yd, T, per_sample=True)
+    e_r = O.compute_energy(yn, x, Sm, samp, T, n, per_sample=True)
+    assert rel_l2(e_g, e_r) < 1e-5
+    # zero motion goes through the coil stage without the transform
+    T0 = np.zeros_like(T)
+    assert rel_l2(E.encode(xd, T0).cpu().numpy(), O.encode(x, Sm, samp, T0, n)) < 1e-5
+    E.close()
+
+
+@pytest.mark.gpu
+def test_folded_cg_and_lm_match_oracle(dev):
+    import disorder_amd as D
+    n, ny = (40, 36, 12), (32, 30)
+    rng = np.random.default_rng(11)
+    x, Sm, samp = _fold_problem(rng, n, ny, C=4, S=4)
+    Sm = Sm.astype(np.complex64).astype(np.complex128)
+    x = x.astype(np.complex64).astype(np.complex128)
+    Tgt = random_motion(rng, samp.S, 3, 1)
+    y = O.encode(x, Sm, samp, Tgt, n) + 0.01 * crandn(rng, (samp.Ns, n[2], 4))
+    y = y.astype(np.complex64).astype(np.complex128)
+    E = D.Encoding(_to_dev(Sm, dev), samp.k1, samp.k2, samp.off, NY=ny)
+    yd = _to_dev(y, dev)
+    xg, _, _ = D.CGsolver(yd, E, Tgt, nIt=6, tolType="RelativeResidual2Error", tol=0.0)
+    xr = O.cg_solver(y, Sm, samp, Tgt, n, nIt=6, tolType="RelativeResidual2Error", tol=0.0)[0]
+    assert rel_l2(xg.cpu().numpy(), xr) < 1e-4, rel_l2(xg.cpu().numpy(), xr)
+    T0 = Tgt + random_motion(rng, samp.S, 1.0, 0.3)
+    En, dG, dH = D.lm_partials(yd, E, _to_dev(x, dev), T0)
+    Er, dGr, dHr = O.lm_partials(y, x, Sm, samp, T0, n)
+    assert rel_l2(En, Er) < 1e-5 and rel_l2(dG, dGr) < 1e-4 and rel_l2(dH, dHr) < 1e-4
+    E.close()
+
