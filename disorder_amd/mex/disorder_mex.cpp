@@ -148,6 +148,13 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     plhs[0] = make_handle(b);
     return;
   }
+  if (cmd == "plan_set_fold") {  // E.Uf{1:2}.NY: disorder_mex('plan_set_fold', plan, [NY1 NY2])
+    PlanBox* b = handle<PlanBox>(prhs[1]);
+    if (mxGetNumberOfElements(prhs[2]) < 2) mexErrMsgIdAndTxt("disorder:arg", "expected [NY1 NY2]");
+    const double* v = mxGetPr(prhs[2]);
+    check(b->ctx, dis_plan_set_fold(b->plan, static_cast<int32_t>(v[0]), static_cast<int32_t>(v[1])));
+    return;
+  }
   if (cmd == "plan_destroy") {
     PlanBox* b = handle<PlanBox>(prhs[1]);
     check(b->ctx, dis_plan_destroy(b->plan));
