@@ -292,6 +292,7 @@ hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStre
 // of one readout plane run back to back on one XCD: they share that plane's
 // coil maps through the XCD's L2.
 constexpr int kRJ = 3;  // samples per thread per chunk on the register path
+constexpr int kHornerBatch = 16;  // rows of a full group whose LDS reads may be in flight together
 
 // Next (state, plane) item for a persistent workgroup whose home queue is
 // `home`: items of queue k are planes i3 = k (mod 8), all states of a plane
@@ -410,23 +411,45 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
               for (int k = 0; k < 16; ++k) st2(Zt + p * N + t + R1 * k, v[k]);
             }
             __syncthreads();
+            if (Gv == 16) {
+              // full group: no per-row guard, so the LDS reads of a sample's
+              // Horner chain are issued kHornerBatch at a time
 #pragma unroll
-            for (int i = 0; i < kRJ; ++i) {
-              if (jk1[i] >= 0) {
-                // sum_q W2^{k2 (r0 + q)} Z[q] = W2^{k2 r0} * (Horner in a = W2^{k2})
-                const float2 a1 = ld2(tw2 + jk2[i]);
-                float2 acc = make_float2(0.f, 0.f);
+              for (int i = 0; i < kRJ; ++i) {
+                if (jk1[i] >= 0) {
+                  const float2 a1 = ld2(tw2 + jk2[i]);
+                  float2 acc = ld2(Zt + 15 * N + jk1[i]);
 #pragma unroll
-                for (int q = 15; q >= 0; --q) {
-                  if (q < Gv) {
+                  for (int q = 14; q >= 0; --q) {
                     const float2 z = ld2(Zt + q * N + jk1[i]);
                     acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
+                    if ((q & (kHornerBatch - 1)) == 0) __builtin_amdgcn_sched_barrier(0);
                   }
-                }
-                acc = cmul(acc, ld2(tw2 + ib[i]));
+                  acc = cmul(acc, ld2(tw2 + ib[i]));
 #pragma unroll
-                for (int cq = 0; cq < kCC; ++cq)
-                  if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
+                  for (int cq = 0; cq < kCC; ++cq)
+                    if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
+                }
+              }
+            } else {
+#pragma unroll
+              for (int i = 0; i < kRJ; ++i) {
+                if (jk1[i] >= 0) {
+                  // sum_q W2^{k2 (r0 + q)} Z[q] = W2^{k2 r0} * (Horner in a = W2^{k2})
+                  const float2 a1 = ld2(tw2 + jk2[i]);
+                  float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+                  for (int q = 15; q >= 0; --q) {
+                    if (q < Gv) {
+                      const float2 z = ld2(Zt + q * N + jk1[i]);
+                      acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
+                    }
+                  }
+                  acc = cmul(acc, ld2(tw2 + ib[i]));
+#pragma unroll
+                  for (int cq = 0; cq < kCC; ++cq)
+                    if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
+                }
               }
             }
             __syncthreads();  // the row tile aliases the next FFT's exchange buffer
