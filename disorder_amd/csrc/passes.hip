@@ -52,6 +52,16 @@ __device__ __forceinline__ TileInfo tile_info(const PassArgs& a, int t) {
   return ti;
 }
 
+// Strided axes: workgroups are dealt to the eight XCDs round-robin, so
+// without a remap an XCD's L2 sees every eighth 128-byte column segment.
+// With the tile count a multiple of 8, XCD x (blocks b = x mod 8) instead
+// takes the contiguous range of tiles [x ntiles/8, (x+1) ntiles/8).
+__device__ __forceinline__ int xcd_tile(const PassArgs& a, int b) {
+  const int n = (int)gridDim.x;
+  if (a.axis == 0 || (n & 7)) return b;
+  return (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // coordinate of pencil p of the tile along axis `src`
 __device__ __forceinline__ int src_coord(const PassArgs& a, const TileInfo& ti, int p) {
   if (a.axis == 0) {
@@ -113,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void pass_kernel(PassArgs a) {
   const int tot = P * n;
   const float scale = 1.0f / (float)n;
   for (int k = tid; k < n; k += kThreads) tw[k] = a.tw[k];
-  const TileInfo ti = tile_info(a, blockIdx.x);
+  const TileInfo ti = tile_info(a, xcd_tile(a, blockIdx.x));
 
   if (MODE == MODE_MAP) {
     const int s = a.states ? a.states[blockIdx.y] : (int)blockIdx.y;
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void rpass_kernel(PassArgs a) {
   lds_float2* xb = (lds_float2*)xraw + p * XB;
   for (int k = tid; k < N; k += kThreads) st2(tw + k, a.tw[k]);
   __syncthreads();
-  const TileInfo ti = tile_info(a, blockIdx.x);
+  const TileInfo ti = tile_info(a, xcd_tile(a, blockIdx.x));
   const float scale = 1.0f / (float)N;
   float2 v[16];
   if (MODE == MODE_MAP) {
