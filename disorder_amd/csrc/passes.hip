@@ -52,13 +52,14 @@ __device__ __forceinline__ TileInfo tile_info(const PassArgs& a, int t) {
   return ti;
 }
 
-// Strided axes: workgroups are dealt to the eight XCDs round-robin, so
-// without a remap an XCD's L2 sees every eighth 128-byte column segment.
-// With the tile count a multiple of 8, XCD x (blocks b = x mod 8) instead
-// takes the contiguous range of tiles [x ntiles/8, (x+1) ntiles/8).
+// Workgroups are dealt to the eight XCDs round-robin, so without a remap an
+// XCD's L2 sees every eighth tile (on strided axes every eighth 128-byte
+// column segment).  With the tile count a multiple of 8, XCD x (blocks
+// b = x mod 8) instead takes the contiguous range of tiles
+// [x ntiles/8, (x+1) ntiles/8).
 __device__ __forceinline__ int xcd_tile(const PassArgs& a, int b) {
   const int n = (int)gridDim.x;
-  if (a.axis == 0 || (n & 7)) return b;
+  if (n & 7) return b;
   return (b & 7) * (n >> 3) + (b >> 3);
 }
 
