@@ -489,9 +489,34 @@ def main():
                           "frac_of_peak": passmodel / (ms * 1e6) / HBM_PEAK_GBS},
         "kernels": kernels,
     }
+    cp = stream_copy_rate(dev)
+    line["stream_copy"] = cp
+    line["pass_model_v1"]["frac_of_stream_copy"] = line["pass_model_v1"]["GBps"] / cp["GBps"]
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, C, S, seed=5)
     print(json.dumps(line), flush=True)
+
+
+def stream_copy_rate(dev, nbytes=2 << 30, reps=10):
+    """Achievable HBM rate on this GPU: a device-to-device copy of nbytes
+    (read + write counted), best of reps, timed with events on torch's stream."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        t = e0.elapsed_time(e1)
+        best = t if best is None else min(best, t)
+    del a, b
+    return {"GBps": 2.0 * nbytes / (best * 1e6), "bytes_moved": 2 * nbytes, "ms": best,
+            "how": "torch copy_ of a 2 GiB float32 buffer, best of 10, read + write counted"}
 
 
 if __name__ == "__main__":
