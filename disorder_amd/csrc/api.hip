@@ -58,6 +58,7 @@ struct dis_plan {
   float* We = nullptr;
   int* rowptr = nullptr;  // [S][n1+1] per-state CSR by k1
   int* csr = nullptr;     // [Ns]
+  int* sord = nullptr;    // [Ns] register coil stage: sample of each slot
   std::vector<int> hoff;
   int fold[2] = {0, 0};   // acquired phase-encode grid under SENSE folding
 };
@@ -434,6 +435,35 @@ int dis_plan_create(dis_ctx* c, const dis_geom* g, const void* S_maps, const int
          hipMemcpy(p->rowptr, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(p->csr, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   }
+  // slot -> sample table of the register coil stage: per state and chunk of
+  // kCoilRegChunk samples, slot q takes a sample with k1 = q (mod 32) while
+  // one is left (else one from the fullest residue), so the 16- and 32-lane
+  // groups of a wave gather their phase-encode rows from distinct LDS banks
+  if (ok) {
+    std::vector<int> so(std::max(1, g->nsamples));
+    for (int s = 0; s < g->nstates; ++s) {
+      for (int c0 = state_off[s]; c0 < state_off[s + 1]; c0 += kCoilRegChunk) {
+        const int m = std::min(kCoilRegChunk, state_off[s + 1] - c0);
+        std::vector<int> bucket[32];
+        for (int j = c0; j < c0 + m; ++j) bucket[k1[j] & 31].push_back(j);
+        size_t head[32] = {0};
+        for (int q = 0; q < m; ++q) {
+          int r = q & 31;
+          if (head[r] >= bucket[r].size()) {
+            size_t best = 0;
+            for (int b = 0; b < 32; ++b)
+              if (bucket[b].size() - head[b] > best) {
+                best = bucket[b].size() - head[b];
+                r = b;
+              }
+          }
+          so[c0 + q] = bucket[r][head[r]++];
+        }
+      }
+    }
+    ok = hipMalloc(&p->sord, so.size() * sizeof(int)) == hipSuccess &&
+         hipMemcpy(p->sord, so.data(), so.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+  }
   if (!ok) {
     dis_plan_destroy(p);
     return fail(c, DIS_E_NOMEM, "dis_plan_create: device allocation failed");
@@ -452,6 +482,7 @@ int dis_plan_destroy(dis_plan* p) {
   if (p->We) (void)hipFree(p->We);
   if (p->rowptr) (void)hipFree(p->rowptr);
   if (p->csr) (void)hipFree(p->csr);
+  if (p->sord) (void)hipFree(p->sord);
   delete p;
   return DIS_OK;
 }
@@ -511,6 +542,7 @@ OpView plan_view(dis_plan* p) {
   v.We = p->We;
   v.rowptr = p->rowptr;
   v.csr = p->csr;
+  v.sord = p->sord;
   v.g = p->g;
   v.hoff = p->hoff.data();
   v.fold[0] = p->fold[0];

@@ -292,6 +292,7 @@ hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStre
 // of one readout plane run back to back on one XCD: they share that plane's
 // coil maps through the XCD's L2.
 constexpr int kRJ = 3;  // samples per thread per chunk on the register path
+static_assert(kRJ * kThreads == kCoilRegChunk, "slot tables are built per chunk of kCoilRegChunk samples");
 constexpr int kHornerBatch = 16;  // rows of a full group whose LDS reads may be in flight together
 
 // Next (state, plane) item for a persistent workgroup whose home queue is
@@ -350,12 +351,16 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   bool first_out = true;
   for (int sc0 = o0; sc0 < o1 || (sc0 == o0 && ADJ); sc0 += kRJ * kThreads) {
     const int sc1 = min(o1, sc0 + kRJ * kThreads);
-    int jk1[kRJ], jk2[kRJ], jstep[kRJ];
+    int jk1[kRJ], jk2[kRJ], jstep[kRJ], jg[kRJ];
     float jw[kRJ];
 #pragma unroll
     for (int i = 0; i < kRJ; ++i) {
-      const int j = sc0 + tid + i * kThreads;
-      const bool ok = j < sc1;
+      // slot -> sample: the plan deals each chunk's samples to slots so that
+      // consecutive lanes read distinct LDS banks in the row gathers
+      const int q = sc0 + tid + i * kThreads;
+      const bool ok = q < sc1;
+      const int j = ok ? (a.sord ? a.sord[q] : q) : -1;
+      jg[i] = j;
       jk1[i] = ok ? a.k1[j] : -1;
       jk2[i] = ok ? a.k2[j] : 0;
       jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
@@ -462,7 +467,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         }
 #pragma unroll
         for (int i = 0; i < kRJ; ++i) {
-          const int j = sc0 + tid + i * kThreads;
+          const int j = jg[i];
           if (jk1[i] >= 0) {
 #pragma unroll
             for (int c = 0; c < kCC; ++c) {
@@ -484,7 +489,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
       if (MODE == COIL_DECODE) {
 #pragma unroll
         for (int i = 0; i < kRJ; ++i) {
-          const int j = sc0 + tid + i * kThreads;
+          const int j = jg[i];
           if (jk1[i] >= 0) {
 #pragma unroll
             for (int c = 0; c < kCC; ++c)
@@ -527,7 +532,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
               for (int cq = 0; cq < kCC; ++cq)
                 if (cq == c) w = W[i][cq];
-              st2(wc + tid + i * kThreads, w);
+              if (jg[i] >= 0) st2(wc + (jg[i] - sc0), w);
             }
             __syncthreads();
             for (int k1 = tid; k1 < N; k1 += kThreads) {
@@ -595,7 +600,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
     if (MODE == COIL_ENERGY) {
 #pragma unroll
       for (int i = 0; i < kRJ; ++i) {
-        const int j = sc0 + tid + i * kThreads;
+        const int j = jg[i];
         if (jk1[i] >= 0) a.epart[j + (long long)a.Ns * i3] = esum[i];
       }
     }

@@ -90,6 +90,7 @@ struct CoilArgs {
   const int* rowptr;      // [S][n1+1] per-state row pointers of the samples sorted by k1
   const int* csr;         // [Ns] (k2 << 16) | (sample index within its state), sorted by k1 per state
   const int* slist;       // optional [nstates] list of the states handled (U/V indexed by list position)
+  const int* sord;        // optional [Ns] register path: sample handled by each slot, per chunk of kCoilRegChunk
   int* queue;             // persistent launch: 8 per-XCD work counters (zeroed before the launch), else null
   // SENSE folding: acquired phase-encode grid fy1 x fy2 smaller than n1 x n2
   // (0 = none); k1/k2 then index the acquired grid
@@ -102,6 +103,9 @@ struct CoilArgs {
 };
 
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };
+
+// Samples per chunk of the register-path coil stage (3 per thread).
+constexpr int kCoilRegChunk = 768;
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st);
 // Persistent register-path coil stage: resident workgroups pull (state,

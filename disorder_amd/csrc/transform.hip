@@ -407,6 +407,7 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
     a.We = v.We;
     a.rowptr = v.rowptr;
     a.csr = v.csr;
+    a.sord = v.sord;
     a.Ns = g.nsamples;
     a.yin = yin;
     a.yout = yout;
@@ -769,6 +770,7 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
     a.We = nullptr;  // LMsolver encodes without EH.We (LMsolver.m:72,99)
     a.rowptr = v.rowptr;
     a.csr = v.csr;
+    a.sord = v.sord;
     a.Ns = g.nsamples;
     a.yin = y;
     a.yout = nullptr;

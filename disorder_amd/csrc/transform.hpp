@@ -39,6 +39,7 @@ struct OpView {
   const float* We;
   const int* rowptr;  // per-state CSR of the samples by k1
   const int* csr;
+  const int* sord;  // register coil stage: sample of each slot (or null)
   const int* hoff;  // host copy of the offsets
   int fold[2];      // acquired phase-encode grid when SENSE folding is on, else {0, 0}
 };
