@@ -165,11 +165,11 @@ int dis_autotune(dis_ctx* ctx, const int32_t n[3], const int32_t Nfull[3], int32
  * or 16). */
 int dis_set_pass_shape(dis_ctx* ctx, const int32_t n[3], int32_t axis, int32_t generic, int32_t P, int32_t maxradix);
 /* Schedule of the coil stage of encode/decode/encodedecode/energy: 0 auto
- * (default: a persistent launch fed by per-XCD (shot, readout-plane) work
- * queues when the shots' line counts differ by more than 25 %, else one
- * workgroup per item), 1 always one workgroup per item, 2 always the
- * persistent work queue.  Results are identical; only the order in which
- * items run changes. */
+ * (default: one workgroup per (shot, readout-plane) item, which measures
+ * faster than the work queue also for uneven shots), 1 always one
+ * workgroup per item, 2 the persistent launch fed by per-XCD
+ * (shot, readout-plane) work queues.  Results are identical; only the
+ * order in which items run changes. */
 int dis_set_coil_schedule(dis_ctx* ctx, int32_t mode);
 /* Forget every tuned shape (back to the built-in choice). */
 int dis_autotune_clear(dis_ctx* ctx);
