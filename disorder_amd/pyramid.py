@@ -44,12 +44,17 @@ from . import ops, robust as RB
 from .sampling import level_sampling
 
 
+def level_size(N, f):
+    """ceil(N / 2^f) per dimension (downsampleOperators.m:34)."""
+    return [int(math.ceil(v / 2 ** f)) for v in N]
+
+
 def pyramid_levels(N, nlevels):
     """[(n_l, resIso_l)] from coarse to fine (pyramidPlan.m:19-25, 1 mm voxels)."""
     out = []
     for l in range(nlevels):
         f = nlevels - 1 - l
-        out.append(([int(math.ceil(v / 2 ** f)) for v in N], math.sqrt(2.0) * 2.0 ** f))
+        out.append((level_size(N, f), math.sqrt(2.0) * 2.0 ** f))
     return out
 
 
@@ -61,17 +66,22 @@ class Level:
     pair (lm_coils); image solves ask cg(ncoil, T) for their encoding,
     motion, data and preconditioner."""
 
-    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None, gibbs=0.0):
+    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None, gibbs=0.0, NY=None, ny=None):
         self.n = list(n)
-        rows, l1, l2, loff = level_sampling(k1, k2, off, N[:2], n[:2])
+        # acquired phase-encode grid (full and at this level); below the
+        # image grid the level's encodings fold (SENSE, E.Uf / EH.Ub)
+        NY = list(N[:2]) if NY is None else [int(v) for v in NY[:2]]
+        ny = list(n[:2]) if ny is None else [int(v) for v in ny[:2]]
+        rows, l1, l2, loff = level_sampling(k1, k2, off, NY, ny)
         self.rows, self.off = rows, loff
-        scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
+        scale = math.sqrt(ny[0] * ny[1] / (NY[0] * NY[1]))
         C = int(S.shape[3])
         g = ops.gather_samples(y, rows if rows0 is None else np.asarray(rows0)[rows], scale, ctx)
         self.yall = ops.resampling(g, (len(rows), n[2], C), ctx) if n[2] != N[2] else g
         if gibbs:  # Gibbs-ringing filter at the level's size (downsampleOperators.m:54-57)
-            ops.filter_rows(self.yall, l1, l2, n, gibbs, ctx)
+            ops.filter_rows(self.yall, l1, l2, (ny[0], ny[1], n[2]), gibbs, ctx)
         self.Sall = ops.resampling(S, n, ctx) if list(n) != list(N) else S
+        self.ny = None if ny == list(n[:2]) else ny
         self.k1, self.k2, self.N, self.ctx = l1, l2, N, ctx
         self.C = C
         self.We = None
@@ -93,7 +103,7 @@ class Level:
         self._check(ncoil)
         if ncoil not in self._enc:
             self._enc[ncoil] = ops.Encoding(self.Sall[..., :ncoil], self.k1, self.k2, self.off, Nfull=self.N,
-                                            We=self.We, ctx=self.ctx)
+                                            We=self.We, ctx=self.ctx, NY=self.ny)
         return self._enc[ncoil]
 
     def precond(self, ncoil):
@@ -123,7 +133,7 @@ class Level:
             if prev is not None:
                 prev[1].close()
             self._binned[ncoil] = (offb, ops.Encoding(self.Sall[..., :ncoil], self.k1, self.k2, offb,
-                                                      Nfull=self.N, We=self.We, ctx=self.ctx))
+                                                      Nfull=self.N, We=self.We, ctx=self.ctx, NY=self.ny))
         return self._binned[ncoil][1], Tb, self.data(ncoil), self.precond(ncoil)
 
     def close(self):
@@ -137,7 +147,7 @@ class Level:
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
             robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None,
-            gibbs=0.05, accel=0):
+            gibbs=0.05, accel=0, NY=None):
     """Returns (x [N] on the device, T [states x 6], info).  callback(level,
     it, x, T, t_solve) runs after every outer iteration (and after a level's
     final CG, with outer = last + 1) outside the timed region
@@ -174,7 +184,12 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     parXT.accel(1)): such a level runs only its closing CG (final_cg
     iterations, 'Energy' 1e-3), with the last robust weights when leff > 0
     (solveXT.m:412-414), and flags shots for a further level as converged
-    levels do."""
+    levels do.
+    NY: the acquired phase-encode grid (size(y) in solveXT.m:69) when it is
+    smaller than N (SENSE folding, E.Uf / EH.Ub of solveXT.m:198-200,
+    :362-364): k1/k2 index it, each level's data live on ceil(NY/2^l)
+    (downsampleOperators.m:34) and its encodings fold the level image onto
+    that grid."""
     import torch
     from . import trajectory as TJ
     from .sampling import harmonic_sampling
@@ -228,7 +243,8 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             rows0 = pos_row[k1c, k2c]
         subT = np.zeros(len(sweep_of_state), bool)
         nstates = len(sweep_of_state)
-        L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0, gibbs=gibbs)
+        ny = None if NY is None else level_size(NY[:2], max(0, nlevels - 1 - li))
+        L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0, gibbs=gibbs, NY=NY, ny=ny)
         full_res = list(n) == list(N)
         sweep_of_sample = sweep_of_state[np.repeat(np.arange(nstates), np.diff(L.off))]
         if x is None:

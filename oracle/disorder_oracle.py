@@ -1158,7 +1158,7 @@ def pyramid_levels(N, nlevels):
 def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
                   traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
                   info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None, n_within=0, traj=None,
-                  gibbs=0.05, accel=0):
+                  gibbs=0.05, accel=0, NY=None):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -1190,7 +1190,10 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     radial Tukey window (downsampleOperators.m:54-57).  accel: the last
     accel levels have estT = 0 from the start (pyramidPlan.m:23): the closing
     CG only, with the last weights when leff > 0 (solveXT.m:412-414), then
-    the outlier flags.
+    the outlier flags.  NY: the acquired phase-encode grid when it is smaller
+    than N (SENSE folding, solveXT.m:198-200, :362-364): k1/k2 index it, each
+    level's data live on ceil(NY/2^l) (downsampleOperators.m:34) and its
+    encoding folds the level image onto that grid.
     Returns (x at full size, T)."""
     from oracle import trajectory_oracle as TO
     NSw = len(off) - 1
@@ -1210,8 +1213,10 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     info = {} if info is None else info
     levels = pyramid_levels(N, nlevels) + [(list(N), res_fine)] * int(n_within)
     Lidx = len(levels) - 1
+    NYg = list(N[:2]) if NY is None else [int(v) for v in NY[:2]]
     for li, (n, resIso) in enumerate(levels):
         leff = li - (nlevels - 1)
+        ny = level_size(NYg, max(0, nlevels - 1 - li))
         if any(v == 2 for v in subT):
             T = np.concatenate([np.repeat(T[i:i + 1], subT[i], axis=0) for i in range(len(subT))])
             subTT = [v for v in subT for _ in range(v)]
@@ -1229,25 +1234,25 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
             ysrc = yc[[row_of[(int(a), int(b))] for a, b in zip(kc1, kc2)]]
         subT = [1] * len(sweep_of_state)
         S = len(sweep_of_state)
-        rows, l1, l2, loff = _level_kspace_rows(kc1, kc2, offc, N[:2], n[:2])
-        scale = math.sqrt(n[0] * n[1] / (N[0] * N[1]))
+        rows, l1, l2, loff = _level_kspace_rows(kc1, kc2, offc, NYg, ny)
+        scale = math.sqrt(ny[0] * ny[1] / (NYg[0] * NYg[1]))
         yl = resample_fourier(ysrc[rows] * scale, (len(rows), n[2], yc.shape[2]))
         if gibbs:
-            yl = filter_level_rows(yl, l1, l2, n, gibbs)
+            yl = filter_level_rows(yl, l1, l2, (ny[0], ny[1], n[2]), gibbs)
         Sl = resample_fourier(Smaps, tuple(n) + (Smaps.shape[3],))
-        samp = Sampling(l1, l2, loff, n[0], n[1])
+        samp = Sampling(l1, l2, loff, ny[0], ny[1])
         full_res = list(n) == list(N)
         sweep_of_sample = np.array(sweep_of_state)[samp.state_of_sample()]
 
         def binned(T, We):
             if not bin_motion:
-                return Sampling(l1, l2, loff, n[0], n[1], We), T
+                return Sampling(l1, l2, loff, ny[0], ny[1], We), T
             Tb, offb, _ = compress_motion(T, loff, res_fine, traLimX)
-            return Sampling(l1, l2, offb, n[0], n[1], We), Tb
+            return Sampling(l1, l2, offb, ny[0], ny[1], We), Tb
 
         def weights(c, finer):
             samp_b, Tb = binned(T, samp.We)
-            ps = compute_energy(yl[..., :c], x, Sl[..., :c], Sampling(l1, l2, samp_b.off, n[0], n[1]), Tb, N,
+            ps = compute_energy(yl[..., :c], x, Sl[..., :c], Sampling(l1, l2, samp_b.off, ny[0], ny[1]), Tb, N,
                                 per_sample=True)
             Wshot = robust_shot_weights(ps, sweep_of_sample, NSw, finer=finer)
             samp.We = Wshot[sweep_of_sample]
