@@ -8,7 +8,8 @@ Input data are the scan's k-space lines on the full phase-encode grid
 (y [NY1, NY2, NY3, C], column-major complex64 on the device, phase encodes
 in k-space with DC first, readout in image space, zero where not acquired)
 with coil maps S [N1, N2, N3, C] and the acquired phase-encode coordinates
-in acquisition order (rec.Assign.z, solveXT.m:82-85).
+in acquisition order (rec.Assign.z, solveXT.m:82-85).  An acquired grid
+NY1 x NY2 smaller than N1 x N2 is SENSE-folded (solveXT.m:198-200).
 """
 from __future__ import annotations
 
@@ -29,8 +30,13 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
     import torch
     ctx = ctx or ops.default_context(y.device.index or 0)
     N = tuple(int(v) for v in S.shape[:3])
-    if tuple(y.shape[:2]) != N[:2] or y.dim() != 4:
-        raise ValueError("y must be [NY1, NY2, NY3, C] on the phase-encode grid of S")
+    if y.dim() != 4 or y.shape[0] > N[0] or y.shape[1] > N[1] or y.shape[2] != N[2]:
+        raise ValueError("y must be [NY1, NY2, N3, C] with NY1 <= N1, NY2 <= N2 (the maps' grid)")
+    NY = (int(y.shape[0]), int(y.shape[1]))
+    folded = NY != N[:2]
+    if folded and cs:
+        raise NotImplementedError("the shearlet-regularised image with SENSE folding (its fidelity trace "
+                                  "would need folding too, traceEstimation.m:66-70) is not built")
     # coil compression (solveXT.m:61): maps and data on the first max(eivaS) components
     if perc is not None:
         S, y, eivaS, _, _ = ops.compressCoils(S, list(perc), y, 3, ctx)
@@ -40,19 +46,22 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
         eivaS = [int(S.shape[3])] * 3
     # trajectory -> states -> harmonic sampling (solveXT.m:75-141, :349-352)
     k1, k2, off, st = TJ.sampling_from_trajectory(kTraj, kRange, n_echoes)
-    NY12 = N[0] * N[1]
+    if tuple(st["H"].shape) != NY:
+        raise ValueError("kRange does not span the acquired phase-encode grid of y")
+    NY12 = NY[0] * NY[1]
     flat = y.as_strided((NY12, y.shape[2], y.shape[3]), (1, NY12, NY12 * y.shape[2]))
-    rows = np.asarray(k1) + N[0] * np.asarray(k2)
+    rows = np.asarray(k1) + NY[0] * np.asarray(k2)
     yc = ops.gather_samples(flat, rows, 1.0, ctx)
     out = {"eivaS": eivaS}
     if uncorrected_cg:
-        L = PY.Level(yc, S, k1, k2, off, N, N, ctx, lm_coils=eivaS[0], gibbs=gibbs)
+        L = PY.Level(yc, S, k1, k2, off, N, N, ctx, lm_coils=eivaS[0], gibbs=gibbs, NY=NY, ny=NY)
         Ec, _, ycc, Se = L.cg(eivaS[0], np.zeros((len(off) - 1, 6)))
         out["x_uncorrected"], _, _ = ops.CGsolver(ycc, Ec, np.zeros((len(off) - 1, 6)), nIt=uncorrected_cg,
                                                   tolType="Energy", tol=1e-3, Se=Se)
         L.close()
     x, T, info = PY.solveXT(yc, S, k1, k2, off, N, nlevels=nlevels, n_outer=n_outer, n_cg=n_cg, final_cg=final_cg,
-                            eivaS=eivaS, n_within=n_within, traj=st, gibbs=gibbs, accel=accel, ctx=ctx, **solve_kw)
+                            eivaS=eivaS, n_within=n_within, traj=st, gibbs=gibbs, accel=accel, ctx=ctx,
+                            NY=NY if folded else None, **solve_kw)
     out.update({"x": x, "T": T, "sweep_of_state": info["sweep_of_state"], "info": info})
     if cs:
         from . import shearlet as SH

@@ -211,3 +211,33 @@ def test_folded_driver_matches_oracle(dev):
     assert rel_l2(xg.cpu().numpy(), x_ref) < 1e-4, rel_l2(xg.cpu().numpy(), x_ref)
     assert np.abs(Tg[:, :3] - T_ref[:, :3]).max() < 0.01
     assert np.abs(Tg[:, 3:] - T_ref[:, 3:]).max() * 180 / np.pi < 0.01
+
+
+@pytest.mark.gpu
+def test_folded_end_to_end_matches_driver(dev):
+    """aligned_reconstruction on data acquired on a 32x24 grid with 40x30
+    maps: the same image and motion as the driver called with NY on the
+    compact samples (the entry point only gathers and forwards)."""
+    from disorder_amd import pyramid as PY
+    from disorder_amd import recon
+    from disorder_amd import trajectory as TR
+    from tests.test_pyramid import _coil_maps_np
+    N, NY, S, C = (40, 30, 16), (32, 24), 16, 3
+    rng = np.random.default_rng(9)
+    g1, g2 = np.meshgrid(np.arange(NY[0]) - NY[0] // 2, np.arange(NY[1]) - NY[1] // 2, indexing="ij")
+    k = np.stack([g1.ravel(order="F"), g2.ravel(order="F")], 1)
+    ko, NP, _, _, _, _ = TR.sampling_disorder(k, (4, 4), "Checkered", "AlternatingZigZag", K=NY)
+    kRange = [[-(NY[0] // 2), NY[0] - NY[0] // 2 - 1], [-(NY[1] // 2), NY[1] - NY[1] // 2 - 1]]
+    k1, k2, off, st = TR.sampling_from_trajectory(ko, kRange, NP // S)
+    x = smooth_volume(rng, N, nblob=8)
+    Sm = _coil_maps_np(N, C, 5)
+    y = O.encode(x, Sm, O.Sampling(k1, k2, off, NY[0], NY[1]), random_motion(rng, S, 2, 1), N)
+    Y = np.zeros(NY + (N[2], C), complex)
+    Y[k1, k2] = y
+    kw = dict(nlevels=2, n_outer=2, n_cg=4, final_cg=4)
+    out = recon.aligned_reconstruction(_to_dev(Y, dev), _to_dev(Sm, dev), ko, kRange, NP // S, perc=None, **kw)
+    xg, Tg, _ = PY.solveXT(_to_dev(y, dev), _to_dev(Sm, dev), k1, k2, off, N, traj=st, NY=NY, **kw)
+    assert np.array_equal(out["T"], Tg)
+    assert rel_l2(out["x"].cpu().numpy(), xg.cpu().numpy()) < 1e-6
+    with pytest.raises(NotImplementedError):
+        recon.aligned_reconstruction(_to_dev(Y, dev), _to_dev(Sm, dev), ko, kRange, NP // S, perc=None, cs=True, **kw)
