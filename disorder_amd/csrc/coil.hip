@@ -688,6 +688,7 @@ hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipS
     case 14: return launch_rcoil_persistent<14>(a, mode, nstates, st);
     case 12: return launch_rcoil_persistent<12>(a, mode, nstates, st);
     case 10: return launch_rcoil_persistent<10>(a, mode, nstates, st);
+    case 9: return launch_rcoil_persistent<9>(a, mode, nstates, st);
     case 8: return launch_rcoil_persistent<8>(a, mode, nstates, st);
     case 4: return launch_rcoil_persistent<4>(a, mode, nstates, st);
     case 2: return launch_rcoil_persistent<2>(a, mode, nstates, st);
@@ -709,6 +710,7 @@ hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st)
       case 14: return launch_rcoil<14>(a, mode, nstates, st);
       case 12: return launch_rcoil<12>(a, mode, nstates, st);
       case 10: return launch_rcoil<10>(a, mode, nstates, st);
+      case 9: return launch_rcoil<9>(a, mode, nstates, st);
       case 8: return launch_rcoil<8>(a, mode, nstates, st);
       case 4: return launch_rcoil<4>(a, mode, nstates, st);
       case 2: return launch_rcoil<2>(a, mode, nstates, st);

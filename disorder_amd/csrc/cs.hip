@@ -404,6 +404,7 @@ int run_rsh(dis_ctx* c, Shearlet* sh, const Pass& ps, AxisArgs& a, int R1, bool 
     DIS_RR(2)
     DIS_RR(4)
     DIS_RR(8)
+    DIS_RR(9)
     DIS_RR(10)
     DIS_RR(12)
     DIS_RR(14)

@@ -562,6 +562,7 @@ int reg_radix(int n) {
     case 224: return 14;
     case 192: return 12;
     case 160: return 10;
+    case 144: return 9;
     case 128: return 8;
     case 64: return 4;
     case 32: return 2;
@@ -599,6 +600,7 @@ hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t
       case 14: return launch_rpass<14>(a, mode, grid_states, st);
       case 12: return launch_rpass<12>(a, mode, grid_states, st);
       case 10: return launch_rpass<10>(a, mode, grid_states, st);
+      case 9: return launch_rpass<9>(a, mode, grid_states, st);
       case 8: return launch_rpass<8>(a, mode, grid_states, st);
       case 4: return launch_rpass<4>(a, mode, grid_states, st);
       case 2: return launch_rpass<2>(a, mode, grid_states, st);

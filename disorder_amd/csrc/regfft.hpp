@@ -47,6 +47,33 @@ template <> struct Dft<12> {
   }
 };
 
+template <> struct Dft<9> {
+  // r = 3a + b, k = c + 3d:  W9^{rk} = W3^{ac} W9^{bc} W3^{bd}
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 y[3][3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      float2 t3[3] = {v[b], v[b + 3], v[b + 6]};
+      DftOdd<3>::run(t3);
+      y[b][0] = t3[0];
+      y[b][1] = t3[1];
+      y[b][2] = t3[2];
+    }
+    y[1][1] = wconst(y[1][1], 0.76604444311897803520f, 0.64278760968653932632f);   // W9^1
+    y[1][2] = wconst(y[1][2], 0.17364817766693034885f, 0.98480775301220805936f);   // W9^2
+    y[2][1] = wconst(y[2][1], 0.17364817766693034885f, 0.98480775301220805936f);   // W9^2
+    y[2][2] = wconst(y[2][2], -0.93969262078590838405f, 0.34202014332566873304f);  // W9^4
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float2 t3[3] = {y[0][c], y[1][c], y[2][c]};
+      DftOdd<3>::run(t3);
+      v[c] = t3[0];
+      v[c + 3] = t3[1];
+      v[c + 6] = t3[2];
+    }
+  }
+};
+
 template <> struct Dft<10> {
   // r = 2a + b, k = c + 5d:  W10^{rk} = W5^{ac} W10^{bc} W2^{bd}
   static __device__ __forceinline__ void run(float2* v) {
@@ -151,20 +178,35 @@ __device__ __forceinline__ void ramp16(float2 (&ph)[16], double c, int k1) {
 }
 
 // Phase ramp over the stage-1 input layout of reg_fft<R1,16> (lane t holds
-// element t + 16 r, r < R1): m = t + 16 r (r < R1/2) or t + 16 r - N.
+// element t + 16 r, r < R1): m = t + 16 r below N/2, t + 16 r - N at or
+// above it.  Even R1: rows r < R1/2 lie below N/2.  Odd R1: rows
+// r < (R1-1)/2 lie below, row (R1-1)/2 straddles N/2 = 16 (R1-1)/2 + 8
+// (lanes t >= 8 at or above it), the rest lie above.
 template <int R1>
 __device__ __forceinline__ void ramp_in(float2 (&ph)[16], double c, int t) {
   constexpr int N = R1 * 16;
   constexpr int H = R1 / 2;
-  const float2 a0 = phase_cycles(c, t);
-  const float2 a1 = phase_cycles(c, t - N / 2);
   const float2 w = phase_cycles(c, 16);
-  ph[0] = a0;
-  ph[H] = a1;
+  if constexpr (R1 % 2 == 0) {
+    const float2 a0 = phase_cycles(c, t);
+    const float2 a1 = phase_cycles(c, t - N / 2);
+    ph[0] = a0;
+    ph[H] = a1;
 #pragma unroll
-  for (int k = 1; k < H; ++k) {
-    ph[k] = cmul(ph[k - 1], w);
-    ph[k + H] = cmul(ph[k + H - 1], w);
+    for (int k = 1; k < H; ++k) {
+      ph[k] = cmul(ph[k - 1], w);
+      ph[k + H] = cmul(ph[k + H - 1], w);
+    }
+  } else {
+    const float2 a0 = phase_cycles(c, t);
+    const float2 a1 = phase_cycles(c, t + 16 * (H + 1) - N);
+    ph[0] = a0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) ph[k] = cmul(ph[k - 1], w);
+    ph[H + 1] = a1;
+#pragma unroll
+    for (int k = H + 2; k < R1; ++k) ph[k] = cmul(ph[k - 1], w);
+    if (t >= 8) ph[H] = cmulc(a1, w);  // m = t + 16 H - N
   }
 }
 

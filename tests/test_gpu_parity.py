@@ -29,7 +29,9 @@ def to_np(t):
                                      ((24, 256, 16), None), ((16, 18, 192), None),
                                      # radix-10 and radix-14 register paths (160, 224)
                                      ((160, 20, 16), None), ((20, 224, 12), (40, 224, 24)), ((16, 18, 160), None),
-                                     ((224, 16, 10), None)])
+                                     ((224, 16, 10), None),
+                                     # odd radix-9 register path (144) on every axis
+                                     ((144, 12, 10), None), ((16, 144, 12), (32, 144, 24)), ((12, 16, 144), None)])
 def test_rigid_forward_inverse(dev, n, Nfull):
     import disorder_amd as D
     rng = np.random.default_rng(sum(n))
@@ -113,6 +115,7 @@ def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
                                                     ((32, 24, 16), 3, 4, None, True),
                                                     ((64, 20, 8), 8, 5, 4, True),
                                                     ((192, 16, 6), 2, 4, None, False),
+                                                    ((144, 18, 6), 3, 4, None, True),
                                                     ((256, 12, 4), 10, 4, None, True),
                                                     # states of ~900 and ~1280 samples: several sample
                                                     # chunks per workgroup on the register path
@@ -220,7 +223,8 @@ def test_cg_matches_oracle(dev, tolType, nIt, lam, use_pre):
 
 
 @pytest.mark.parametrize("n,Nfull,flip", [((16, 12, 10), None, False), ((32, 64, 16), (64, 128, 32), False),
-                                          ((15, 12, 14), None, True), ((32, 16, 12), None, True)])
+                                          ((15, 12, 14), None, True), ((32, 16, 12), None, True),
+                                          ((144, 12, 10), None, False)])
 def test_gradient_matches_oracle(dev, n, Nfull, flip):
     """sincRigidTransformGradient: every d(T x)/dT_g against the FP64 oracle."""
     import disorder_amd as D

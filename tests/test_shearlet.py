@@ -109,11 +109,13 @@ def _to_dev(a, dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("grid", [N3, (64, 40, 64), (64, 64, 32 + 32)])
+@pytest.mark.parametrize("grid", [N3, (64, 40, 64), (64, 64, 32 + 32), (144, 36, 48), (48, 36, 144)])
 def test_gpu_weights_regulariser_and_traces_match_oracle(dev, grid):
     """(40,36,35): every axis on the LDS engine, regulariser without the fused
     middle pass; (64,40,64): axes 0 and 2 on the register path, axis 1 on the
-    LDS engine, fused middle pass; (64,64,64): register path throughout."""
+    LDS engine, fused middle pass; (64,64,64): register path throughout;
+    (144,36,48) and (48,36,144): the odd radix-9 register path on the
+    contiguous axis and in the fused middle pass."""
     import torch
     rng = np.random.default_rng(3)
     x = crandn(rng, grid)
