@@ -46,7 +46,33 @@ def set_fft_workers(n):
     _FFT_WORKERS = n
 
 
+_DENSE_DFT = False
+_DFT_CACHE = {}
+
+
+def set_dense_dft(flag):
+    """Route every DFT along an axis through a GEMM with the single-precision
+    matrix single(dftmtx(n)) (inverse conj(F)/n), as Methods/fftGPU.m:28-45
+    and ifftGPU.m:24-48 apply it: the reference's own algorithm, used only to
+    time it on the host.  Results equal the FFT path to single precision."""
+    global _DENSE_DFT
+    _DENSE_DFT = bool(flag)
+
+
+def _dense(x, axis, inverse):
+    n = x.shape[axis]
+    F = _DFT_CACHE.get(n)
+    if F is None:
+        F = _DFT_CACHE[n] = dftmtx(n).astype(np.complex64)
+    M = np.conj(F) if inverse else F
+    y = np.tensordot(M, np.asarray(x, np.complex64), axes=(1, axis))   # cgemm on the host BLAS
+    y = np.moveaxis(y, 0, axis)
+    return y / np.float32(n) if inverse else y
+
+
 def _fft(x, axis):
+    if _DENSE_DFT:
+        return _dense(x, axis, False)
     if _FFT_WORKERS:
         import scipy.fft as sf
         return sf.fft(x, axis=axis, workers=_FFT_WORKERS)
@@ -54,6 +80,8 @@ def _fft(x, axis):
 
 
 def _ifft(x, axis):
+    if _DENSE_DFT:
+        return _dense(x, axis, True)
     if _FFT_WORKERS:
         import scipy.fft as sf
         return sf.ifft(x, axis=axis, workers=_FFT_WORKERS)
@@ -61,6 +89,8 @@ def _ifft(x, axis):
 
 
 def _fft2o(x):
+    if _DENSE_DFT:
+        return _dense(_dense(x, 0, False), 1, False) / np.float32(math.sqrt(x.shape[0] * x.shape[1]))
     if _FFT_WORKERS:
         import scipy.fft as sf
         return sf.fft2(x, axes=(0, 1), norm="ortho", workers=_FFT_WORKERS)
@@ -68,6 +98,8 @@ def _fft2o(x):
 
 
 def _ifft2o(x):
+    if _DENSE_DFT:
+        return _dense(_dense(x, 0, True), 1, True) * np.float32(math.sqrt(x.shape[0] * x.shape[1]))
     if _FFT_WORKERS:
         import scipy.fft as sf
         return sf.ifft2(x, axes=(0, 1), norm="ortho", workers=_FFT_WORKERS)

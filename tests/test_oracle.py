@@ -353,3 +353,26 @@ def test_lm_singular_system_takes_no_step_and_wrap_is_bounded():
     R = O.restrict_transform(T)
     assert np.all(np.abs(R[:, 3:]) <= np.pi)
     assert abs(R[0, 4] - (-7.0 + 2 * np.pi)) < 1e-15 and abs(R[0, 5] - (4.0 - 2 * np.pi)) < 1e-15
+
+
+def test_dense_dft_mode_matches_fft_path():
+    """set_dense_dft routes every DFT through a complex64 GEMM with
+    dftmtx(n) (Methods/fftGPU.m): E^H E equals the FFT path to single
+    precision, and a single axis DFT equals numpy's FFT."""
+    from tests.helpers import crandn, rel_l2
+    rng = np.random.default_rng(4)
+    n, C = (16, 12, 10), 3
+    x, Sm = crandn(rng, n), crandn(rng, n + (C,))
+    H = rng.integers(1, 5, size=n[:2])
+    k1, k2, off, _ = O.build_harmonic_sampling(H, 4)
+    samp = O.Sampling(k1, k2, off[:5], n[0], n[1])
+    T = np.array([[1.2, -0.7, 0.4, 0.03, -0.05, 0.02]] * 4)
+    a = O.encodedecode(x, Sm, samp, T, n)
+    O.set_dense_dft(True)
+    try:
+        b = O.encodedecode(x, Sm, samp, T, n)
+        f = O.fft_dim(x, 1)
+    finally:
+        O.set_dense_dft(False)
+    assert rel_l2(b, a) < 2e-6
+    assert rel_l2(f, np.fft.fft(x, axis=1)) < 1e-6
