@@ -29,6 +29,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pyramid", action="store_true",
+                    help="E^H E line without the time-to-NRMSE summary of the joint pyramid (plain and learned)")
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched", "cs"], default="ehe",
                     help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid; "
@@ -504,6 +506,24 @@ def main():
     cp = stream_copy_rate(dev)
     line["stream_copy"] = cp
     line["pass_model_v1"]["frac_of_stream_copy"] = line["pass_model_v1"]["GBps"] / cp["GBps"]
+    E.close()
+    del E, x, Sm, out
+    torch.cuda.empty_cache()
+    if not args.no_pyramid:
+        # the metric's second half: time to 1e-3 NRMSE of the joint pyramid, plain and learned
+        # (bench.py --workload pyramid gives the full record)
+        try:
+            pl = bench_pyramid(args)
+            lr = pl.get("learned", {})
+            line["time_to_1e-3_nrmse"] = {
+                "plain_s": pl["value"], "learned_s": lr.get("time_to_1e-3_own_final_s"),
+                "learned_vs_plain_time_ratio": pl.get("learned_vs_plain_time_ratio"),
+                "plain_final_nrmse_vs_gt": pl["plain"]["final_nrmse_vs_gt"],
+                "learned_nrmse_gt_minus_plain": pl.get("learned_nrmse_gt_minus_plain"),
+                "learned_proposals": lr.get("proposals"),
+                "workload": pl["config"]["workload"], "unit": "s", "higher_is_better": False}
+        except Exception as exc:  # reported, not fatal to the E^H E line
+            line["time_to_1e-3_nrmse"] = {"error": repr(exc)}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, C, S, seed=5)
     print(json.dumps(line), flush=True)
