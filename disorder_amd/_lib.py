@@ -63,6 +63,8 @@ SIGNATURES = {
     "dis_shearlet_regularize": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_double, c_void_p]),
     "dis_shearlet_energy": (c_int, [c_void_p, c_void_p, c_void_p, c_double_p]),
     "dis_fidelity_trace": (c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_float_p, c_void_p, c_double_p]),
+    "dis_fidelity_trace_folded": (c_int, [c_void_p, c_int32_p, c_void_p, c_int32, c_int32, c_int32, c_float_p, c_void_p,
+                                   c_double_p]),
     "dis_cg_cs": (c_int, [c_void_p, c_double_p, c_void_p, c_float_p, c_float_p, ctypes.c_double, c_void_p,
                           ctypes.c_double, c_void_p, c_int32, c_int32, c_int32, ctypes.c_double, c_int32_p]),
     "dis_autotune_clear": (c_int, [c_void_p]),

@@ -756,6 +756,15 @@ int dis_shearlet_energy(dis_ctx* c, dis_shearlet* sh, const void* x, double* out
   return shearlet_energy(c, sh, (const float2*)x, out);
 }
 
+int dis_fidelity_trace_folded(dis_ctx* c, const int32_t n[3], const void* S, int32_t C, int32_t ny1, int32_t ny2,
+                              const float* Wgrid, const void* x, double* out) {
+  if (!c) return DIS_E_INVALID;
+  if (!n || !S || !Wgrid || !x || !out || C <= 0 || C > 4096 || n[0] <= 0 || n[1] <= 0 || n[2] <= 0)
+    return ctx_fail(c, DIS_E_INVALID, "dis_fidelity_trace_folded: invalid argument");
+  if (ctx_set_device(c) != hipSuccess) return ctx_fail(c, DIS_E_HIP, "hipSetDevice failed");
+  return fold_fidelity_trace(c, n, (const float2*)S, C, ny1, ny2, Wgrid, (const float2*)x, out);
+}
+
 int dis_fidelity_trace(dis_ctx* c, dis_shearlet* sh, const void* S, int32_t C, const float* Wgrid, const void* x,
                        double* out) {
   if (!c) return DIS_E_INVALID;

@@ -21,6 +21,7 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "transform.hpp"
 
 namespace dis {
 namespace {
@@ -213,7 +214,109 @@ __global__ __launch_bounds__(kThreads) void rows_adj_kernel(CoilArgs a, FoldGeom
 
 int tile_pencils(int n) { return std::max(1, kMaxTile / n); }
 
+// Strided FFT along dim 2 (length n2 = pl.n) of Z [L x n2 x H] (L = ny1
+// contiguous positions), P positions per tile, then the weighted energy
+// sum W[k1 + L k2] |X|^2 of the tile into part[blockIdx.x] (FP64, fixed order).
+__global__ __launch_bounds__(kThreads) void fft2_energy_kernel(const float2* __restrict__ z, long long L, long long H,
+                                                                int P, long long tiles_lo, FftPlan pl,
+                                                                const float2* __restrict__ gtw,
+                                                                const float* __restrict__ W, double* __restrict__ part) {
+  __shared__ float2 buf[kMaxTile];
+  __shared__ float2 tw[1024];
+  __shared__ double red[kThreads];
+  const int tid = threadIdx.x;
+  const int n = pl.n;
+  for (int k = tid; k < n; k += kThreads) tw[k] = gtw[k];
+  const long long hi0 = blockIdx.x / tiles_lo;
+  const long long lo0 = (blockIdx.x - hi0 * tiles_lo) * (long long)P;
+  const int Pv = (int)(L - lo0 < P ? L - lo0 : P);
+  const int tot = P * n;
+  for (int e = tid; e < tot; e += kThreads) {
+    const int j = e / P, p = e - j * P;
+    buf[e] = p < Pv ? z[(lo0 + p) + L * ((long long)j + (long long)n * hi0)] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_tile<false>(buf, tw, pl, P, tid);
+  double acc = 0.0;
+  for (int e = tid; e < tot; e += kThreads) {
+    const int j = e / P, p = e - j * P;
+    if (p >= Pv) continue;
+    const float2 v = buf[e];
+    acc += (double)W[(lo0 + p) + L * j] * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int k = kThreads / 2; k > 0; k >>= 1) {
+    if (tid < k) red[tid] += red[tid + k];
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void sum_partials_kernel(const double* __restrict__ part, long long n, double* __restrict__ out) {
+  __shared__ double red[kThreads];
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += kThreads) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = kThreads / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
 }  // namespace
+
+// traceEstimation.m:61-79 with SENSE folding: sum over coils of
+// sum_k W(k1, k2) |F2(fold(S_c x)) / sqrt(ny1 ny2)|^2, the readout in image
+// space.  Scratch from context buffer slot 8.
+int fold_fidelity_trace(dis_ctx* c, const int32_t n[3], const float2* S, int C, int ny1, int ny2, const float* W,
+                        const float2* x, double* out) {
+  FftPlan p1, p2;
+  if (ny1 <= 0 || ny2 <= 0 || ny1 > n[0] || ny2 > n[1] || ny1 > 1024 || ny2 > 1024 || make_plan(ny1, &p1) ||
+      make_plan(ny2, &p2))
+    return ctx_fail(c, DIS_E_UNSUPPORTED, "dis_fidelity_trace_folded: unsupported acquired grid");
+  const float2 *tw1 = nullptr, *tw2 = nullptr;
+  int rc = ctx_twiddles(c, ny1, &tw1);
+  if (!rc) rc = ctx_twiddles(c, ny2, &tw2);
+  if (rc) return rc;
+  FoldGeom f;
+  f.n1 = n[0];
+  f.n2 = n[1];
+  f.n3 = n[2];
+  f.C = C;
+  f.ny1 = ny1;
+  f.ny2 = ny2;
+  f.o1 = (n[0] - ny1 + 1) / 2;
+  f.o2 = (n[1] - ny2 + 1) / 2;
+  const long long nz = (long long)ny1 * ny2 * n[2] * C;
+  const long long L = ny1, H = (long long)n[2] * C;
+  const int P2 = (int)std::min<long long>(tile_pencils(ny2), L);
+  const long long tiles_lo = (L + P2 - 1) / P2;
+  const long long ntiles = tiles_lo * H;
+  void* buf = nullptr;
+  rc = ctx_buffer(c, 8, sizeof(float2) * nz + sizeof(double) * (ntiles + 1) + 256, &buf);
+  if (rc) return rc;
+  float2* Z = (float2*)buf;
+  double* part = (double*)((char*)buf + ((sizeof(float2) * nz + 255) & ~size_t(255)));
+  double* tot = part + ntiles;
+  hipStream_t st = ctx_stream(c);
+  hipLaunchKernelGGL(fold_mul_kernel, dim3(blocks_for(nz)), dim3(kThreads), 0, st, x, S, f, Z);
+  const long long Hy = (long long)ny2 * n[2] * C;
+  const int Py = tile_pencils(ny1);
+  hipLaunchKernelGGL(fft1_kernel<false>, dim3((unsigned)((Hy + Py - 1) / Py)), dim3(kThreads), 0, st, Z, ny1, Hy, Py,
+                     p1, tw1);
+  hipLaunchKernelGGL(fft2_energy_kernel, dim3((unsigned)ntiles), dim3(kThreads), 0, st, Z, L, H, P2, tiles_lo, p2,
+                     tw2, W, part);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kThreads), 0, st, part, ntiles, tot);
+  double h = 0.0;
+  if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&h, tot, sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return ctx_fail(c, DIS_E_HIP, "dis_fidelity_trace_folded failed");
+  *out = h / ((double)ny1 * (double)ny2);
+  return DIS_OK;
+}
 
 size_t coil_fold_work_bytes(int n1, int n2, int n3, int C, int ny1, int ny2, int maxnec) {
   const size_t x = (size_t)n1 * n2 * n3;

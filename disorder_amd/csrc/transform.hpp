@@ -48,6 +48,10 @@ struct OpView {
 // without folding): plan and twiddles of the acquired grid, scratch for
 // states of up to maxnec samples (context buffer slot 6).
 int fold_setup(dis_ctx* c, const OpView& v, int maxnec, CoilArgs* a);
+// Fidelity trace sample of traceEstimation.m:61-79 with SENSE folding onto
+// the acquired ny1 x ny2 grid (weights W [ny1 x ny2] on that grid).
+int fold_fidelity_trace(dis_ctx* c, const int32_t n[3], const float2* S, int C, int ny1, int ny2, const float* W,
+                        const float2* x, double* out);
 
 struct EnergyOut {
   const float* Mbe = nullptr;  // host [n3] or null

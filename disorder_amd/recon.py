@@ -34,9 +34,6 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
         raise ValueError("y must be [NY1, NY2, N3, C] with NY1 <= N1, NY2 <= N2 (the maps' grid)")
     NY = (int(y.shape[0]), int(y.shape[1]))
     folded = NY != N[:2]
-    if folded and cs:
-        raise NotImplementedError("the shearlet-regularised image with SENSE folding (its fidelity trace "
-                                  "would need folding too, traceEstimation.m:66-70) is not built")
     # coil compression (solveXT.m:61): maps and data on the first max(eivaS) components
     if perc is not None:
         S, y, eivaS, _, _ = ops.compressCoils(S, list(perc), y, 3, ctx)
@@ -68,12 +65,12 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
         sm = info["sampling"]
         yf = ops.gather_samples(yc, sm["rows"], 1.0, ctx)
         if gibbs:
-            ops.filter_rows(yf, sm["k1"], sm["k2"], N, gibbs, ctx)
+            ops.filter_rows(yf, sm["k1"], sm["k2"], (NY[0], NY[1], N[2]), gibbs, ctx)
         sweep_of_sample = info["sweep_of_state"][np.repeat(np.arange(len(sm["off"]) - 1), np.diff(sm["off"]))]
         We = None if "We" not in info else np.asarray(info["We"])[sweep_of_sample]
         c0 = eivaS[0]
         Sf = S[..., :c0]
-        E = ops.Encoding(Sf, sm["k1"], sm["k2"], sm["off"], We=We, ctx=ctx)
+        E = ops.Encoding(Sf, sm["k1"], sm["k2"], sm["off"], We=We, ctx=ctx, NY=NY)
         out["x_cs"], out["lambdas"] = SH.CSsolver(yf[..., :c0], E, T, x, Sf, sm["k1"], sm["k2"], We)
         E.close()
     if write is not None:

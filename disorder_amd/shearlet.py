@@ -211,6 +211,17 @@ class ShearletSystem:
         return r.value
 
 
+def fidelity_folded(S, Wgrid, x, NY, ctx):
+    """One fidelity trace sample with SENSE folding onto the acquired grid NY
+    (traceEstimation.m:61-79 with E.Uf): dis_fidelity_trace_folded."""
+    n = (ctypes.c_int32 * 3)(*[int(v) for v in x.shape[:3]])
+    r = ctypes.c_double(0.0)
+    ctx._prep()
+    ops.check(ctx.lib.dis_fidelity_trace_folded(ctx.h, n, ops._ptr(S), int(S.shape[3]), int(NY[0]), int(NY[1]),
+                                                ops._fptr(Wgrid), ops._ptr(x), ctypes.byref(r)), ctx.h)
+    return r.value
+
+
 def grid_weights(k1, k2, We, n12):
     """EH.We on the k-space grid (traceEstimation.m:43-48): the weight of the
     sample at each sampled (k1, k2), 1 elsewhere."""
@@ -248,10 +259,14 @@ def CSsolver(y, E: ops.Encoding, T, x, S, k1, k2, We=None, nOu=2, p=1.0, epsil=1
     rng = np.random.default_rng(seed)
     n = tuple(int(v) for v in x.shape)
     sh = system or ShearletSystem(n, E.ctx)
-    Wgrid = torch.from_numpy(grid_weights(k1, k2, We, n[:2]).reshape(-1, order="F")).to(x.device)
+    NY = tuple(getattr(E, "NY", n[:2]))  # acquired phase-encode grid (SENSE folding below n[:2])
+    Wgrid = torch.from_numpy(grid_weights(k1, k2, We, NY).reshape(-1, order="F")).to(x.device)
     x = ops.colmajor(x.clone())
     sh.weights(x, epsil, p)
-    tr_fid, nf = trace_estimation(lambda z: sh.fidelity(S, Wgrid, z), n, rng, x.device, tol)
+    if NY != n[:2]:
+        tr_fid, nf = trace_estimation(lambda z: fidelity_folded(S, Wgrid, z, NY, E.ctx), n, rng, x.device, tol)
+    else:
+        tr_fid, nf = trace_estimation(lambda z: sh.fidelity(S, Wgrid, z), n, rng, x.device, tol)
     tr_reg, nr = trace_estimation(sh.energy, n, rng, x.device, tol)
     la = (2.0 / p) * tr_fid / (tr_reg + np.finfo(np.float64).eps)
     lambdas = [la]

@@ -315,6 +315,12 @@ int dis_shearlet_energy(dis_ctx* ctx, dis_shearlet* sh, const void* x, double* o
  * [n1 x n2], x DEVICE complex64 volume; uses sh for its work buffers. */
 int dis_fidelity_trace(dis_ctx* ctx, dis_shearlet* sh, const void* S, int32_t C, const float* Wgrid, const void* x,
                        double* out);
+/* The same fidelity trace sample with SENSE folding (traceEstimation.m:61-79
+ * with E.Uf): sum_c sum_k Wgrid(k1,k2) |F2(fold(S_c x))/sqrt(ny1 ny2)|^2 on the
+ * acquired ny1 x ny2 grid (Wgrid: DEVICE float [ny1 x ny2]); S [n1,n2,n3,C]
+ * and x [n1,n2,n3] on the device; out: host double. */
+int dis_fidelity_trace_folded(dis_ctx* ctx, const int32_t n[3], const void* S, int32_t C, int32_t ny1, int32_t ny2,
+                              const float* Wgrid, const void* x, double* out);
 /* Reconstruction/CGsolver.m:1 with the shearlet term (CSsolver.m:76):
  * dis_cg with R x = lambda x + (la_sh p / 2) R_sh x.  tolType 'Energy' is
  * not supported with the shearlet term. */

@@ -280,13 +280,18 @@ def trace_samples_regularizer(probes, S, We):
 
 def trace_samples_fidelity(probes, Smaps, Wgrid):
     """Per probe: sum_c sum_k Wgrid(k1, k2) |F2(S_c x) / sqrt(N1 N2)|^2
-    (traceEstimation.m:61-79 with EH.We on the k-space grid)."""
-    N1, N2 = Smaps.shape[:2]
+    (traceEstimation.m:61-79 with EH.We on the k-space grid).  A weight grid
+    smaller than the maps' grid is the acquired grid of SENSE folding
+    (traceEstimation.m:72-76): S_c x is folded onto it (fold.m) first."""
+    from oracle.disorder_oracle import fold
+    NX1, NX2 = Smaps.shape[:2]
+    NY1, NY2 = Wgrid.shape
     out = []
     for x in probes:
         t = 0.0
         for c in range(Smaps.shape[3]):
-            F = np.fft.fft2(Smaps[..., c] * x, axes=(0, 1)) / math.sqrt(N1 * N2)
+            v = fold(fold(Smaps[..., c] * x, 0, NX1, NY1), 1, NX2, NY2)
+            F = np.fft.fft2(v, axes=(0, 1)) / math.sqrt(NY1 * NY2)
             t += float(np.sum(Wgrid[:, :, None] * np.abs(F) ** 2))
         out.append(t)
     return np.array(out)

@@ -215,14 +215,15 @@ def test_folded_driver_matches_oracle(dev):
 
 @pytest.mark.gpu
 def test_folded_end_to_end_matches_driver(dev):
-    """aligned_reconstruction on data acquired on a 32x24 grid with 40x30
+    """aligned_reconstruction on data acquired on a 32x24 grid with 40x36
     maps: the same image and motion as the driver called with NY on the
-    compact samples (the entry point only gathers and forwards)."""
+    compact samples (the entry point only gathers and forwards); with cs the
+    shearlet-regularised image stays close to the unregularised one."""
     from disorder_amd import pyramid as PY
     from disorder_amd import recon
     from disorder_amd import trajectory as TR
     from tests.test_pyramid import _coil_maps_np
-    N, NY, S, C = (40, 30, 16), (32, 24), 16, 3
+    N, NY, S, C = (40, 36, 35), (32, 24), 16, 3  # every shearlet plane >= 33
     rng = np.random.default_rng(9)
     g1, g2 = np.meshgrid(np.arange(NY[0]) - NY[0] // 2, np.arange(NY[1]) - NY[1] // 2, indexing="ij")
     k = np.stack([g1.ravel(order="F"), g2.ravel(order="F")], 1)
@@ -239,5 +240,44 @@ def test_folded_end_to_end_matches_driver(dev):
     xg, Tg, _ = PY.solveXT(_to_dev(y, dev), _to_dev(Sm, dev), k1, k2, off, N, traj=st, NY=NY, **kw)
     assert np.array_equal(out["T"], Tg)
     assert rel_l2(out["x"].cpu().numpy(), xg.cpu().numpy()) < 1e-6
-    with pytest.raises(NotImplementedError):
-        recon.aligned_reconstruction(_to_dev(Y, dev), _to_dev(Sm, dev), ko, kRange, NP // S, perc=None, cs=True, **kw)
+    # the shearlet-regularised image on folded data (folded fidelity trace)
+    o2 = recon.aligned_reconstruction(_to_dev(Y, dev), _to_dev(Sm, dev), ko, kRange, NP // S, perc=None, cs=True, **kw)
+    xc = o2["x_cs"].cpu().numpy()
+    assert xc.shape == N and np.isfinite(xc).all() and len(o2["lambdas"]) == 2
+    assert rel_l2(xc, out["x"].cpu().numpy()) < 0.5
+
+
+def test_oracle_folded_fidelity_sample():
+    """traceEstimation's fidelity sample with folding: unit weights give
+    sum_c ||fold(S_c x)||^2 (Parseval on the acquired grid); NY = NX is the
+    unfolded formula."""
+    from oracle import shearlet_oracle as SO
+    rng = np.random.default_rng(6)
+    n, NY = (20, 15, 6), (16, 12)
+    Sm = crandn(rng, n + (3,))
+    z = rng.normal(size=n)
+    got = SO.trace_samples_fidelity([z], Sm, np.ones(NY))[0]
+    ref = sum(float(np.sum(np.abs(O.fold(O.fold(Sm[..., c] * z, 0, n[0], NY[0]), 1, n[1], NY[1])) ** 2))
+              for c in range(3))
+    assert abs(got / ref - 1) < 1e-12
+    W = rng.uniform(0.2, 1.0, n[:2])
+    un = sum(float(np.sum(W[:, :, None] * np.abs(np.fft.fft2(Sm[..., c] * z, axes=(0, 1))) ** 2)) / (n[0] * n[1])
+             for c in range(3))
+    assert abs(SO.trace_samples_fidelity([z], Sm, W)[0] / un - 1) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,ny", [((40, 36, 12), (32, 30)), ((64, 48, 16), (32, 24))])
+def test_folded_fidelity_trace_matches_oracle(dev, n, ny):
+    import torch
+    from disorder_amd import shearlet as SH
+    from oracle import shearlet_oracle as SO
+    import disorder_amd as D
+    rng = np.random.default_rng(sum(n))
+    Sm = crandn(rng, n + (3,)).astype(np.complex64).astype(np.complex128)
+    z = rng.normal(size=n).astype(np.float32).astype(np.float64)
+    W = rng.uniform(0.2, 1.0, ny).astype(np.float32)
+    Wd = torch.from_numpy(W.reshape(-1, order="F")).to(dev)
+    got = SH.fidelity_folded(_to_dev(Sm, dev), Wd, _to_dev(z, dev), ny, D.default_context(0))
+    ref = SO.trace_samples_fidelity([z], Sm, W.astype(np.float64))[0]
+    assert abs(got / ref - 1) < 2e-5, (got, ref)

@@ -144,7 +144,10 @@ def test_gpu_weights_regulariser_and_traces_match_oracle(dev, grid):
 
 
 @pytest.mark.gpu
-def test_cssolver_matches_oracle(dev):
+@pytest.mark.parametrize("NY", [None, (32, 28)])
+def test_cssolver_matches_oracle(dev, NY):
+    """CSsolver on the GPU against the oracle; NY: data acquired on a smaller
+    phase-encode grid (SENSE folding in the operators and the fidelity trace)."""
     from disorder_amd import ops
     from disorder_amd import sampling as SA
     from oracle import disorder_oracle as O
@@ -152,23 +155,24 @@ def test_cssolver_matches_oracle(dev):
     from tests.test_pyramid import _coil_maps_np
     rng = np.random.default_rng(4)
     N, S, C = N3, 16, 4
+    G = N[:2] if NY is None else NY
     x_gt = smooth_volume(rng, N, nblob=8)
     Sm = _coil_maps_np(N, C, 7)
-    H = checkered_sampling(rng, N[0], N[1], S, 4, 4)
+    H = checkered_sampling(rng, G[0], G[1], S, 4, 4)
     k1, k2, off = SA.harmonic_sampling(H, S)
     T = np.zeros((S, 6))
     T[:, :3] = rng.uniform(-0.5, 0.5, (S, 3))
     We = rng.uniform(0.5, 1.0, S)[np.repeat(np.arange(S), np.diff(off))]
-    samp = O.Sampling(k1, k2, off, N[0], N[1], We)
+    samp = O.Sampling(k1, k2, off, G[0], G[1], We)
     y = O.encode(x_gt, Sm, samp, T, N)
     y = y + 0.05 * np.sqrt(np.mean(np.abs(y) ** 2)) * (rng.normal(size=y.shape) + 1j * rng.normal(size=y.shape))
     y = y.astype(np.complex64).astype(np.complex128)
     Sm32 = Sm.astype(np.complex64).astype(np.complex128)
     x0, _, _ = O.cg_solver(y, Sm32, samp, T, N, nIt=6, tolType="None", tol=0)
     x0 = x0.astype(np.complex64).astype(np.complex128)
-    Wg = SH.grid_weights(k1, k2, We, N[:2]).astype(np.float64)
+    Wg = SH.grid_weights(k1, k2, We, G).astype(np.float64)
     x_ref, la_ref = SO.cs_solver(y, Sm32, samp, T, N, x0, Wg, seed=9)
-    E = ops.Encoding(_to_dev(Sm, dev), k1, k2, off, We=We)
+    E = ops.Encoding(_to_dev(Sm, dev), k1, k2, off, We=We, NY=NY)
     xg, la_g = SH.CSsolver(_to_dev(y, dev), E, T, _to_dev(x0, dev), _to_dev(Sm, dev), k1, k2, We, seed=9)
     E.close()
     assert len(la_g) == len(la_ref) == 2
