@@ -192,7 +192,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     that grid."""
     import torch
     from . import trajectory as TJ
-    from .sampling import harmonic_sampling
+    from .sampling import harmonic_sampling_repeats
     ctx = ctx or ops.default_context(y.device.index or 0)
     nsweeps = len(off) - 1
     T = np.zeros((nsweeps, 6)) if T0 is None else np.array(T0, np.float64, copy=True)
@@ -211,12 +211,15 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     sweep_of_state = np.arange(nsweeps)
     k1c, k2c, offc, rows0 = k1, k2, np.asarray(off), None
     if traj is not None:
-        kk1, kk2, koff = harmonic_sampling(traj["H"], traj["sweeps"])
+        H0 = np.asarray(traj["H"])
+        H0 = H0[..., None] if H0.ndim == 2 else H0  # [NY1, NY2, repeats]
+        kk1, kk2, krep, koff = harmonic_sampling_repeats(H0, traj["sweeps"])
         if not (np.array_equal(kk1, k1) and np.array_equal(kk2, k2) and np.array_equal(koff, off)):
             raise ValueError("traj does not describe the sampling k1, k2, off")
         state_of_profile = np.array(traj["sweep_of_profile"], np.int64)
-        pos_row = np.full(traj["H"].shape, -1, np.int64)
-        pos_row[k1, k2] = np.arange(len(k1))
+        pos_row = np.full(H0.shape, -1, np.int64)
+        pos_row[k1, k2, krep] = np.arange(len(k1))
+        Ti = np.asarray(traj["time_index"]).reshape(H0.shape)
     subT = np.zeros(nsweeps, bool)
     subTT = np.zeros(nsweeps, bool)
     last_We = None
@@ -235,12 +238,11 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             subTT = np.repeat(subT, np.where(subT, 2, 1))
             sweep_of_state = np.repeat(sweep_of_state, np.where(subT, 2, 1))
             state_of_profile, _ = TJ.subdivide_states(state_of_profile, subT)
-            Ti = traj["time_index"]
             H = np.where(Ti > 0, state_of_profile[np.maximum(Ti, 1) - 1], 0)
-            k1c, k2c, offc = harmonic_sampling(H, len(sweep_of_state))
+            k1c, k2c, repc, offc = harmonic_sampling_repeats(H, len(sweep_of_state))
             if (np.diff(offc) == 0).any():
                 raise ValueError("a split left a state without samples")
-            rows0 = pos_row[k1c, k2c]
+            rows0 = pos_row[k1c, k2c, repc]
         subT = np.zeros(len(sweep_of_state), bool)
         nstates = len(sweep_of_state)
         ny = None if NY is None else level_size(NY[:2], max(0, nlevels - 1 - li))

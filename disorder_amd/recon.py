@@ -30,10 +30,16 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
     import torch
     ctx = ctx or ops.default_context(y.device.index or 0)
     N = tuple(int(v) for v in S.shape[:3])
-    if y.dim() != 4 or y.shape[0] > N[0] or y.shape[1] > N[1] or y.shape[2] != N[2]:
-        raise ValueError("y must be [NY1, NY2, N3, C] with NY1 <= N1, NY2 <= N2 (the maps' grid)")
+    if y.dim() not in (4, 5) or y.shape[0] > N[0] or y.shape[1] > N[1] or y.shape[2] != N[2]:
+        raise ValueError("y must be [NY1, NY2, N3, C] or [NY1, NY2, N3, C, R] with NY1 <= N1, NY2 <= N2 "
+                         "(the maps' grid)")
     NY = (int(y.shape[0]), int(y.shape[1]))
     folded = NY != N[:2]
+    R = int(y.shape[4]) if y.dim() == 5 else 1
+    if y.dim() == 5:  # repeats next to the phase encodes, coils last: [NY1, NY2, R, N3, C]
+        yp = ops.empty_colmajor((NY[0], NY[1], R, N[2], int(y.shape[3])), y.device)
+        yp.copy_(y.permute(0, 1, 4, 2, 3))
+        y = yp
     # coil compression (solveXT.m:61): maps and data on the first max(eivaS) components
     if perc is not None:
         S, y, eivaS, _, _ = ops.compressCoils(S, list(perc), y, 3, ctx)
@@ -42,12 +48,13 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
     else:
         eivaS = [int(S.shape[3])] * 3
     # trajectory -> states -> harmonic sampling (solveXT.m:75-141, :349-352)
-    k1, k2, off, st = TJ.sampling_from_trajectory(kTraj, kRange, n_echoes)
-    if tuple(st["H"].shape) != NY:
+    k1, k2, off, st = TJ.sampling_from_trajectory(kTraj, kRange, n_echoes, n_repeats=R)
+    if tuple(st["H"].shape[:2]) != NY:
         raise ValueError("kRange does not span the acquired phase-encode grid of y")
-    NY12 = NY[0] * NY[1]
-    flat = y.as_strided((NY12, y.shape[2], y.shape[3]), (1, NY12, NY12 * y.shape[2]))
-    rows = np.asarray(k1) + NY[0] * np.asarray(k2)
+    NYR = NY[0] * NY[1] * R
+    C = int(y.shape[-1])
+    flat = y.as_strided((NYR, N[2], C), (1, NYR, NYR * N[2]))
+    rows = np.asarray(k1) + NY[0] * np.asarray(k2) + NY[0] * NY[1] * np.asarray(st["sample_repeat"], np.int64)
     yc = ops.gather_samples(flat, rows, 1.0, ctx)
     out = {"eivaS": eivaS}
     if uncorrected_cg:

@@ -64,6 +64,25 @@ def harmonic_sampling(H: np.ndarray, nstates: int, with_outer: bool = False):
     return k1, k2, off.astype(np.int32)
 
 
+def harmonic_sampling_repeats(H: np.ndarray, nstates: int):
+    """buildHarmonicSampling over a state map with repeats, H [n1, n2] or
+    [n1, n2, R] (positions taken in MATLAB linear order over all three
+    dimensions): (k1, k2, repeat, off), repeat the 0-based repeat of each
+    sample."""
+    H = np.asarray(H)
+    n1, n2 = H.shape[:2]
+    Hl = H.reshape(-1, order="F")
+    order = np.argsort(np.where(Hl == 0, nstates + 1, Hl), kind="stable")
+    counts = np.bincount(Hl, minlength=nstates + 1)
+    order = order[: len(order) - counts[0]]
+    off = np.zeros(nstates + 1, np.int64)
+    off[1:] = np.cumsum(counts[1:nstates + 1])
+    k1 = (order % n1).astype(np.int32)
+    k2 = ((order // n1) % n2).astype(np.int32)
+    rep = (order // (n1 * n2)).astype(np.int32)
+    return k1, k2, rep, off.astype(np.int32)
+
+
 def signed_freq(k, n):
     """Signed frequency of FFT-order index k of a length-n transform (the
     Nyquist bin of an even length is negative)."""

@@ -208,16 +208,18 @@ def _dct2_ortho(x):
     return dct(np.asarray(x, np.float64), type=2, norm="ortho", axis=0)
 
 
-def trajectory_states(kTraj, kRange, n_echoes, group_sweeps=1):
-    """solveXT.m:75-141, :349-350 for one repeat.
+def trajectory_states(kTraj, kRange, n_echoes, group_sweeps=1, n_repeats=1):
+    """solveXT.m:75-141, :349-350.
 
     kTraj [NProfs x 2] phase-encode coordinates in acquisition order; kRange
     [[lo1, hi1], [lo2, hi2]] the k-space extent; n_echoes the echo train
-    length (1: the whole scan is one train).  Returns a dict with shots,
-    echoes, sweeps, steady_state, sweep_of_profile [NProfs] (1-based),
-    time_index [NY1 x NY2] (1-based profile, 0 unsampled, DC first) and
-    H [NY1 x NY2] (state 1..NSweeps of each position, 0 unsampled, DC
-    first)."""
+    length (1: the whole scan is one train); n_repeats the averages of the
+    data (NY(5)): with fewer shots than repeats the trajectory is repeated
+    (:85-90) and each profile takes the first repeat slot of its position
+    still free (:132-140).  Returns a dict with shots, echoes, sweeps,
+    steady_state, repeats, sweep_of_profile [NProfs] (1-based), time_index
+    and H, [NY1 x NY2] for one repeat or [NY1 x NY2 x R] (1-based profile /
+    state 1..NSweeps of each position, 0 unsampled, DC first)."""
     kTraj = np.asarray(kTraj, np.int64)
     NProfs = kTraj.shape[0]
     if NProfs <= 1:
@@ -226,13 +228,23 @@ def trajectory_states(kTraj, kRange, n_echoes, group_sweeps=1):
     if NProfs % NEchos:
         raise ValueError(f"{NProfs} profiles are not a whole number of {NEchos}-echo shots (interrupted scan?)")
     NShots = NProfs // NEchos
+    R = int(n_repeats)
+    if R < 1:
+        raise ValueError("n_repeats must be positive")
+    if NShots < R:  # :85-90
+        if NShots > 1:  # the repeated list would hold R x NProfs profiles for R x NEchos sweep labels
+            raise ValueError(f"{NShots} shots for {R} repeats: fewer shots than repeats needs a single-shot scan")
+        NShots = R
+        NProfs = NShots * NEchos
+        kTraj = np.tile(kTraj, (R, 1))
+        NEchos = NProfs // NShots
     kRange = np.asarray(kRange, np.int64)
     kShift = (kRange[:, 1] - kRange[:, 0] + 1) // 2 + 1
     kSize = kRange[:, 1] - kRange[:, 0] + 1
     kIndex = kTraj + kShift[None, :]  # 1-based
     if (kIndex < 1).any() or (kIndex > kSize[None, :]).any():
         raise ValueError("trajectory outside kRange")
-    steady = NShots == 1
+    steady = NShots == R  # :112-116, one shot per repeat
     if steady:
         A = np.abs(_dct2_ortho(kTraj))
         iAs = np.argmax(A[:NProfs // 2], axis=0) + 1
@@ -244,15 +256,18 @@ def trajectory_states(kTraj, kRange, n_echoes, group_sweeps=1):
     else:
         NSweeps = NShots
     sweep = np.ceil(NSweeps * ((np.arange(1, NProfs + 1) - 0.5) / NProfs)).astype(np.int64)
-    T = np.zeros(tuple(int(v) for v in kSize), np.int64)
+    T = np.zeros(tuple(int(v) for v in kSize) + (R,), np.int64)
     for n in range(NProfs):
         i, j = kIndex[n] - 1
-        if T[i, j] != 0:
-            raise ValueError("a k-space position is acquired twice: repeats are not supported")
-        T[i, j] = n + 1
-    T = np.fft.ifftshift(T)
+        free = np.flatnonzero(T[i, j] == 0)
+        if len(free) == 0:
+            raise ValueError(f"a k-space position is acquired more than {R} time(s)")
+        T[i, j, free[0]] = n + 1
+    T = np.fft.ifftshift(T, axes=(0, 1))
     H = np.where(T > 0, sweep[np.maximum(T, 1) - 1], 0)
-    return {"shots": NShots, "echoes": NEchos, "sweeps": NSweeps, "steady_state": steady,
+    if R == 1:
+        T, H = T[..., 0], H[..., 0]
+    return {"shots": NShots, "echoes": NEchos, "sweeps": NSweeps, "steady_state": steady, "repeats": R,
             "sweep_of_profile": sweep, "time_index": T, "H": H}
 
 
@@ -278,12 +293,14 @@ def subdivide_states(state_of_profile, split):
     return st, new_of_old
 
 
-def sampling_from_trajectory(kTraj, kRange, n_echoes, group_sweeps=1):
+def sampling_from_trajectory(kTraj, kRange, n_echoes, group_sweeps=1, n_repeats=1):
     """The harmonic sampling of an acquired trajectory: trajectory_states,
     then Methods/buildHarmonicSampling.m on its state map.  Returns (k1, k2,
     off, states) with k1/k2 the 0-based FFT-order indices of the sampled
-    lines grouped by state and states the dict of trajectory_states."""
-    from .sampling import harmonic_sampling
-    st = trajectory_states(kTraj, kRange, n_echoes, group_sweeps)
-    k1, k2, off = harmonic_sampling(st["H"], st["sweeps"])
+    lines grouped by state and states the dict of trajectory_states, whose
+    'sample_repeat' gives the repeat (0-based) of every sample."""
+    from .sampling import harmonic_sampling_repeats
+    st = trajectory_states(kTraj, kRange, n_echoes, group_sweeps, n_repeats)
+    k1, k2, rep, off = harmonic_sampling_repeats(st["H"], st["sweeps"])
+    st["sample_repeat"] = rep
     return k1, k2, off, st

@@ -470,21 +470,25 @@ class Sampling:
 def build_harmonic_sampling(H, nstates):
     """Methods/buildHarmonicSampling.m:36-49 (integer part).
 
-    H: (n1, n2) int array; H[k1,k2] = state (1..nstates) that sampled that
-    phase-encode position, 0 if unsampled.  For s = 1..nstates+1 the samples
-    of state mod(s, nstates+1) are taken in MATLAB linear (column-major)
-    order, so the last group is the unsampled positions (:37).
-    Returns (k1, k2, off) with off of length nstates+2 (``E.nSt`` with the
-    outer group appended, :49-50) and the column-major linear indices.
+    H: (n1, n2) or (n1, n2, repeats) int array; H[k1,k2(,r)] = state
+    (1..nstates) that sampled that phase-encode position (in repeat r), 0 if
+    unsampled.  For s = 1..nstates+1 the samples of state mod(s, nstates+1)
+    are taken in MATLAB linear (column-major) order over all of H's
+    dimensions (:27-28, :36-38), so the last group is the unsampled
+    positions (:37).  Returns (k1, k2, off, idx) with off of length
+    nstates+2 (``E.nSt`` with the outer group appended, :49-50) and idx the
+    column-major linear indices; the repeat of a sample is idx // (n1 n2),
+    which is also its row in the data permuted to [n1, n2, repeats, ...]
+    (:29-30).
     """
-    n1, n2 = H.shape
+    n1, n2 = H.shape[:2]
     Hl = H.reshape(-1, order="F")
     k1s, k2s, idx, off = [], [], [], [0]
     for s in range(1, nstates + 2):
         ind = np.nonzero(Hl == (s % (nstates + 1)))[0]
         idx.append(ind)
         k1s.append(ind % n1)
-        k2s.append(ind // n1)
+        k2s.append((ind // n1) % n2)
         off.append(off[-1] + len(ind))
     return (np.concatenate(k1s), np.concatenate(k2s), np.asarray(off, np.int64),
             np.concatenate(idx))
@@ -1238,7 +1242,13 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     ysrc = yc
     if traj is not None:
         stateSample = np.array(traj["sweep_of_profile"])
-        row_of = {(int(a), int(b)): r for r, (a, b) in enumerate(zip(k1, k2))}
+        Ti = np.asarray(traj["time_index"])
+        Ti = Ti[..., None] if Ti.ndim == 2 else Ti  # [NY1, NY2, repeats]
+        H0 = np.asarray(traj["H"]).reshape(Ti.shape)
+        hk1, hk2, hoff, hidx = build_harmonic_sampling(H0, NSw)
+        if not (np.array_equal(hk1[:hoff[-2]], k1) and np.array_equal(hk2[:hoff[-2]], k2)):
+            raise ValueError("traj does not describe the sampling k1, k2, off")
+        row_of = {int(i): r for r, i in enumerate(hidx[:hoff[-2]])}  # linear index over [NY1, NY2, repeats]
     subT = [1] * NSw
     subTT = [1] * NSw
     last_W = None
@@ -1254,16 +1264,16 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
             subTT = [v for v in subT for _ in range(v)]
             sweep_of_state = [sw for sw, v in zip(sweep_of_state, subT) for _ in range(v)]
             stateSample = TO.subdivide_states(stateSample, subT)
-            Ti = traj["time_index"]
             H = np.zeros_like(Ti)
             for i in range(Ti.shape[0]):
                 for j in range(Ti.shape[1]):
-                    if Ti[i, j]:
-                        H[i, j] = stateSample[Ti[i, j] - 1]
-            kc1, kc2, offh, _ = build_harmonic_sampling(H, len(sweep_of_state))
+                    for r in range(Ti.shape[2]):
+                        if Ti[i, j, r]:
+                            H[i, j, r] = stateSample[Ti[i, j, r] - 1]
+            kc1, kc2, offh, hidx = build_harmonic_sampling(H, len(sweep_of_state))
             offc = offh[:-1]
             kc1, kc2 = kc1[:offc[-1]], kc2[:offc[-1]]
-            ysrc = yc[[row_of[(int(a), int(b))] for a, b in zip(kc1, kc2)]]
+            ysrc = yc[[row_of[int(i)] for i in hidx[:offc[-1]]]]
         subT = [1] * len(sweep_of_state)
         S = len(sweep_of_state)
         rows, l1, l2, loff = _level_kspace_rows(kc1, kc2, offc, NYg, ny)

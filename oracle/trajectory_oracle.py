@@ -196,15 +196,22 @@ def dct2_ortho(x):
     return C @ x
 
 
-def trajectory_states(kTraj, kRange, NEchos, groupSweeps=1):
+def trajectory_states(kTraj, kRange, NEchos, groupSweeps=1, NRepeats=1):
     kTraj = np.asarray(kTraj, np.int64)
     NProfs = kTraj.shape[0]
     if NEchos == 1:
         NEchos = NProfs
     NShots = NProfs // NEchos
+    if NShots < NRepeats:  # solveXT.m:85-90
+        NShots = NRepeats
+        NProfs = NShots * NEchos
+        kTraj = np.concatenate([kTraj] * NRepeats, axis=0)
+        NEchos = NProfs // NShots
     kShift = [math.floor((kRange[d][1] - kRange[d][0] + 1) / 2) + 1 for d in range(2)]
     kSize = [kRange[d][1] - kRange[d][0] + 1 for d in range(2)]
-    steady = NShots == 1
+    steady = False  # solveXT.m:110-115
+    if NShots % NRepeats == 0:
+        steady = NShots // NRepeats == 1
     if steady:
         A = np.abs(dct2_ortho(kTraj))
         iAs = [int(np.argmax(A[:NProfs // 2, c])) + 1 for c in range(2)]
@@ -215,18 +222,23 @@ def trajectory_states(kTraj, kRange, NEchos, groupSweeps=1):
     else:
         NSweeps = NShots
     sweep = [math.ceil(NSweeps * ((n - 0.5) / NProfs)) for n in range(1, NProfs + 1)]
-    T = np.zeros(kSize, np.int64)
-    for n in range(1, NProfs + 1):
+    T = np.zeros(kSize + [NRepeats], np.int64)  # solveXT.m:132-141
+    for n in range(1, kTraj.shape[0] + 1):  # size(kIndex, 1)
         i, j = kTraj[n - 1, 0] + kShift[0], kTraj[n - 1, 1] + kShift[1]
-        if T[i - 1, j - 1] == 0:
-            T[i - 1, j - 1] = n
+        for s in range(NRepeats):
+            if T[i - 1, j - 1, s] == 0:
+                T[i - 1, j - 1, s] = n
+                break
     for m in range(2):
         T = np.fft.ifftshift(T, axes=m)
     H = np.zeros_like(T)
     for i in range(T.shape[0]):
         for j in range(T.shape[1]):
-            if T[i, j]:
-                H[i, j] = sweep[T[i, j] - 1]
+            for s in range(NRepeats):
+                if T[i, j, s]:
+                    H[i, j, s] = sweep[T[i, j, s] - 1]
+    if NRepeats == 1:
+        T, H = T[..., 0], H[..., 0]
     return {"shots": NShots, "echoes": NEchos, "sweeps": NSweeps, "steady_state": steady,
             "sweep_of_profile": np.array(sweep), "time_index": T, "H": H}
 

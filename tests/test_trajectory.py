@@ -179,3 +179,80 @@ def test_subdivide_states_hand_example():
     assert new_of_old.tolist() == [1, 3, 4]
     # a one-profile state moves whole to the next label, leaving its own empty
     assert TR.subdivide_states([1, 2, 2], [True, False])[0].tolist() == [2, 3, 3]
+
+
+def test_repeats_multishot_slots():
+    """Two passes over the shutter in 2 x 16 shots, two repeats (solveXT.m:84-141):
+    no repetition of the list (32 shots >= 2 repeats), not steady state
+    (32 / 2 != 1), 32 sweeps; every position's first acquisition (shots
+    1..16) takes repeat slot 1 and its second (shots 17..32) slot 2, so the
+    state map is the shot map in slot 1 and the shot map + 16 in slot 2.
+    The harmonic sampling returns each state's lines with their repeat, and
+    a third acquisition of a position is refused (the reference drops it)."""
+    K, U = (24, 20), (4, 4)
+    k = shutter_points(*K)
+    S = 16
+    k = k[: len(k) - len(k) % S]
+    ko, NP, _, _, shot, _ = TR.sampling_disorder(k, U, "RandomCheckered", "AlternatingZigZag", K=K,
+                                                 rng=np.random.default_rng(5))
+    kRange = [[-(K[0] // 2), K[0] - K[0] // 2 - 1], [-(K[1] // 2), K[1] - K[1] // 2 - 1]]
+    k2p = np.concatenate([ko, ko])
+    st = TR.trajectory_states(k2p, kRange, NP // S, n_repeats=2)
+    assert st["shots"] == 2 * S and st["sweeps"] == 2 * S and st["repeats"] == 2 and not st["steady_state"]
+    H0 = np.fft.ifftshift(shot)
+    assert st["H"].shape == K + (2,)
+    assert np.array_equal(st["H"][..., 0], H0)
+    assert np.array_equal(st["H"][..., 1], np.where(H0 > 0, H0 + S, 0))
+    T = st["time_index"]
+    assert np.array_equal(T[..., 1], np.where(T[..., 0] > 0, T[..., 0] + NP, 0))
+    ref = TO.trajectory_states(k2p, kRange, NP // S, NRepeats=2)
+    for key in ("H", "time_index", "sweep_of_profile"):
+        assert np.array_equal(st[key], ref[key]), key
+    k1, k2, off, st2 = TR.sampling_from_trajectory(k2p, kRange, NP // S, n_repeats=2)
+    rep = st2["sample_repeat"]
+    assert np.array_equal(np.diff(off), np.full(2 * S, NP // S))
+    state = np.repeat(np.arange(1, 2 * S + 1), NP // S)
+    assert np.array_equal(rep, (state > S).astype(rep.dtype))
+    assert (st["H"][k1, k2, rep] == state).all()
+    # the same lines in the same order in both passes
+    assert np.array_equal(k1[: off[S]], k1[off[S]:]) and np.array_equal(k2[: off[S]], k2[off[S]:])
+    # against the oracle's buildHarmonicSampling over the 3-D state map
+    from oracle import disorder_oracle as O
+    ok1, ok2, ooff, oidx = O.build_harmonic_sampling(st["H"], 2 * S)
+    assert np.array_equal(ok1[: ooff[-2]], k1) and np.array_equal(ok2[: ooff[-2]], k2)
+    assert np.array_equal(oidx[: ooff[-2]] // (K[0] * K[1]), rep)
+    with pytest.raises(ValueError):
+        TR.trajectory_states(np.concatenate([ko, ko, ko]), kRange, NP // S, n_repeats=2)
+    # one repeat: the second pass has no slot
+    with pytest.raises(ValueError):
+        TR.trajectory_states(k2p, kRange, NP // S)
+
+
+@pytest.mark.parametrize("R", [2, 3])
+def test_repeats_single_shot_steady_state(R):
+    """One echo train and R repeats (solveXT.m:85-90, :110-126): the
+    trajectory is repeated R times, the scan is steady state (R shots / R
+    repeats = 1) and the DCT of the repeated triangle wave peaks at R times
+    the single scan's frequency, so 2PR sweeps of M profiles; repeat r's
+    profiles fill slot r."""
+    P, M = 2, 12
+    c = 2 * P
+    up = np.arange(M) - M // 2
+    tri = np.concatenate([up, up[::-1]] * P)
+    half = np.repeat(np.arange(2 * P), M)
+    kTraj = np.stack([tri, c * tri + half], 1)
+    kRange = [[-M // 2, M // 2 - 1], [int(kTraj[:, 1].min()), int(kTraj[:, 1].max())]]
+    NP = len(tri)
+    a = TR.trajectory_states(kTraj, kRange, 1, n_repeats=R)
+    b = TO.trajectory_states(kTraj, kRange, 1, NRepeats=R)
+    assert a["steady_state"] and b["steady_state"] and a["shots"] == R
+    assert a["sweeps"] == b["sweeps"] == 2 * P * R
+    assert np.array_equal(a["sweep_of_profile"], np.repeat(np.arange(1, 2 * P * R + 1), M))
+    for key in ("H", "time_index", "sweep_of_profile"):
+        assert np.array_equal(a[key], b[key]), key
+    T = a["time_index"]
+    for r in range(1, R):
+        assert np.array_equal(T[..., r], np.where(T[..., 0] > 0, T[..., 0] + r * NP, 0))
+    assert np.array_equal(a["H"][..., R - 1], np.where(a["H"][..., 0] > 0, a["H"][..., 0] + 2 * P * (R - 1), 0))
+    with pytest.raises(ValueError):  # two shots for three repeats
+        TR.trajectory_states(kTraj, kRange, NP // 2, n_repeats=3)
