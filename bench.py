@@ -32,10 +32,13 @@ def parse():
     ap.add_argument("--no-pyramid", action="store_true",
                     help="E^H E line without the time-to-NRMSE summary of the joint pyramid (plain and learned)")
     ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched", "cs"], default="ehe",
+    ap.add_argument("--cpu-full", dest="cpu_full", action="store_true",
+                    help="--workload small: run the oracle's whole 64^3 case (minutes), not a bounded sample")
+    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched", "cs", "small"], default="ehe",
                     help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid; "
                          "compress: 64 -> 8 coil compression of maps and data; batched: six uneven volumes in one "
-                         "batched CG with hipGraph-captured iterations")
+                         "batched CG with hipGraph-captured iterations; small: the 64^3 reference case (CG with "
+                         "known motion, then the 3-level pyramid)")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--outer", type=int, default=10)
     ap.add_argument("--cg", type=int, default=8)
@@ -434,6 +437,9 @@ def main():
     if args.workload == "batched":
         print(json.dumps(bench_batched(args)), flush=True)
         return
+    if args.workload == "small":
+        print(json.dumps(bench_small(args)), flush=True)
+        return
     import numpy as np
     import torch
     import disorder_amd as D
@@ -527,6 +533,97 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(n, C, S, seed=5)
     print(json.dumps(line), flush=True)
+
+
+def bench_small(args):
+    """The reference's CPU-runnable case at 64^3: 10 random ellipsoids
+    (magnitudes U(0.2, 1), quadratic phase within +-pi/4), 8 Gaussian coils
+    (sigma 24, radius 48 voxels), 16 shots of 4x4 random-checkered tiles
+    (256 lines each), rotations U(-5, 5) deg and translations U(-3, 3)
+    voxels, 30 dB noise; 20 CG iterations with the known motion, then the
+    joint CG+LM pyramid over 3 levels.  value = GPU wall time of both steps.
+    cpu_baseline = the FP64 oracle on the same inputs: by default a bounded
+    sample (3 of the 20 known-motion CG iterations, scaled to 20, to compare
+    with cg20_s; the pyramid is not sampled); with --cpu-full the whole run,
+    and the distance of the GPU's image and motion to the oracle's."""
+    import numpy as np
+    import torch
+    import disorder_amd as D
+    from disorder_amd import pyramid as PY
+    from disorder_amd import synth
+
+    dev = torch.device("cuda", 0)
+    n, C, S = (64, 64, 64), 8, 16
+    _, k1, k2, off = synth.disorder_sampling(n[0], n[1], S, 4, 4, seed=1)
+    x_gt = synth.phantom(n, 10, seed=2, device=dev, mag=(0.2, 1.0))
+    Sm = synth.coil_maps(n, C, seed=3, device=dev)
+    T_gt = synth.uniform_motion(S, seed=4)
+    E = D.Encoding(Sm, k1, k2, off)
+    y = E.encode(x_gt, T_gt)
+    y = synth.add_noise(y, synth.noise_level(x_gt, Sm, 30.0), seed=5)
+    kw = dict(nlevels=3, n_outer=args.outer, n_cg=args.cg, final_cg=args.final_cg)
+    for _ in range(args.warmup):  # first-call set-up (plans, twiddles) outside the timed runs
+        D.CGsolver(y, E, T_gt, nIt=2, tolType="RelativeResidual2Error", tol=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xk, _, _ = D.CGsolver(y, E, T_gt, nIt=20, tolType="RelativeResidual2Error", tol=0)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t0
+    E.close()
+    t0 = time.perf_counter()
+    x, T, info = PY.solveXT(y, Sm, k1, k2, off, n, **kw)
+    torch.cuda.synchronize()
+    t_pyr = time.perf_counter() - t0
+    ng = torch.linalg.vector_norm(x_gt).item()
+    T_err = T_gt - T
+    T_err = T_err - T_err.mean(axis=0, keepdims=True)
+    line = {"metric": "64^3 reference case: known-motion CG (20 it) + 3-level joint pyramid, wall s",
+            "value": t_cg + t_pyr, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "complex64",
+            "data": "synthetic (seeded: 10 ellipsoids, 8 Gaussian coils, 4x4 random-checkered sampling, "
+                    "uniform motion 5 deg / 3 voxels, 30 dB noise)",
+            "config": {"workload": f"64x64x64, {C} coils, {S} shots of {off[1] - off[0]} lines; CG 20 it "
+                                   f"(known motion), then pyramid 3 levels, {args.outer} outer, {args.cg} CG, "
+                                   f"final CG {args.final_cg}"},
+            "cg20_s": t_cg, "pyramid_s": t_pyr, "levels": info["levels"],
+            "cg_nrmse_vs_gt": torch.linalg.vector_norm(xk - x_gt).item() / ng,
+            "pyramid_nrmse_vs_gt": torch.linalg.vector_norm(x - x_gt).item() / ng,
+            "motion_err_max_vox": float(np.abs(T_err[:, :3]).max()),
+            "motion_err_max_deg": float(np.abs(T_err[:, 3:]).max() * 180 / math.pi)}
+    if args.no_cpu_baseline:
+        return line
+    from oracle import disorder_oracle as O
+    workers = min(os.cpu_count() or 1, 16)
+    O.set_fft_workers(workers)
+    yh = np.asarray(y.cpu().numpy(), np.complex128)
+    Sh = np.asarray(Sm.cpu().numpy(), np.complex128)
+    samp = O.Sampling(k1, k2, off, n[0], n[1])
+    kcg = 20 if args.cpu_full else 3
+    t0 = time.perf_counter()
+    O.cg_solver(yh, Sh, samp, T_gt, n, nIt=kcg, tolType="RelativeResidual2Error", tol=0)
+    t_ocg = (time.perf_counter() - t0) * 20 / kcg
+    print(f"[small] oracle CG {kcg} it: {t_ocg * kcg / 20:.1f} s", file=sys.stderr, flush=True)
+    cpu = {"unit": "s", "cores": workers, "kind": "port", "cg20_s": t_ocg}
+    if args.cpu_full:
+        def cb(level, it, xo, To):
+            print(f"[small] oracle pyramid level {level} outer {it}: {time.perf_counter() - t1:.1f} s",
+                  file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        xo, To = O.pyramid_solve(yh, Sh, k1, k2, off, n, callback=cb, **kw)
+        t_opyr = time.perf_counter() - t1
+        xg = x.cpu().numpy()
+        cpu.update({"value": t_ocg + t_opyr, "pyramid_s": t_opyr,
+                    "sample": f"the whole run: FP64 oracle (scipy.fft, {workers} threads) CG 20 it + pyramid"})
+        line["parity_vs_oracle"] = {
+            "image_nrmse": float(np.linalg.norm(xg - xo) / np.linalg.norm(xo)),
+            "motion_max_vox": float(np.abs(T[:, :3] - To[:, :3]).max()),
+            "motion_max_deg": float(np.abs(T[:, 3:] - To[:, 3:]).max() * 180 / math.pi)}
+    else:
+        cpu.update({"value": t_ocg, "compare_to": "cg20_s",
+                    "sample": f"FP64 oracle (scipy.fft, {workers} threads) known-motion CG, {kcg} of the 20 "
+                              f"iterations timed and scaled to 20; the pyramid is timed only with --cpu-full"})
+    O.set_fft_workers(None)
+    line["cpu_baseline"] = cpu
+    return line
 
 
 def stream_copy_rate(dev, nbytes=2 << 30, reps=10):
