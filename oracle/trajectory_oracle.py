@@ -11,8 +11,9 @@ Follows, loop for loop and with 1-based indices kept as in the MATLAB text:
 Random permutations (randperm) are drawn from a numpy Generator in the
 reference's order.  Pinning: hand-worked electrostatic orders for U = 2x2
 and 3x3, closed forms of the orders on full grids (one point per shot per
-tile, echo counts, zig-zag key order), and the DCT frequency of a sweep
-trajectory; see tests/test_trajectory.py.
+tile, echo counts, zig-zag key order), the DCT frequency of a sweep
+trajectory (also repeated R times), and the repeat slots of a scan acquired
+twice (slot 2 = slot 1 shifted by one pass); see tests/test_trajectory.py.
 """
 from __future__ import annotations
 
