@@ -28,7 +28,6 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
     x_uncorrected (with uncorrected_cg > 0, CG without motion, solveXT.m:242),
     files (with write = (path_out, name[, A]))."""
     import torch
-    ctx = ctx or ops.default_context(y.device.index or 0)
     N = tuple(int(v) for v in S.shape[:3])
     if y.dim() not in (4, 5) or y.shape[0] > N[0] or y.shape[1] > N[1] or y.shape[2] != N[2]:
         raise ValueError("y must be [NY1, NY2, N3, C] or [NY1, NY2, N3, C, R] with NY1 <= N1, NY2 <= N2 "
@@ -36,10 +35,15 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
     NY = (int(y.shape[0]), int(y.shape[1]))
     folded = NY != N[:2]
     R = int(y.shape[4]) if y.dim() == 5 else 1
+    if int(y.shape[3]) != int(S.shape[3]):
+        raise ValueError("y and S must hold the same coils")
+    if y.dim() == 4 and not ops.is_colmajor(y):  # the rows below are read through a column-major view
+        raise ValueError("y must have column-major (MATLAB) strides; use disorder_amd.colmajor(y)")
     if y.dim() == 5:  # repeats next to the phase encodes, coils last: [NY1, NY2, R, N3, C]
         yp = ops.empty_colmajor((NY[0], NY[1], R, N[2], int(y.shape[3])), y.device)
         yp.copy_(y.permute(0, 1, 4, 2, 3))
         y = yp
+    ctx = ctx or ops.default_context(y.device.index or 0)
     # coil compression (solveXT.m:61): maps and data on the first max(eivaS) components
     if perc is not None:
         S, y, eivaS, _, _ = ops.compressCoils(S, list(perc), y, 3, ctx)

@@ -60,3 +60,19 @@ def test_aligned_reconstruction_end_to_end(dev, tmp_path):
                             "scan_RePh.nii", "scan_Tr.mat"])
     back, _, _ = nifti.read_pair(os.path.join(str(tmp_path), "An-Ve", "scan"), "Di")
     assert back.shape == N and rel_l2(back, x) < 1e-6
+
+
+def test_entry_point_checks_data_before_touching_the_device():
+    """Shape, coil count and memory layout of y are checked before any
+    device work: a row-major 4-D y (read through a column-major view below)
+    and a coil count that differs from the maps' are refused."""
+    import torch
+    from disorder_amd import colmajor, recon
+    S = colmajor(torch.zeros((8, 6, 4, 2), dtype=torch.complex64))
+    kTraj, kRange = np.zeros((2, 2), np.int64), [[-4, 3], [-3, 2]]
+    with pytest.raises(ValueError, match="column-major"):
+        recon.aligned_reconstruction(torch.zeros((8, 6, 4, 2), dtype=torch.complex64), S, kTraj, kRange, 1)
+    with pytest.raises(ValueError, match="same coils"):
+        recon.aligned_reconstruction(colmajor(torch.zeros((8, 6, 4, 3), dtype=torch.complex64)), S, kTraj, kRange, 1)
+    with pytest.raises(ValueError, match="NY1"):
+        recon.aligned_reconstruction(colmajor(torch.zeros((8, 6, 4), dtype=torch.complex64)), S, kTraj, kRange, 1)
