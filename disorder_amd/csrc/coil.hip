@@ -327,8 +327,18 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   lds_float2* xb = (lds_float2*)xzraw + p * XB;
   lds_float2* Zt = (lds_float2*)xzraw;  // [16 rows][N] natural order
   const int n2 = a.n2;
+  // dim-2 DFT of the sampled rows: length n2, or under SENSE folding along
+  // dim 2 (fy1 == n1) the acquired length m2 = fy2 with the image rows
+  // aliased at offset o2 = ceil((n2 - m2) / 2) (Methods/fold.m): the m2-point
+  // DFT of the alias sum at k2 is sum_r W_m2^{k2 (r - o2)} v[r], so the same
+  // row recurrences run with m2-point twiddles started at -k2 o2
+  const bool fold2 = a.fy1 > 0;
+  const int m2 = fold2 ? a.fy2 : n2;
+  const int o2 = fold2 ? (n2 - m2 + 1) / 2 : 0;
+  const int o2p = (m2 - o2 % m2) % m2;  // row shift equal to -o2 modulo m2, so k2 (r0 + o2p) >= 0
+  const float2* tws = fold2 ? a.fy2tw : a.tw2;
   for (int k = tid; k < N; k += kThreads) st2(tw1 + k, a.tw1[k]);
-  for (int k = tid; k < n2; k += kThreads) st2(tw2 + k, a.tw2[k]);
+  for (int k = tid; k < m2; k += kThreads) st2(tw2 + k, tws[k]);
   __syncthreads();
 
   // XCD-aware items: XCD x takes planes x, x+8, ... with all states of a
@@ -339,7 +349,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   const int s = a.slist ? a.slist[sl] : a.state0 + sl;
   const long long nplane = (long long)N * n2;
   const long long nvox = nplane * a.n3;
-  const float inv_sqrt = rsqrtf((float)N * (float)n2);
+  const float inv_sqrt = rsqrtf((float)N * (float)m2);
   const float inv_n = 1.0f / (float)N;
   const int o0 = a.off[s], o1 = a.off[s + 1];
   const float2* Up = FWD ? a.U + (a.us_rel ? sl : s) * a.us + i3 * nplane : nullptr;
@@ -364,7 +374,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
       jk1[i] = ok ? a.k1[j] : -1;
       jk2[i] = ok ? a.k2[j] : 0;
       jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
-      jstep[i] = (16 * jk2[i]) % n2;  // twiddle-index advance per 16-row group
+      jstep[i] = (16 * jk2[i]) % m2;  // twiddle-index advance per 16-row group
     }
     float esum[kRJ];
 #pragma unroll
@@ -379,9 +389,9 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         for (int c = 0; c < kCC; ++c) W[i][c] = make_float2(0.f, 0.f);
 
       if (FWD) {
-        int ib[kRJ];  // W2 index of row r0 for each sample: (k2 * r0) mod n2
+        int ib[kRJ];  // W2 index of row r0 for each sample: (k2 (r0 - o2)) mod m2
 #pragma unroll
-        for (int i = 0; i < kRJ; ++i) ib[i] = 0;
+        for (int i = 0; i < kRJ; ++i) ib[i] = (jk2[i] * o2p) % m2;
         for (int g = 0; g < ngroups; ++g) {
           const int r0 = g * 16;
           const int Gv = min(16, n2 - r0);
@@ -462,7 +472,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
           for (int i = 0; i < kRJ; ++i) {
             ib[i] += jstep[i];
-            if (ib[i] >= n2) ib[i] -= n2;
+            if (ib[i] >= m2) ib[i] -= m2;
           }
         }
 #pragma unroll
@@ -509,7 +519,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         lds_int* entl = rpl + (N + 1);
         const int* rp = a.rowptr + (long long)s * (N + 1);
         const int jl0 = sc0 - o0, jl1 = sc1 - o0;
-        const float fn2 = 1.0f / (float)n2;
+        const float fn2 = 1.0f / (float)m2;
         const int nent = o1 - o0;
         const bool csr_lds = nent <= kEntCap;
         if (csr_lds) {
@@ -547,10 +557,10 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                 if (jl >= jl0 && jl < jl1) {
                   const int k2 = ent >> 16;
                   const float2 w = ld2(wc + (jl - jl0));
-                  const int prod = k2 * r0;
-                  int idx = prod - n2 * (int)((float)prod * fn2);
-                  idx += idx < 0 ? n2 : 0;
-                  idx -= idx >= n2 ? n2 : 0;
+                  const int prod = k2 * (r0 + o2p);  // = k2 (r0 - o2) modulo m2
+                  int idx = prod - m2 * (int)((float)prod * fn2);
+                  idx += idx < 0 ? m2 : 0;
+                  idx -= idx >= m2 ? m2 : 0;
                   // conj(W2^{k2 (r0+q)}) w = (w conj(W2^{k2 r0})) conj(W2^{k2})^q
                   float2 wq = cmulc(w, ld2(tw2 + idx));
                   const float2 pst = ld2(tw2 + k2);
@@ -680,8 +690,15 @@ hipError_t launch_rcoil_persistent(const CoilArgs& a, int mode, int nstates, hip
   return hipGetLastError();
 }
 
+bool coil_fold_register(int n1, int n2, int fy1) { return fy1 == n1 && n2 <= 1024 && reg_radix(n1) != 0; }
+
+// a folded plan stays on the register path when only dim 2 folds
+static bool fold_stage(const CoilArgs& a) {
+  return a.fy1 > 0 && (a.force_generic || !coil_fold_register(a.n1, a.n2, a.fy1));
+}
+
 hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
-  if (a.fy1 > 0) return launch_coil_fold(a, mode, nstates, st);
+  if (fold_stage(a)) return launch_coil_fold(a, mode, nstates, st);
   if (!a.queue || a.n2 > 1024 || a.force_generic) return hipErrorInvalidValue;
   switch (reg_radix(a.n1)) {
     case 16: return launch_rcoil_persistent<16>(a, mode, nstates, st);
@@ -698,12 +715,13 @@ hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipS
 
 bool coil_queue_available(int n1, int n2) { return n2 <= 1024 && reg_radix(n1) != 0; }
 
+
 int coil_chunk(int n1, int n2) {
   return (n2 <= 1024 && reg_radix(n1)) ? kRJ * kThreads : kMaxJ * kThreads;
 }
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
-  if (a.fy1 > 0) return launch_coil_fold(a, mode, nstates, st);
+  if (fold_stage(a)) return launch_coil_fold(a, mode, nstates, st);
   if (a.n2 <= 1024 && !a.force_generic) {
     switch (reg_radix(a.n1)) {
       case 16: return launch_rcoil<16>(a, mode, nstates, st);

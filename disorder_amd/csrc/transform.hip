@@ -424,8 +424,9 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
   // E^H E runs the coil stage as its two halves, the forward into a k-space
   // scratch [Ns x n3 x C] and the adjoint from it: at the benchmark size the
   // two launches take 14.5 + 20.1 ms against 63.7 ms for the fused on-chip
-  // kernel, for 2 x 805 MB of extra traffic (unfolded plans)
-  const bool split = mode == COIL_EHE && v.fold[0] == 0;
+  // kernel, for 2 x 805 MB of extra traffic (every plan on the register
+  // path, including plans folded along dim 2 only)
+  const bool split = mode == COIL_EHE && (v.fold[0] == 0 || coil_fold_register(g.n[0], g.n[1], v.fold[0]));
   float2* yscr = nullptr;
   if (split) {
     void* yb = nullptr;

@@ -100,8 +100,11 @@ def _to_dev(a, dev):
 
 
 # (reconstruction grid, acquired phase-encode grid): non-multiple, odd
-# difference with a 3^3 acquired length, and multiples (register-path n1)
-GRIDS = [((40, 36, 16), (32, 30)), ((40, 36, 12), (35, 27)), ((64, 48, 16), (32, 24))]
+# difference with a 3^3 acquired length, and multiples (register-path n1);
+# then folds along dim 2 only on register-path n1, which run in the register
+# coil kernel (non-multiple, multiple, odd acquired length)
+GRIDS = [((40, 36, 16), (32, 30)), ((40, 36, 12), (35, 27)), ((64, 48, 16), (32, 24)),
+         ((64, 48, 16), (64, 32)), ((64, 48, 12), (64, 24)), ((32, 40, 12), (32, 27))]
 
 
 @pytest.mark.gpu
@@ -134,9 +137,9 @@ def test_folded_operators_match_oracle(dev, n, ny):
 
 
 @pytest.mark.gpu
-def test_folded_cg_and_lm_match_oracle(dev):
+@pytest.mark.parametrize("n,ny", [((40, 36, 12), (32, 30)), ((64, 48, 12), (64, 30))])
+def test_folded_cg_and_lm_match_oracle(dev, n, ny):
     import disorder_amd as D
-    n, ny = (40, 36, 12), (32, 30)
     rng = np.random.default_rng(11)
     x, Sm, samp = _fold_problem(rng, n, ny, C=4, S=4)
     Sm = Sm.astype(np.complex64).astype(np.complex128)
