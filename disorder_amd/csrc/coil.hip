@@ -336,6 +336,20 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   const int m2 = fold2 ? a.fy2 : n2;
   const int o2 = fold2 ? (n2 - m2 + 1) / 2 : 0;
   const int o2p = (m2 - o2 % m2) % m2;  // row shift equal to -o2 modulo m2, so k2 (r0 + o2p) >= 0
+  // folding along dim 1 by an integer factor q1 = N / m1: the m1-point DFT
+  // of the alias sum at k1 is P(k1) FFT_N(v)[q1 k1] with P(k1) = W_m1^{-k1 o1}
+  // = exp(2 pi i k1 o1 / m1), so samples gather spectrum column q1 k1 and
+  // take the phase; the adjoint publishes conj(P) y into column q1 k1 and
+  // the inverse FFT replicates it periodically (Methods/ifold.m)
+  const int m1 = fold2 ? a.fy1 : N;
+  const int q1 = N / m1;
+  const int o1 = (N - m1 + 1) / 2;
+  auto fold1_phase = [&](int k1) -> float2 {
+    if (q1 == 1) return make_float2(1.f, 0.f);
+    float sn, cs;
+    sincospif(2.0f * (float)((k1 * o1) % m1) / (float)m1, &sn, &cs);
+    return make_float2(cs, sn);
+  };
   const float2* tws = fold2 ? a.fy2tw : a.tw2;
   for (int k = tid; k < N; k += kThreads) st2(tw1 + k, a.tw1[k]);
   for (int k = tid; k < m2; k += kThreads) st2(tw2 + k, tws[k]);
@@ -349,7 +363,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   const int s = a.slist ? a.slist[sl] : a.state0 + sl;
   const long long nplane = (long long)N * n2;
   const long long nvox = nplane * a.n3;
-  const float inv_sqrt = rsqrtf((float)N * (float)m2);
+  const float inv_sqrt = rsqrtf((float)m1 * (float)m2);
   const float inv_n = 1.0f / (float)N;
   const int o0 = a.off[s], o1 = a.off[s + 1];
   const float2* Up = FWD ? a.U + (a.us_rel ? sl : s) * a.us + i3 * nplane : nullptr;
@@ -433,10 +447,10 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
               for (int i = 0; i < kRJ; ++i) {
                 if (jk1[i] >= 0) {
                   const float2 a1 = ld2(tw2 + jk2[i]);
-                  float2 acc = ld2(Zt + 15 * N + jk1[i]);
+                  float2 acc = ld2(Zt + 15 * N + jk1[i] * q1);
 #pragma unroll
                   for (int q = 14; q >= 0; --q) {
-                    const float2 z = ld2(Zt + q * N + jk1[i]);
+                    const float2 z = ld2(Zt + q * N + jk1[i] * q1);
                     acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
                     if ((q & (kHornerBatch - 1)) == 0) __builtin_amdgcn_sched_barrier(0);
                   }
@@ -456,7 +470,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
                   for (int q = 15; q >= 0; --q) {
                     if (q < Gv) {
-                      const float2 z = ld2(Zt + q * N + jk1[i]);
+                      const float2 z = ld2(Zt + q * N + jk1[i] * q1);
                       acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
                     }
                   }
@@ -479,17 +493,18 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         for (int i = 0; i < kRJ; ++i) {
           const int j = jg[i];
           if (jk1[i] >= 0) {
+            const float2 ph = fold1_phase(jk1[i]);
 #pragma unroll
             for (int c = 0; c < kCC; ++c) {
               if (c < cc) {
-                const float2 Y = cscale(W[i][c], inv_sqrt);
+                const float2 Y = cscale(q1 > 1 ? cmul(W[i][c], ph) : W[i][c], inv_sqrt);
                 const long long yi = j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c));
                 if (MODE == COIL_ENCODE) a.yout[yi] = Y;
                 if (MODE == COIL_ENERGY) {
                   const float2 r = csub(Y, a.yin[yi]);
                   esum[i] += r.x * r.x + r.y * r.y;
                 }
-                if (MODE == COIL_EHE) W[i][c] = cscale(Y, jw[i]);
+                if (MODE == COIL_EHE) W[i][c] = cscale(q1 > 1 ? cmulc(Y, ph) : Y, jw[i]);
               }
             }
           }
@@ -501,9 +516,14 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         for (int i = 0; i < kRJ; ++i) {
           const int j = jg[i];
           if (jk1[i] >= 0) {
+            const float2 ph = fold1_phase(jk1[i]);
 #pragma unroll
-            for (int c = 0; c < kCC; ++c)
-              if (c < cc) W[i][c] = cscale(a.yin[j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c))], jw[i]);
+            for (int c = 0; c < kCC; ++c) {
+              if (c < cc) {
+                const float2 yv = a.yin[j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c))];
+                W[i][c] = cscale(q1 > 1 ? cmulc(yv, ph) : yv, jw[i]);
+              }
+            }
           }
         }
       }
@@ -549,8 +569,11 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
               float2 acc[16];
 #pragma unroll
               for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
-              const int e0 = csr_lds ? rpl[k1] : rp[k1];
-              const int e1 = csr_lds ? rpl[k1 + 1] : rp[k1 + 1];
+              // spectrum column k1 holds acquired row k1 / q1 when q1 divides it
+              const int ka = k1 / q1;
+              const bool own = k1 == ka * q1;
+              const int e0 = own ? (csr_lds ? rpl[ka] : rp[ka]) : 0;
+              const int e1 = own ? (csr_lds ? rpl[ka + 1] : rp[ka + 1]) : 0;
               for (int e = e0; e < e1; ++e) {
                 const int ent = csr_lds ? entl[e] : a.csr[o0 + e];
                 const int jl = ent & 0xffff;
@@ -690,9 +713,12 @@ hipError_t launch_rcoil_persistent(const CoilArgs& a, int mode, int nstates, hip
   return hipGetLastError();
 }
 
-bool coil_fold_register(int n1, int n2, int fy1) { return fy1 == n1 && n2 <= 1024 && reg_radix(n1) != 0; }
+bool coil_fold_register(int n1, int n2, int fy1) {
+  return fy1 > 0 && n1 % fy1 == 0 && n2 <= 1024 && reg_radix(n1) != 0;
+}
 
-// a folded plan stays on the register path when only dim 2 folds
+// a folded plan stays on the register path when dim 1 folds by an integer
+// factor (or not at all)
 static bool fold_stage(const CoilArgs& a) {
   return a.fy1 > 0 && (a.force_generic || !coil_fold_register(a.n1, a.n2, a.fy1));
 }

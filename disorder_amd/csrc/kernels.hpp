@@ -116,7 +116,8 @@ hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st)
 hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st);
 bool coil_queue_available(int n1, int n2);
 // true when a plan folded to fy1 x fy2 runs its coil stage on the register
-// path (fy1 == n1: only dim 2 folds), false when it needs the fold stage
+// path (n1 a register length that fy1 divides), false when it needs the fold
+// stage
 bool coil_fold_register(int n1, int n2, int fy1);
 // Samples per chunk of the coil-stage kernel that launch_coil picks for a
 // (n1, n2) plane: a state with more samples is processed in several chunks.

@@ -101,10 +101,13 @@ def _to_dev(a, dev):
 
 # (reconstruction grid, acquired phase-encode grid): non-multiple, odd
 # difference with a 3^3 acquired length, and multiples (register-path n1);
-# then folds along dim 2 only on register-path n1, which run in the register
-# coil kernel (non-multiple, multiple, odd acquired length)
+# folds on register-path n1 run in the register coil kernel: along dim 2
+# only (non-multiple, multiple, odd acquired length), along dim 1 by an
+# integer factor (alone, and by 3 on the radix-9 length 144 with a
+# non-multiple fold along dim 2); the grids with n1 = 40 take the fold stage
 GRIDS = [((40, 36, 16), (32, 30)), ((40, 36, 12), (35, 27)), ((64, 48, 16), (32, 24)),
-         ((64, 48, 16), (64, 32)), ((64, 48, 12), (64, 24)), ((32, 40, 12), (32, 27))]
+         ((64, 48, 16), (64, 32)), ((64, 48, 12), (64, 24)), ((32, 40, 12), (32, 27)),
+         ((64, 48, 12), (32, 48)), ((144, 36, 8), (48, 30))]
 
 
 @pytest.mark.gpu
@@ -137,7 +140,7 @@ def test_folded_operators_match_oracle(dev, n, ny):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,ny", [((40, 36, 12), (32, 30)), ((64, 48, 12), (64, 30))])
+@pytest.mark.parametrize("n,ny", [((40, 36, 12), (32, 30)), ((64, 48, 12), (64, 30)), ((64, 48, 12), (32, 30))])
 def test_folded_cg_and_lm_match_oracle(dev, n, ny):
     import disorder_amd as D
     rng = np.random.default_rng(11)
