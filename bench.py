@@ -9,7 +9,14 @@ rigid motion per shot, FP32 complex.  One step = one E^H We E x
 Prints one JSON line.  ``roofline`` is measured live with HIP events recorded
 by the library around every launch of the dominant kernel during the timed
 steps; ``cpu_baseline`` times the FP64 CPU oracle on a bounded sample (one
-shot's E^H E contribution) on this host.
+shot's E^H E contribution) on this host.  The line also carries the median
+of 20 single applications and the joint pyramid's time to 1e-3 NRMSE.
+
+Other workloads (--workload): pyramid (the joint CG+LM reconstruction, plain
+and with the learned proposal), compress (64 -> 8 coil compression),
+batched (six uneven volumes, hipGraph-captured CG), cs (the shearlet
+regulariser), small (the 64^3 reference case end to end, with the oracle's
+run beside it) and fold (E^H E with SENSE folding of the phase encodes).
 """
 from __future__ import annotations
 
