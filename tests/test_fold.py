@@ -107,7 +107,9 @@ def _to_dev(a, dev):
 # non-multiple fold along dim 2); the grids with n1 = 40 take the fold stage
 GRIDS = [((40, 36, 16), (32, 30)), ((40, 36, 12), (35, 27)), ((64, 48, 16), (32, 24)),
          ((64, 48, 16), (64, 32)), ((64, 48, 12), (64, 24)), ((32, 40, 12), (32, 27)),
-         ((64, 48, 12), (32, 48)), ((144, 36, 8), (48, 30))]
+         ((64, 48, 12), (32, 48)), ((144, 36, 8), (48, 30)),
+         # offset larger than the acquired length (64 -> 18: five-fold aliasing), and a larger grid
+         ((32, 64, 8), (32, 18)), ((128, 96, 24), (64, 72))]
 
 
 @pytest.mark.gpu
