@@ -207,12 +207,14 @@ def test_oracle_driver_fold_identity_and_motion():
 
 
 @pytest.mark.gpu
-def test_folded_driver_matches_oracle(dev):
-    """The pyramid driver on folded data (40x30x16 image, 32x24 acquired
-    phase encodes, two levels): GPU against the FP64 oracle driver, image
-    NRMSE <= 1e-4, motion within 0.01 voxel and 0.01 degree."""
+@pytest.mark.parametrize("N,NY", [((40, 30, 16), (32, 24)), ((64, 48, 16), (32, 36))])
+def test_folded_driver_matches_oracle(dev, N, NY):
+    """The pyramid driver on folded data (two levels; 40x30x16 on 32x24
+    through the fold kernels, 64x48x16 on 32x36 in the register coil kernel
+    at both levels): GPU against the FP64 oracle driver, image NRMSE <= 1e-4,
+    motion within 0.01 voxel and 0.01 degree."""
     from disorder_amd import pyramid as PY
-    x_gt, Sm, y, k1, k2, off, N, NY, _ = _folded_scan()
+    x_gt, Sm, y, k1, k2, off, N, NY, _ = _folded_scan(N=N, NY=NY)
     x_ref, T_ref = O.pyramid_solve(y, Sm, k1, k2, off, N, nlevels=2, n_outer=2, n_cg=4, NY=NY)
     xg, Tg, info = PY.solveXT(_to_dev(y, dev), _to_dev(Sm, dev), k1, k2, off, N, nlevels=2, n_outer=2, n_cg=4,
                               NY=NY)
