@@ -200,7 +200,7 @@ def bench_pyramid(args):
         learned = summary(xl, Tl, il, sl)
         learned["time_to_1e-3_own_final_s"] = ttl
         learned["final_nrmse_vs_plain_final"] = sl[-1]["nrmse_vs_ref"]
-        learned["proposals"] = {"proposed": prop.proposed, "accepted": prop.accepted}
+        learned["proposals"] = {"proposed": prop.proposed, "accepted": prop.accepted, "retired": prop.retired}
         learned["curve"] = sl
         line["learned"] = learned
         line["learned_nrmse_gt_minus_plain"] = learned["final_nrmse_vs_gt"] - plain["final_nrmse_vs_gt"]

@@ -172,12 +172,17 @@ class Proposer:
     is tried at the step fractions `alphas` in turn (one energy evaluation
     each, a shot keeps the first fraction that lowers its energy), on the
     first `max_outer` outer iterations of every level only (where the motion
-    errors the network was trained on are largest)."""
+    errors the network was trained on are largest).  With `retire`, a round
+    in which no shot keeps its proposal ends the proposals for the rest of
+    the solve, so a network that does not help costs one round of features
+    and energies, not one per level and outer iteration."""
 
-    def __init__(self, net: MotionNet, alphas=(1.0,), max_outer=2):
+    def __init__(self, net: MotionNet, alphas=(1.0,), max_outer=2, retire=True):
         self.net = net
         self.alphas = tuple(alphas)
         self.max_outer = max_outer
+        self.retire = retire
+        self.retired = False
         self.accepted = 0
         self.proposed = 0
         self._level = None
@@ -189,7 +194,7 @@ class Proposer:
         self._calls += 1
         active = ~cT
         none = np.zeros(len(T), bool)
-        if not active.any() or self._calls > self.max_outer:
+        if self.retired or not active.any() or self._calls > self.max_outer:
             return T, none
         sums = lm_sums(L.y, L.E, x, T)
         # translations in voxels of the level grid
@@ -214,6 +219,8 @@ class Proposer:
             take |= ok
         self.proposed += int(active.sum())
         self.accepted += int(take.sum())
+        if self.retire and not take.any():
+            self.retired = True
         return Tn, take
 
 
