@@ -200,3 +200,54 @@ def test_trained_proposal_against_plain_pyramid(dev, capsys):
     assert el <= ep * (1 + 1e-3)
     assert np.abs(Tl[:, :3] - Tp[:, :3]).max() <= 0.01 or prop.accepted > 0
     assert np.abs(Tl[:, 3:] - Tp[:, 3:]).max() * 180 / np.pi <= 0.01 or prop.accepted > 0
+
+
+def test_proposer_retires_after_a_round_that_keeps_nothing(monkeypatch):
+    """Host logic of Proposer(retire=True) with the GPU calls stubbed: a
+    round in which some shot's energy drops keeps proposing; the first round
+    in which no shot keeps its proposal retires the proposer, and later calls
+    (other levels too) return the motion unchanged without evaluating
+    anything."""
+    import types
+    import torch
+    from disorder_amd import learned as LN
+    S = 4
+    calls = {"sums": 0, "energy": 0}
+    drop = {"on": True}
+
+    def fake_sums(y, E, x, T):
+        calls["sums"] += 1
+        s = np.zeros((S, 28))
+        s[:, 0] = 1.0
+        s[:, 7:13] = 1.0
+        return s
+
+    def fake_energy(E, x, y, T, off):
+        calls["energy"] += 1
+        return np.full(S, 0.5 if drop["on"] else 2.0)
+
+    class FakeNet:
+        def forward(self, F, off):
+            return torch.zeros((F.shape[0], 6))
+
+    monkeypatch.setattr(LN, "lm_sums", fake_sums)
+    monkeypatch.setattr(LN, "state_energy", fake_energy)
+    monkeypatch.setattr(LN, "level_scale", lambda L: 1.0)
+    P = LN.Proposer(FakeNet(), max_outer=2)
+    L1 = types.SimpleNamespace(y=None, E=None, off=np.arange(S + 1))
+    L2 = types.SimpleNamespace(y=None, E=None, off=np.arange(S + 1))
+    T = np.zeros((S, 6))
+    cT = np.zeros(S, bool)
+    _, take = P(L1, torch.zeros(1), T, cT)
+    assert take.all() and not P.retired and P.accepted == S
+    drop["on"] = False
+    _, take = P(L1, torch.zeros(1), T, cT)
+    assert not take.any() and P.retired
+    n = dict(calls)
+    Tn, take = P(L2, torch.zeros(1), T, cT)  # a new level: still retired
+    assert not take.any() and np.array_equal(Tn, T) and calls == n and P.proposed == 2 * S
+    # without retirement the new level proposes again
+    Q = LN.Proposer(FakeNet(), max_outer=2, retire=False)
+    Q(L1, torch.zeros(1), T, cT)
+    Q(L2, torch.zeros(1), T, cT)
+    assert Q.proposed == 2 * S and not Q.retired
