@@ -61,6 +61,7 @@ struct dis_plan {
   int* sord = nullptr;    // [Ns] register coil stage: sample of each slot
   std::vector<int> hoff;
   int fold[2] = {0, 0};   // acquired phase-encode grid under SENSE folding
+  int roi[2] = {0, 0};    // readout range [lo, hi) of the LM sums (E.nF); {0, 0} = all
 };
 
 namespace {
@@ -514,6 +515,19 @@ int dis_plan_set_fold(dis_plan* p, int32_t ny1, int32_t ny2) {
   return DIS_OK;
 }
 
+int dis_plan_set_readout_roi(dis_plan* p, int32_t lo, int32_t hi) {
+  if (!p) return DIS_E_INVALID;
+  if (lo == 0 && (hi == 0 || hi == p->g.n[2])) {
+    p->roi[0] = p->roi[1] = 0;
+    return DIS_OK;
+  }
+  if (lo < 0 || hi <= lo || hi > p->g.n[2])
+    return fail(p->ctx, DIS_E_INVALID, "dis_plan_set_readout_roi: need 0 <= lo < hi <= n3");
+  p->roi[0] = lo;
+  p->roi[1] = hi;
+  return DIS_OK;
+}
+
 int dis_plan_set_weights(dis_plan* p, const float* We) {
   if (!p) return DIS_E_INVALID;
   dis_ctx* c = p->ctx;
@@ -547,6 +561,8 @@ OpView plan_view(dis_plan* p) {
   v.hoff = p->hoff.data();
   v.fold[0] = p->fold[0];
   v.fold[1] = p->fold[1];
+  v.roi[0] = p->roi[0];
+  v.roi[1] = p->roi[1];
   return v;
 }
 

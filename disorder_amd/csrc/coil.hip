@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
   __shared__ float2 tw2[1024];
   const int tid = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
-  const int i3 = blockIdx.x;
+  const int i3 = a.i3lo + blockIdx.x;
   const int sl = blockIdx.y;
   const int s = a.slist ? a.slist[sl] : a.state0 + sl;
   const int G = a.G;
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
 
 hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
   if (a.fy1 > 0) return launch_coil_fold(a, mode, nstates, st);
-  dim3 grid(a.n3, nstates);
+  dim3 grid(a.i3n ? a.i3n : a.n3, nstates);
   switch (mode) {
     case COIL_ENCODE: hipLaunchKernelGGL(coil_kernel<COIL_ENCODE>, grid, dim3(kThreads), 0, st, a); break;
     case COIL_DECODE: hipLaunchKernelGGL(coil_kernel<COIL_DECODE>, grid, dim3(kThreads), 0, st, a); break;
@@ -355,10 +355,11 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   for (int k = tid; k < m2; k += kThreads) st2(tw2 + k, tws[k]);
   __syncthreads();
 
-  // XCD-aware items: XCD x takes planes x, x+8, ... with all states of a
-  // plane consecutive (static: one item per workgroup from blockIdx;
-  // persistent: items pulled from the per-XCD queues).
-  const int per_queue = ((a.n3 + 7) / 8) * nstates;
+  // XCD-aware items: XCD x takes planes i3lo + x, i3lo + x + 8, ... with all
+  // states of a plane consecutive (static: one item per workgroup from
+  // blockIdx; persistent: items pulled from the per-XCD queues).
+  const int n3e = a.i3n ? a.i3n : a.n3;
+  const int per_queue = ((n3e + 7) / 8) * nstates;
   auto item_body = [&](const int sl, const int i3) {
   const int s = a.slist ? a.slist[sl] : a.state0 + sl;
   const long long nplane = (long long)N * n2;
@@ -650,15 +651,15 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
       if (item < 0) break;  // uniform: every thread read the same item
       const int xcd = item / per_queue, local = item - xcd * per_queue;
       const int i3 = (local / nstates) * 8 + xcd;
-      if (i3 < a.n3) {
-        item_body(local - (local / nstates) * nstates, i3);
+      if (i3 < n3e) {
+        item_body(local - (local / nstates) * nstates, a.i3lo + i3);
         __syncthreads();  // the next item's LDS writes follow every read of this one
       }
     }
   } else {
     const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3;
     const int i3 = (local / nstates) * 8 + xcd;
-    if (i3 < a.n3) item_body(local - (local / nstates) * nstates, i3);
+    if (i3 < n3e) item_body(local - (local / nstates) * nstates, a.i3lo + i3);
   }
 }
 
@@ -668,7 +669,7 @@ __global__ void zero_queue_kernel(int* q) {
 
 template <int R1>
 hipError_t launch_rcoil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
-  const int nblk = ((a.n3 + 7) / 8) * 8 * nstates;
+  const int nblk = (((a.i3n ? a.i3n : a.n3) + 7) / 8) * 8 * nstates;
   dim3 grid(nblk);
   switch (mode) {
     case COIL_ENCODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_ENCODE>), grid, dim3(kThreads), 0, st, a, nstates); break;
@@ -700,7 +701,7 @@ hipError_t launch_rcoil_persistent(const CoilArgs& a, int mode, int nstates, hip
     if (e != hipSuccess) return e;
     resident[mi] = std::max(8, (ncu * std::max(1, per_cu)) / 8 * 8);
   }
-  const int items = ((a.n3 + 7) / 8) * 8 * nstates;
+  const int items = (((a.i3n ? a.i3n : a.n3) + 7) / 8) * 8 * nstates;
   const int nblk = std::max(8, std::min(resident[mi], (items + 7) / 8 * 8));
   hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, st, a.queue);
   dim3 grid(nblk);
@@ -724,6 +725,7 @@ static bool fold_stage(const CoilArgs& a) {
 }
 
 hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  if (a.i3lo < 0 || a.i3n < 0 || a.i3lo + (a.i3n ? a.i3n : a.n3) > a.n3) return hipErrorInvalidValue;
   if (fold_stage(a)) return launch_coil_fold(a, mode, nstates, st);
   if (!a.queue || a.n2 > 1024 || a.force_generic) return hipErrorInvalidValue;
   switch (reg_radix(a.n1)) {
@@ -747,6 +749,7 @@ int coil_chunk(int n1, int n2) {
 }
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
+  if (a.i3lo < 0 || a.i3n < 0 || a.i3lo + (a.i3n ? a.i3n : a.n3) > a.n3) return hipErrorInvalidValue;
   if (fold_stage(a)) return launch_coil_fold(a, mode, nstates, st);
   if (a.n2 <= 1024 && !a.force_generic) {
     switch (reg_radix(a.n1)) {

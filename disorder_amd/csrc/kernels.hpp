@@ -100,6 +100,10 @@ struct CoilArgs {
   FftPlan fy1pl;          // FFT plan and twiddles of length fy1; twiddles of fy2
   const float2* fy1tw;
   const float2* fy2tw;
+  // readout planes handled: i3lo .. i3lo + i3n - 1 (i3n = 0: every plane).
+  // Kernels that do not restrict the planes compute all of them; callers
+  // read only the range.
+  int i3lo, i3n;
 };
 
 enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 };
@@ -133,7 +137,8 @@ size_t coil_fold_work_bytes(int n1, int n2, int n3, int C, int ny1, int ny2, int
 hipError_t launch_bcast(const float2* x, float2* out, long long nvox, int S, hipStream_t st);
 hipError_t launch_sum_states(const float2* in, long long sstride, float2* out, long long nvox, int S,
                              int accumulate, hipStream_t st);
-hipError_t launch_energy_reduce(const float* epart, int Ns, int n3, const float* Mbe, double* out,
-                                hipStream_t st);
+// per-sample energy over readout planes [i3lo, i3hi), weighted by Mbe (or 1)
+hipError_t launch_energy_reduce(const float* epart, int Ns, int n3, const float* Mbe, int i3lo, int i3hi,
+                                double* out, hipStream_t st);
 
 }  // namespace dis

@@ -23,21 +23,23 @@ __constant__ int kHi[21] = {0, 1, 2, 3, 4, 5, 0, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2
 __constant__ int kHj[21] = {0, 1, 2, 3, 4, 5, 1, 2, 3, 4, 5, 2, 3, 4, 5, 3, 4, 5, 4, 5, 5};
 
 // grid (kLmBlocks, nlist): block b of listed state sl sums a slice of the
-// state's (sample, readout, coil) elements.
+// state's (sample, readout plane in [i3lo, i3lo + i3n), coil) elements.
 __global__ __launch_bounds__(kThreads) void lm_reduce_kernel(LmReduceArgs a) {
   const int sl = blockIdx.y;
   const int s = a.slist[sl];
   const int o0 = a.off[s], o1 = a.off[s + 1];
   const int nec = o1 - o0;
-  const long long tot = (long long)nec * a.n3 * a.C;
+  const long long tot = (long long)nec * a.i3n * a.C;
   double acc[kLmSums];
 #pragma unroll
   for (int q = 0; q < kLmSums; ++q) acc[q] = 0.0;
   for (long long idx = (long long)blockIdx.x * kThreads + threadIdx.x; idx < tot;
        idx += (long long)gridDim.x * kThreads) {
     const long long j = o0 + idx % nec;
-    const long long rest = idx / nec;  // i3 + n3 c
-    const long long yi = j + (long long)a.Ns * rest;
+    const long long rest = idx / nec;  // (i3 - i3lo) + i3n c
+    const long long c = rest / a.i3n;
+    const long long i3 = a.i3lo + (rest - c * a.i3n);
+    const long long yi = j + (long long)a.Ns * (i3 + (long long)a.n3 * c);
     const float2 Y = a.Yr[yi], d = a.y[yi];
     const float2 r = make_float2(Y.x - d.x, Y.y - d.y);
     acc[0] += (double)r.x * r.x + (double)r.y * r.y;
@@ -131,6 +133,7 @@ __global__ void lm_solve_kernel(const double* sums, const double* w, int nlist, 
 hipError_t launch_lm_reduce(const LmReduceArgs& a, int nlist, double* partial_out, double* sums_out,
                             hipStream_t st) {
   (void)partial_out;
+  if (a.i3lo < 0 || a.i3n <= 0 || a.i3lo + a.i3n > a.n3) return hipErrorInvalidValue;
   hipLaunchKernelGGL(lm_reduce_kernel, dim3(kLmBlocks, nlist), dim3(kThreads), 0, st, a);
   hipLaunchKernelGGL(lm_finalize_kernel, dim3(nlist), dim3(64), 0, st, a.partial, kLmBlocks, nlist, sums_out);
   return hipGetLastError();

@@ -15,6 +15,7 @@ struct LmReduceArgs {
   const int* off;        // [S+1]
   const int* slist;      // [nlist] states reduced
   int Ns, n3, C;
+  int i3lo, i3n;         // readout planes summed (E.nF of LMsolver.m:33,68,98)
   double* partial;       // [nlist][lm_partial_count()][kLmSums]
 };
 

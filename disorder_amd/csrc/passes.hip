@@ -653,14 +653,14 @@ hipError_t launch_sum_states(const float2* in, long long sstride, float2* out, l
   return hipGetLastError();
 }
 
-// per-sample energy: sum over planes (in fixed order) of the per-(sample,
-// plane) coil sums, weighted by the readout mask Mbe (or 1).
-__global__ void energy_reduce_kernel(const float* __restrict__ epart, int Ns, int n3, const float* __restrict__ Mbe,
-                                     double* __restrict__ out) {
+// per-sample energy: sum over planes i3lo..i3hi-1 (in fixed order) of the
+// per-(sample, plane) coil sums, weighted by the readout mask Mbe (or 1).
+__global__ void energy_reduce_kernel(const float* __restrict__ epart, int Ns, int i3lo, int i3hi,
+                                     const float* __restrict__ Mbe, double* __restrict__ out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < Ns) {
     double acc = 0.0;
-    for (int i3 = 0; i3 < n3; ++i3) {
+    for (int i3 = i3lo; i3 < i3hi; ++i3) {
       const double w = Mbe ? (double)Mbe[i3] : 1.0;
       acc += w * (double)epart[j + (long long)Ns * i3];
     }
@@ -668,8 +668,10 @@ __global__ void energy_reduce_kernel(const float* __restrict__ epart, int Ns, in
   }
 }
 
-hipError_t launch_energy_reduce(const float* epart, int Ns, int n3, const float* Mbe, double* out, hipStream_t st) {
-  hipLaunchKernelGGL(energy_reduce_kernel, dim3((Ns + 255) / 256), dim3(256), 0, st, epart, Ns, n3, Mbe, out);
+hipError_t launch_energy_reduce(const float* epart, int Ns, int n3, const float* Mbe, int i3lo, int i3hi, double* out,
+                                hipStream_t st) {
+  if (i3lo < 0 || i3hi > n3 || i3lo > i3hi) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(energy_reduce_kernel, dim3((Ns + 255) / 256), dim3(256), 0, st, epart, Ns, i3lo, i3hi, Mbe, out);
   return hipGetLastError();
 }
 

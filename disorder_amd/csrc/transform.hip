@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "lm.hpp"
+#include "lm_host.hpp"
 #include "transform.hpp"
 
 namespace dis {
@@ -514,7 +515,7 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
     }
     std::vector<double> h(ns, 0.0);
     if (Ns > 0) {
-      DCHK(c, launch_energy_reduce(epart, Ns, g.n[2], eo->Mbe ? (const float*)d : nullptr, esamp, stream));
+      DCHK(c, launch_energy_reduce(epart, Ns, g.n[2], eo->Mbe ? (const float*)d : nullptr, 0, g.n[2], esamp, stream));
       DCHK(c, hipMemcpyAsync(h.data(), esamp, sizeof(double) * Ns, hipMemcpyDeviceToHost, stream));
     }
     DCHK(c, hipStreamSynchronize(stream));
@@ -734,11 +735,18 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
   const int NM = g.nmotion;
   const long long nv = gr.nvox;
   const long long ny = (long long)g.nsamples * g.n[2] * g.ncoils;
-  const double NElY = (double)ny;
+  const double NElY = (double)ny;  // numel(y) before the readout crop (LMsolver.m:32-33)
+  // readout ROI E.nF (LMsolver.m:33,68,98): sums over planes [ro_lo, ro_hi)
+  const int ro_lo = v.roi[1] > 0 ? v.roi[0] : 0;
+  const int ro_hi = v.roi[1] > 0 ? v.roi[1] : g.n[2];
+  {
+    const int mc = lm_check_motion(NM, T, w);
+    if (mc) return ctx_fail(c, DIS_E_INVALID, lm_status_text(mc));
+  }
   std::vector<int> est;
   for (int s = 0; s < NM; ++s)
     if (!cT[s]) est.push_back(s);
-  std::vector<double> EnPrev(NM, 0.0), En(NM, 0.0), sums((size_t)NM * kLmSums, 0.0);
+  std::vector<double> EnPrev(NM, 0.0), sums((size_t)NM * kLmSums, 0.0);
   if (n_trials) *n_trials = 0;
   if (est.empty()) {
     for (int s = 0; s < NM; ++s) En_out[s] = 0.0;
@@ -814,6 +822,8 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
     a.plan1 = gr.plan[0];
     a.G = G16;
     a.slist = dlist;
+    a.i3lo = ro_lo;
+    a.i3n = ro_hi - ro_lo;
     (void)mode;
     return a;
   };
@@ -851,6 +861,8 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
     ra.Ns = g.nsamples;
     ra.n3 = g.n[2];
     ra.C = g.ncoils;
+    ra.i3lo = ro_lo;
+    ra.i3n = ro_hi - ro_lo;
     ra.partial = partial;
     DCHK(c, launch_lm_reduce(ra, sb, partial, dsums, stream));
     std::vector<double> h((size_t)sb * kLmSums);
@@ -863,62 +875,32 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
     std::copy(sums.begin(), sums.end(), sums_only);
     return DIS_OK;
   }
-  for (int s : est) EnPrev[s] = sums[(size_t)s * kLmSums] / NElY;
-  En = EnPrev;
-  // ---- damped steps, trial energies, accept / reject (LMsolver.m:126-215)
-  std::vector<double> dGEff((size_t)NM * 6, 0.0);
-  std::vector<unsigned char> flagw(NM, 0);
-  for (int s = 0; s < NM; ++s)
-    if (cT[s]) flagw[s] = 1;  // En = EnPrev = 0 for converged states
-  std::vector<double> Tup((size_t)NM * 6);
-  for (int iter = 0; iter < 10000; ++iter) {
-    // solves for the non-converged states with w < 1e10 (FP64, on the device)
-    std::vector<int> sol;
-    for (int s : est)
-      if (w[s] < 1e10) sol.push_back(s);
-    if (!sol.empty()) {
-      const int ns = (int)sol.size();
-      std::vector<double> hs((size_t)ns * kLmSums), hw(ns);
-      for (int i = 0; i < ns; ++i) {
-        for (int q = 0; q < kLmSums; ++q) hs[(size_t)i * kLmSums + q] = sums[(size_t)sol[i] * kLmSums + q];
-        hw[i] = w[sol[i]];
-      }
-      Staging st;
-      const size_t os = st.add(hs.data(), sizeof(double) * hs.size());
-      const size_t ow = st.add(hw.data(), sizeof(double) * ns);
-      const unsigned char* d = nullptr;
-      rc = ctx_upload_bytes(c, st, &d);
-      if (rc) return rc;
-      DCHK(c, launch_lm_solve((const double*)(d + os), (const double*)(d + ow), ns, winit, dstep, stream));
-      std::vector<double> hstep((size_t)ns * 6);
-      DCHK(c, hipMemcpyAsync(hstep.data(), dstep, sizeof(double) * hstep.size(), hipMemcpyDeviceToHost, stream));
-      DCHK(c, hipStreamSynchronize(stream));
-      for (int i = 0; i < ns; ++i) {
-        // a singular damped system (mldivide returns Inf/NaN in the
-        // reference) gives no step here
-        bool finite = true;
-        for (int gi = 0; gi < 6; ++gi) finite = finite && std::isfinite(hstep[(size_t)i * 6 + gi]);
-        for (int gi = 0; gi < 6; ++gi) dGEff[(size_t)sol[i] * 6 + gi] = finite ? hstep[(size_t)i * 6 + gi] : 0.0;
-      }
+  LmUpdate u;
+  u.NM = NM;
+  u.est = est;
+  u.EnPrev = EnPrev;
+  for (int s : est) u.EnPrev[s] = sums[(size_t)s * kLmSums] / NElY;
+  // damped 6x6 solves (FP64, on the device) for the listed states
+  auto solve = [&](const std::vector<int>& sol, std::vector<double>& steps) -> int {
+    const int ns = (int)sol.size();
+    std::vector<double> hs((size_t)ns * kLmSums), hw(ns);
+    for (int i = 0; i < ns; ++i) {
+      for (int q = 0; q < kLmSums; ++q) hs[(size_t)i * kLmSums + q] = sums[(size_t)sol[i] * kLmSums + q];
+      hw[i] = w[sol[i]];
     }
-    // Tup = restrictTransform(T + dGEff) (rotations into [-pi, pi])
-    for (int s = 0; s < NM; ++s)
-      for (int gi = 0; gi < 6; ++gi) {
-        double vv = T[s + (size_t)NM * gi] + dGEff[(size_t)s * 6 + gi];
-        if (gi >= 3) {
-          // restrictTransform.m:28-29 (repeated +-2 pi); far-out values are
-          // first reduced with fmod so the loop stays bounded
-          if (std::fabs(vv) > 64 * M_PI) vv = std::fmod(vv, 2 * M_PI);
-          while (vv > M_PI) vv -= 2 * M_PI;
-          while (vv < -M_PI) vv += 2 * M_PI;
-        }
-        Tup[s + (size_t)NM * gi] = vv;
-      }
-    // trial energies of the states neither converged nor accepted
-    std::vector<int> todo;
-    for (int s : est)
-      if (!flagw[s]) todo.push_back(s);
-    if (n_trials) *n_trials += (int)todo.size();
+    Staging st;
+    const size_t os = st.add(hs.data(), sizeof(double) * hs.size());
+    const size_t ow = st.add(hw.data(), sizeof(double) * ns);
+    const unsigned char* d = nullptr;
+    int r = ctx_upload_bytes(c, st, &d);
+    if (r) return r;
+    DCHK(c, launch_lm_solve((const double*)(d + os), (const double*)(d + ow), ns, winit, dstep, stream));
+    DCHK(c, hipMemcpyAsync(steps.data(), dstep, sizeof(double) * (size_t)ns * 6, hipMemcpyDeviceToHost, stream));
+    DCHK(c, hipStreamSynchronize(stream));
+    return DIS_OK;
+  };
+  // trial energies at Tup of the listed states
+  auto trial = [&](const std::vector<int>& todo, const std::vector<double>& Tup, std::vector<double>& Et) -> int {
     for (size_t b0 = 0; b0 < todo.size(); b0 += Sb) {
       const int sb = (int)std::min<size_t>(Sb, todo.size() - b0);
       std::vector<double> Tb((size_t)sb * 6);
@@ -927,50 +909,32 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
         lst[i] = todo[b0 + i];
         for (int gi = 0; gi < 6; ++gi) Tb[i + (size_t)sb * gi] = Tup[lst[i] + (size_t)NM * gi];
       }
-      rc = rigid_forward(c, gr, Tb.data(), sb, sb, x, 0, Tx, false);
-      if (rc) return rc;
+      int r = rigid_forward(c, gr, Tb.data(), sb, sb, x, 0, Tx, false);
+      if (r) return r;
       Staging st;
       const size_t ol = st.add(lst.data(), sizeof(int) * sb);
       const unsigned char* d = nullptr;
-      rc = ctx_upload_bytes(c, st, &d);
-      if (rc) return rc;
+      r = ctx_upload_bytes(c, st, &d);
+      if (r) return r;
       CoilArgs a = coil_args((const int*)(d + ol), Tx, COIL_ENERGY);
       DCHK(c, launch_coil(a, COIL_ENERGY, sb, stream));
-      DCHK(c, launch_energy_reduce(epart, g.nsamples, g.n[2], nullptr, esamp, stream));
+      DCHK(c, launch_energy_reduce(epart, g.nsamples, g.n[2], nullptr, ro_lo, ro_hi, esamp, stream));
       std::vector<double> he(std::max(1, g.nsamples));
       DCHK(c, hipMemcpyAsync(he.data(), esamp, sizeof(double) * g.nsamples, hipMemcpyDeviceToHost, stream));
       DCHK(c, hipStreamSynchronize(stream));
       for (int s : lst) {
         double e = 0;
         for (int j = v.hoff[s]; j < v.hoff[s + 1]; ++j) e += he[j];
-        En[s] = e / NElY;
+        Et[s] = e / NElY;
       }
     }
-    bool all = true;
-    for (int s = 0; s < NM; ++s) {
-      if (w[s] > 1e10) En[s] = EnPrev[s];
-      if (En[s] <= EnPrev[s]) flagw[s] = 1;
-      if (!flagw[s]) all = false;
-    }
-    if (!all) {
-      for (int s = 0; s < NM; ++s)
-        if (En[s] > EnPrev[s] && !cT[s]) w[s] *= 2.0;
-      continue;
-    }
-    for (int s = 0; s < NM; ++s) {
-      if (!cT[s]) w[s] /= 1.2;
-      if (w[s] < 1e-8) w[s] *= 1.2;
-    }
-    for (size_t k = 0; k < Tup.size(); ++k) T[k] = Tup[k];
-    for (int s = 0; s < NM; ++s) {
-      double tra = 0, rot = 0;
-      for (int gi = 0; gi < 3; ++gi) tra = std::max(tra, std::fabs(dGEff[(size_t)s * 6 + gi]));
-      for (int gi = 3; gi < 6; ++gi) rot = std::max(rot, std::fabs(dGEff[(size_t)s * 6 + gi] * 180.0 / M_PI));
-      if (tra < tL[0] && rot < tL[1]) cT[s] = 1;
-    }
-    break;
-  }
-  for (int s = 0; s < NM; ++s) En_out[s] = std::min(EnPrev[s], En[s]);
+    return DIS_OK;
+  };
+  int cb_rc = DIS_OK;
+  const int lst = lm_update(u, T, w, cT, tL, solve, trial, n_trials, &cb_rc);
+  if (lst == LM_CALLBACK) return cb_rc;
+  if (lst) return ctx_fail(c, DIS_E_INVALID, lm_status_text(lst));
+  for (int s = 0; s < NM; ++s) En_out[s] = std::min(u.EnPrev[s], u.En[s]);
   return DIS_OK;
 }
 

@@ -154,8 +154,8 @@ def lm_sums(y, E, x, T):
     return sums
 
 
-def state_energy(E, x, y, T, off):
-    ps = E.energy(x, y, T, per_sample=True)
+def state_energy(E, x, y, T, off, Mbe=None):
+    ps = E.energy(x, y, T, Mbe=Mbe, per_sample=True)
     return np.add.reduceat(np.asarray(ps, np.float64), off[:-1]) if len(ps) else np.zeros(len(off) - 1)
 
 
@@ -182,9 +182,15 @@ class Proposer:
         self.alphas = tuple(alphas)
         self.max_outer = max_outer
         self.retire = retire
-        self.retired = False
+        self.reset()
         self.accepted = 0
         self.proposed = 0
+
+    def reset(self):
+        """Start of a solve: proposals resume (pyramid.solveXT calls this
+        once per solve, so a proposer retired in one solve proposes again in
+        the next); the accepted/proposed counters keep running."""
+        self.retired = False
         self._level = None
         self._calls = 0
 
@@ -213,7 +219,8 @@ class Proposer:
                 break
             Tp = T + a * d
             Tp[:, 3:] = (Tp[:, 3:] + math.pi) % (2 * math.pi) - math.pi  # restrictTransform.m:28-29
-            e_new = state_energy(L.E, x, L.y, Tp, L.off)
+            # over the readout ROI, like the LM energies in sums[:, 0]
+            e_new = state_energy(L.E, x, L.y, Tp, L.off, Mbe=getattr(L, "Mbe", None))
             ok = todo & (e_new < sums[:, 0])
             Tn[ok] = Tp[ok]
             take |= ok

@@ -234,6 +234,13 @@ class Encoding:
             self.We = np.ascontiguousarray(We, dtype=np.float32)
             check(self.ctx.lib.dis_plan_set_weights(self.h, self.We.ctypes.data_as(_lib.c_float_p)), self.ctx.h)
 
+    def set_readout_roi(self, nF):
+        """E.nF (LMsolver.m:33,68,98): the readout planes [lo, hi) (0-based)
+        over which LMsolver and lm_partials sum; None for every plane."""
+        lo, hi = (0, 0) if nF is None else (int(nF[0]), int(nF[1]))
+        check(self.ctx.lib.dis_plan_set_readout_roi(self.h, lo, hi), self.ctx.h)
+        self.nF = None if nF is None else (lo, hi)
+
     def _T(self, T):
         if self.nmotion == 0:
             return np.zeros(6), None

@@ -14,9 +14,9 @@ repeat), driving the C-ABI operators level by level:
     * w = winit, cT reset; states with fewer than echoesMin samples frozen
       (:313-326); tL = traLimXT * resIso(l) (:370);
   each outer iteration: motion-convergence reset schedule (:376-381), CG with
-  'RelativeResidual2Error', tol 0 and n_cg iterations (:431), one LMsolver
-  call (:507); when every state has converged a final 'Energy' CG closes the
-  level (:425-431, :523);
+  'RelativeResidual2Error', tol 0 and the nX iterations of :417-425 (:431),
+  one LMsolver call on the readout ROI E.nF (:345, :507); when every state
+  has converged a final 'Energy' CG closes the level (:425-431, :523);
   at the full-resolution level (AlignedRec = 2, leff = 0) the per-sample
   residuals of computeEnergy after each CG give robust per-shot weights
   (disorder_amd.robust, :441-481) that decode and the later CG calls of the
@@ -49,6 +49,43 @@ def level_size(N, f):
     return [int(math.ceil(v / 2 ** f)) for v in N]
 
 
+def readout_roi(n3, fat_shift_dir="H", redFOV=1.0 / 3.0):
+    """E.nF of Reconstruction/solveXT.m:345 as a 0-based range [lo, hi):
+    fat shift 'F' keeps the first floor((1 - redFOV) n3) readout planes,
+    'H' the planes floor(1 + redFOV n3) .. n3 (1-based), any other
+    direction every plane (the reference then prints that performance is
+    probably suboptimal)."""
+    if fat_shift_dir == "F":
+        return 0, int(math.floor((1.0 - redFOV) * n3))
+    if fat_shift_dir == "H":
+        return int(math.floor(1.0 + redFOV * n3)) - 1, int(n3)
+    return 0, int(n3)
+
+
+def cg_iterations(li, it, sizes, nX_prev, nIt2):
+    """CG iterations nX of an outer iteration while motion is estimated
+    (Reconstruction/solveXT.m:417-425): nIt(2) on the first outer iteration
+    of the first level and one fewer on each later one (at least 1); at a
+    new resolution nIt(2) - 1, then one fewer each time; at a repeated
+    resolution (the within-shot levels) none on the first outer iteration,
+    nIt(2) - 2 on the second, then nX - (it - 2).  li: level (0-based),
+    it: outer iteration (1-based), sizes: level grids."""
+    if li == 0 and it == 1:
+        return nIt2
+    if li == 0:
+        return max(nX_prev - 1, 1)
+    same = list(sizes[li]) == list(sizes[li - 1])
+    if not same and it == 1:
+        return max(nIt2 - 1, 1)
+    if not same:
+        return max(nX_prev - 1, 1)
+    if it == 1:
+        return 0
+    if it == 2:
+        return max(nIt2 - 2, 1)
+    return max(nX_prev - (it - 2), 1)
+
+
 def pyramid_levels(N, nlevels):
     """[(n_l, resIso_l)] from coarse to fine (pyramidPlan.m:19-25, 1 mm voxels)."""
     out = []
@@ -66,7 +103,8 @@ class Level:
     pair (lm_coils); image solves ask cg(ncoil, T) for their encoding,
     motion, data and preconditioner."""
 
-    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None, gibbs=0.0, NY=None, ny=None):
+    def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None, gibbs=0.0, NY=None, ny=None,
+                 nF=None):
         self.n = list(n)
         # acquired phase-encode grid (full and at this level); below the
         # image grid the level's encodings fold (SENSE, E.Uf / EH.Ub)
@@ -90,6 +128,14 @@ class Level:
         self._Se = {}
         lm = C if lm_coils is None else lm_coils
         self.E, self.y = self.encoding(lm), self.data(lm)
+        # readout ROI of the motion estimation (E.nF) and the readout weights
+        # of the energies (EH.Mbe, solveXT.m:345-346)
+        self.nF = None if nF is None or tuple(nF) == (0, n[2]) else (int(nF[0]), int(nF[1]))
+        self.E.set_readout_roi(self.nF)
+        self.Mbe = None
+        if self.nF is not None:
+            self.Mbe = np.zeros(n[2], np.float32)
+            self.Mbe[self.nF[0]:self.nF[1]] = 1.0
 
     def _check(self, ncoil):
         if not 1 <= ncoil <= self.C:
@@ -147,7 +193,7 @@ class Level:
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
             robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None,
-            gibbs=0.05, accel=0, NY=None):
+            gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0):
     """Returns (x [N] on the device, T [states x 6], info).  callback(level,
     it, x, T, t_solve) runs after every outer iteration (and after a level's
     final CG, with outer = last + 1) outside the timed region
@@ -189,7 +235,15 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     smaller than N (SENSE folding, E.Uf / EH.Ub of solveXT.m:198-200,
     :362-364): k1/k2 index it, each level's data live on ceil(NY/2^l)
     (downsampleOperators.m:34) and its encodings fold the level image onto
-    that grid."""
+    that grid.
+    cg_schedule: 'nX' (default) runs the CG iterations of solveXT.m:417-425
+    with nIt(2) = n_cg (cg_iterations; no CG on the first outer iteration
+    of a repeated full-resolution level), 'fixed' runs n_cg every time.
+    fat_shift_dir ('H' by default, IO/disorderAlgorithm.m:71) and redFOV
+    (1/3, :97): the readout ROI E.nF of every LMsolver call and the readout
+    weights EH.Mbe of the energies behind the robust weights
+    (solveXT.m:345-346, :443); a direction other than 'F' or 'H' keeps
+    every readout plane."""
     import torch
     from . import trajectory as TJ
     from .sampling import harmonic_sampling_repeats
@@ -207,6 +261,10 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
         raise ValueError("eivaS holds three coil counts")
     if n_within and traj is None:
         raise ValueError("within-shot levels need the trajectory (traj)")
+    if cg_schedule not in ("nX", "fixed"):
+        raise ValueError("cg_schedule is 'nX' or 'fixed'")
+    if proposer is not None and hasattr(proposer, "reset"):
+        proposer.reset()
     # state bookkeeping for within-shot subdivision (solveXT.m:270-287)
     sweep_of_state = np.arange(nsweeps)
     k1c, k2c, offc, rows0 = k1, k2, np.asarray(off), None
@@ -246,7 +304,8 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
         subT = np.zeros(len(sweep_of_state), bool)
         nstates = len(sweep_of_state)
         ny = None if NY is None else level_size(NY[:2], max(0, nlevels - 1 - li))
-        L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0, gibbs=gibbs, NY=NY, ny=ny)
+        L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0, gibbs=gibbs, NY=NY, ny=ny,
+                  nF=readout_roi(n[2], fat_shift_dir, redFOV))
         full_res = list(n) == list(N)
         sweep_of_sample = sweep_of_state[np.repeat(np.arange(nstates), np.diff(L.off))]
         if x is None:
@@ -260,10 +319,11 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
         tL = (traLimXT[0] * resIso, traLimXT[1] * resIso)
         nReset, cont = 0, 0
         its = 0
+        nX = n_cg
 
         def weights(x, final):
             Ec, Tc, yc, _ = cg_ops(L, T, final)
-            ps = ops.computeEnergy(yc, x, Ec, Tc, per_sample=True)
+            ps = ops.computeEnergy(yc, x, Ec, Tc, per_sample=True, Mbe=L.Mbe)
             We, _ = RB.shot_weights(ps, sweep_of_sample, nsweeps, finer=leff > 0)
             L.set_weights(RB.sample_weights(We, sweep_of_sample))
             info["We"] = We
@@ -301,7 +361,9 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                 cT = cT | ~subTT
             cont += 1
             Ec, Tc, yc, Se = cg_ops(L, T, False)
-            x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0, Se=Se)
+            nX = cg_iterations(li, it, [lv[0] for lv in levels], nX, n_cg) if cg_schedule == "nX" else n_cg
+            if nX:
+                x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=nX, tolType="RelativeResidual2Error", tol=0.0, Se=Se)
             if robust and full_res:
                 weights(x, False)
             if proposer is not None:

@@ -231,18 +231,25 @@ def grid_weights(k1, k2, We, n12):
     return W
 
 
-def trace_estimation(sample, shape, rng, device, tol=1e-2, block=4):
+def trace_estimation(sample, shape, rng, device, tol=1e-2, block=4, max_probes=4096):
     """traceEstimation.m:52-110: mean of sample(probe) over blocks of
     `block` real N(0,1) probes normalised to unit mean square, until
-    std/sqrt(n) < tol * mean (at least two probes).  Returns (mean, n)."""
+    std/sqrt(n) < tol * mean (at least two probes).  Returns (mean, n).
+    A probe value that is not finite, or max_probes probes without
+    reaching the tolerance, raises (the reference's loop would not end)."""
     import torch
     vals = []
     while True:
+        if len(vals) >= max_probes:
+            raise RuntimeError(f"trace estimation did not reach tol {tol} in {max_probes} probes")
         for _ in range(block):
             z = rng.standard_normal(shape)
             z = z / math.sqrt(float(np.sum(z * z)) / z.size)
             zt = ops.colmajor(torch.from_numpy(np.asfortranarray(z.astype(np.complex64))).to(device))
-            vals.append(sample(zt))
+            v = float(sample(zt))
+            if not math.isfinite(v):
+                raise RuntimeError("trace estimation: probe value is not finite")
+            vals.append(v)
         n = len(vals)
         m = float(np.mean(vals))
         if n >= 2 and float(np.std(vals, ddof=1)) / math.sqrt(n) / (m + np.finfo(np.float64).eps) < tol:

@@ -106,6 +106,15 @@ int dis_plan_set_weights(dis_plan* plan, const float* We);
  * encodedecode, energy, CG, LM) then runs folded. */
 int dis_plan_set_fold(dis_plan* plan, int32_t ny1, int32_t ny2);
 
+/* Readout region of interest of the motion estimation (E.nF of
+ * Reconstruction/LMsolver.m:33,68,98, set per level by solveXT.m:345 from
+ * FatShiftDir and redFOV): dis_lm and dis_lm_partials sum residuals,
+ * gradients and Hessians over readout planes lo .. hi-1 (0-based; MATLAB
+ * E.nF = lo+1:hi) only, still dividing by numel(y) of the whole data
+ * (LMsolver.m:32).  lo = hi = 0 (or the full range) restores every plane.
+ * Only a contiguous range is supported (solveXT builds E.nF as one). */
+int dis_plan_set_readout_roi(dis_plan* plan, int32_t lo, int32_t hi);
+
 /* y = E x  (Reconstruction/encode.m:1).  T: host [nmotion x 6]. */
 int dis_encode(dis_plan* plan, const double* T, const void* x, void* y);
 /* x = E^H (We .* y)  (Reconstruction/decode.m:1). */

@@ -764,14 +764,39 @@ def cg_solver(y, Smaps, samp, T, Nfull, x0=None, nIt=300, tolType="Energy", tol=
 # ----------------------------------------------------------------------------
 # LM (Reconstruction/LMsolver.m:43-225)
 # ----------------------------------------------------------------------------
-def lm_partials(y, x, Smaps, samp, T, Nfull, states=None):
+def readout_roi(n3, fat_shift_dir="H", redFOV=1.0 / 3.0):
+    """E.nF of Reconstruction/solveXT.m:345 as a 0-based half-open range
+    [lo, hi): 'F' keeps 1:floor((1-redFOV) n3), 'H' keeps
+    floor(1+redFOV n3):n3 (1-based, inclusive), anything else every plane."""
+    if fat_shift_dir == "F":
+        return 0, int(math.floor((1.0 - redFOV) * n3))
+    if fat_shift_dir == "H":
+        return int(math.floor(1.0 + redFOV * n3)) - 1, int(n3)
+    return 0, int(n3)
+
+
+def _crop_readout(a, nF, axis):
+    """dynInd(a, E.nF, axis) for a range nF = (lo, hi) (LMsolver.m:33,68,98)."""
+    if nF is None:
+        return a
+    idx = [slice(None)] * a.ndim
+    idx[axis] = slice(nF[0], nF[1])
+    return a[tuple(idx)]
+
+
+def lm_partials(y, x, Smaps, samp, T, Nfull, states=None, nF=None):
     """Per-state residual energy, gradient dG and Gauss-Newton Hessian dH
     (Reconstruction/LMsolver.m:52-122), all states transformed (NMs = S).
-    Returns EnPrev (S,), dG (S,6), dH (S,21); energies are divided by numel(y)."""
+    nF: readout ROI (lo, hi) or None: the maps and data are cropped to it
+    (:33), and so are the transformed image (:68) and its derivatives (:98)
+    before encoding; energies are still divided by numel(y) of the uncropped
+    data (:32).  Returns EnPrev (S,), dG (S,6), dH (S,21)."""
     T = np.atleast_2d(np.asarray(T, np.float64))
     S = T.shape[0]
     states = range(S) if states is None else states
     NElY = y.size
+    Smaps = _crop_readout(Smaps, nF, 2)
+    y = _crop_readout(y, nF, 1)
     n = x.shape
     En = np.zeros(S)
     dG = np.zeros((S, 6))
@@ -782,9 +807,9 @@ def lm_partials(y, x, Smaps, samp, T, Nfull, states=None):
         G = sinc_rigid_transform_gradient(xB, f)
         sub = _one_state(samp, s)
         Tz = np.zeros((1, 6))
-        r = encode(xt[..., 0], Smaps, sub, Tz, Nfull) - y[samp.state(s)[0]]
+        r = encode(_crop_readout(xt[..., 0], nF, 2), Smaps, sub, Tz, Nfull) - y[samp.state(s)[0]]
         En[s] = float(np.sum(np.abs(r) ** 2)) / NElY
-        J = [encode(G[g][..., 0], Smaps, sub, Tz, Nfull) for g in range(6)]
+        J = [encode(_crop_readout(G[g][..., 0], nF, 2), Smaps, sub, Tz, Nfull) for g in range(6)]
         for g in range(6):
             dG[s, g] = float(np.sum(np.real(J[g] * np.conj(r))))
         for h, (i, j) in enumerate(HA):
@@ -798,13 +823,18 @@ def _one_state(samp, s):
                     None if samp.We is None else samp.We[sl])
 
 
-def lm_state_energy(y, x, Smaps, samp, T, Nfull, states):
+def lm_state_energy(y, x, Smaps, samp, T, Nfull, states, nF=None):
+    """Trial energies of LMsolver.m:151-192: transform (:166), readout crop
+    (:167), encode minus the cropped data (:171), over numel(y) (:32)."""
     T = np.atleast_2d(np.asarray(T, np.float64))
+    NElY = y.size
+    Sc, yc = _crop_readout(Smaps, nF, 2), _crop_readout(y, nF, 1)
     out = {}
     for s in states:
         sub = _one_state(samp, s)
-        r = encode(x, Smaps, sub, T[s:s + 1], Nfull) - y[samp.state(s)[0]]
-        out[s] = float(np.sum(np.abs(r) ** 2)) / y.size
+        xt = rigid_forward(x, T[s:s + 1], x.shape, Nfull)[..., 0] if np.any(T[s] != 0) else x
+        r = encode(_crop_readout(xt, nF, 2), Sc, sub, np.zeros((1, 6)), Nfull) - yc[samp.state(s)[0]]
+        out[s] = float(np.sum(np.abs(r) ** 2)) / NElY
     return out
 
 
@@ -825,15 +855,15 @@ def lm_step(dH, dG, w, winit):
     return step if np.all(np.isfinite(step)) else np.zeros(6)
 
 
-def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None):
-    """Reconstruction/LMsolver.m:43-225 for one outer call.  Returns the new
-    (T, w, cT, Enmin)."""
+def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None, nF=None):
+    """Reconstruction/LMsolver.m:43-225 for one outer call, with the readout
+    ROI E.nF = nF (lo, hi) or None.  Returns the new (T, w, cT, Enmin)."""
     T = np.array(np.atleast_2d(T), np.float64, copy=True)
     w = np.array(w, np.float64, copy=True)
     cT = np.array(cT, bool, copy=True)
     S = T.shape[0]
     est = [s for s in range(S) if not cT[s]]
-    EnPrev, dG, dH = lm_partials(y, x, Smaps, samp, T, Nfull, est)
+    EnPrev, dG, dH = lm_partials(y, x, Smaps, samp, T, Nfull, est, nF=nF)
     En = EnPrev.copy()
     dGEff = np.zeros((S, 6))
     flagw = np.zeros(S, bool)
@@ -844,7 +874,7 @@ def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None):
         Tup = restrict_transform(T + dGEff)
         cfull = cT | flagw
         todo = [s for s in range(S) if not cfull[s]]
-        for s, e in lm_state_energy(y, x, Smaps, samp, Tup, Nfull, todo).items():
+        for s, e in lm_state_energy(y, x, Smaps, samp, Tup, Nfull, todo, nF=nF).items():
             En[s] = e
         big = w > 1e10
         En[big] = EnPrev[big]
@@ -1191,10 +1221,32 @@ def pyramid_levels(N, nlevels):
     return out
 
 
+def cg_iterations(li, it, sizes, nX_prev, nIt2):
+    """nX of Reconstruction/solveXT.m:417-425 while motion is estimated:
+    level li (0-based, l = li + 1), outer iteration it (1-based), sizes[l]
+    the level grids (resPyr(l) == resPyr(l-1) exactly when the grid
+    repeats), nX_prev the value of the previous outer iteration of the
+    level, nIt2 = nIt(2)."""
+    if li == 0 and it == 1:
+        return nIt2
+    if li == 0:
+        return max(nX_prev - 1, 1)
+    same = list(sizes[li]) == list(sizes[li - 1])
+    if not same and it == 1:
+        return max(nIt2 - 1, 1)
+    if not same:
+        return max(nX_prev - 1, 1)
+    if it == 1:
+        return 0
+    if it == 2:
+        return max(nIt2 - 2, 1)
+    return max(nX_prev - (it - 2), 1)
+
+
 def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
                   traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
                   info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None, n_within=0, traj=None,
-                  gibbs=0.05, accel=0, NY=None):
+                  gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -1229,7 +1281,12 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     the outlier flags.  NY: the acquired phase-encode grid when it is smaller
     than N (SENSE folding, solveXT.m:198-200, :362-364): k1/k2 index it, each
     level's data live on ceil(NY/2^l) (downsampleOperators.m:34) and its
-    encoding folds the level image onto that grid.
+    encoding folds the level image onto that grid.  cg_schedule: 'nX' runs
+    the CG iterations of solveXT.m:417-425 with nIt(2) = n_cg (none on the
+    first outer iteration of a repeated full-resolution level), 'fixed' runs
+    n_cg every time.  fat_shift_dir, redFOV: the readout ROI E.nF of LMsolver
+    and the readout weights EH.Mbe of the energies behind the robust weights
+    (solveXT.m:345-346, :443).
     Returns (x at full size, T)."""
     from oracle import trajectory_oracle as TO
     NSw = len(off) - 1
@@ -1292,10 +1349,14 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
             Tb, offb, _ = compress_motion(T, loff, res_fine, traLimX)
             return Sampling(l1, l2, offb, ny[0], ny[1], We), Tb
 
+        nF = readout_roi(n[2], fat_shift_dir, redFOV)
+        Mbe = np.zeros(n[2])
+        Mbe[nF[0]:nF[1]] = 1.0
+
         def weights(c, finer):
             samp_b, Tb = binned(T, samp.We)
             ps = compute_energy(yl[..., :c], x, Sl[..., :c], Sampling(l1, l2, samp_b.off, ny[0], ny[1]), Tb, N,
-                                per_sample=True)
+                                per_sample=True, Mbe=Mbe)
             Wshot = robust_shot_weights(ps, sweep_of_sample, NSw, finer=finer)
             samp.We = Wshot[sweep_of_sample]
             if info is not None:
@@ -1310,6 +1371,7 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
         cT = cT0.copy()
         tL = (traLimXT[0] * resIso, traLimXT[1] * resIso)
         nReset, cont = 0, 0
+        nX = n_cg
         if li >= len(levels) - int(accel):
             if robust and leff > 0 and last_W is not None:
                 samp.We = last_W[sweep_of_sample]
@@ -1340,11 +1402,14 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
             cont += 1
             samp_b, Tb = binned(T, samp.We)
             c = ev[2]
-            x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=n_cg,
-                                tolType="RelativeResidual2Error", tol=0.0, Se=Se)
+            nX = cg_iterations(li, it, [lv[0] for lv in levels], nX, n_cg) if cg_schedule == "nX" else n_cg
+            if nX:
+                x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=nX,
+                                    tolType="RelativeResidual2Error", tol=0.0, Se=Se)
             if robust and full_res:
                 weights(c, leff > 0)
-            T, w, cT, _ = lm_solver(yl[..., :ev[1]], x, Sl[..., :ev[1]], samp, T, N, w, cT, winit, tL)
+            T, w, cT, _ = lm_solver(yl[..., :ev[1]], x, Sl[..., :ev[1]], samp, T, N, w, cT, winit, tL,
+                                    nF=None if nF == (0, n[2]) else nF)
             if callback is not None:
                 callback(li, it, x, T)
             if cT.all():

@@ -93,6 +93,42 @@ def test_rigid_properties_full_size(dev):
     assert (torch.linalg.vector_norm(back - x) / nx).item() < 1e-5
 
 
+def test_rigid_rotation_sense_and_order_against_analytic(dev):
+    """The HIP transform on an anisotropic Gaussian blob with all three
+    angles and a translation equals the analytically moved blob: output at
+    r = input at R4 R5 R6 (r - t), R_m rotating plane (1,2), (3,1), (2,3)
+    as (r_a, r_b) -> (cos th r_a + sin th r_b, -sin th r_a + cos th r_b)
+    (sincRigidTransform.m:42-72 with the factors of
+    precomputeFactorsSincRigidTransform.m:58-60); the reversed order and the
+    opposite sense miss."""
+    import math
+    import disorder_amd as D
+    n = (64, 48, 40)
+    g = np.meshgrid(*[np.arange(1, v + 1) - math.ceil((v + 1) / 2) for v in n], indexing="ij")
+    sig, c = (4.0, 4.5, 3.5), (2.0, -1.5, 1.0)
+
+    def blob(r):
+        return np.exp(-sum((r[d] - c[d]) ** 2 / (2 * sig[d] ** 2) for d in range(3)))
+
+    t, th = np.array([0.7, -1.2, 0.4]), (0.12, -0.09, 0.15)
+    y = to_np(D.sincRigidTransform(to_dev(blob(g), dev), np.array([[*t, *th]]), di=1))[..., 0]
+
+    def chain(order, sgn):
+        q = [g[d] - t[d] for d in range(3)]
+        for m in order:
+            a, b = ((0, 1), (2, 0), (1, 2))[m]
+            qa, qb = q[a], q[b]
+            q = list(q)
+            q[a] = np.cos(sgn * th[m]) * qa + np.sin(sgn * th[m]) * qb
+            q[b] = -np.sin(sgn * th[m]) * qa + np.cos(sgn * th[m]) * qb
+        return blob(q)
+
+    good = np.abs(y - chain((2, 1, 0), 1)).max()
+    assert good < 5e-4, good
+    assert np.abs(y - chain((0, 1, 2), 1)).max() > 10 * good
+    assert np.abs(y - chain((2, 1, 0), -1)).max() > 0.05
+
+
 def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
     x = smooth_volume(rng, n) + 0.1 * crandn(rng, n)
     Sm = crandn(rng, n + (C,)) * 0.5
@@ -247,9 +283,13 @@ def test_gradient_matches_oracle(dev, n, Nfull, flip):
 
 
 @pytest.mark.parametrize("n,C,S", [((16, 12, 8), 3, 4), ((32, 24, 10), 4, 4)])
-def test_lm_partials_and_step_match_oracle(dev, n, C, S):
+@pytest.mark.parametrize("fat", [None, "H", "F"])
+def test_lm_partials_and_step_match_oracle(dev, n, C, S, fat):
     """LMsolver: per-state energy, gradient and Hessian sums, then one full
-    accept/reject update, against the FP64 oracle."""
+    accept/reject update, against the FP64 oracle; with a readout ROI E.nF
+    (LMsolver.m:33,68,98) from fat-shift direction 'H' or 'F' the sums run
+    over that readout slab only (the (32, 24, 10) grid on the register coil
+    path, whose launches then cover only the slab's planes)."""
     import disorder_amd as D
     rng = np.random.default_rng(41)
     x, Sm, samp, _, _, Nf = _problem(rng, n, C, S)
@@ -257,17 +297,22 @@ def test_lm_partials_and_step_match_oracle(dev, n, C, S):
     y = O.encode(x, Sm, samp, Tgt, Nf) + 0.01 * crandn(rng, (samp.Ns, n[2], C))
     T0 = Tgt + random_motion(rng, S, 1.0, 0.3)
     E = D.Encoding(to_dev(Sm, dev), samp.k1, samp.k2, samp.off, Nfull=Nf)
+    nF = None if fat is None else O.readout_roi(n[2], fat)
+    E.set_readout_roi(nF)
     xd, yd = to_dev(x, dev), to_dev(y, dev)
     En, dG, dH = D.lm_partials(yd, E, xd, T0)
-    Er, dGr, dHr = O.lm_partials(y, x, Sm, samp, T0, Nf)
+    Er, dGr, dHr = O.lm_partials(y, x, Sm, samp, T0, Nf, nF=nF)
     assert rel_l2(En, Er) < 1e-5
     assert rel_l2(dG, dGr) < 1e-4
     assert rel_l2(dH, dHr) < 1e-4
+    if nF is not None:  # the slab changes the sums: the ROI is not a no-op
+        Ef, _, _ = O.lm_partials(y, x, Sm, samp, T0, Nf)
+        assert rel_l2(En, Ef) > 0.05
     w = np.full(S, 1e-3)
     cT = np.zeros(S, bool)
     cT[1] = True
     Tg, wg, cg, Eg, _ = D.LMsolver(yd, E, xd, T0, w, cT, winit=1e-3, tL=(0.05, 0.02))
-    Tr, wr, cr, Ern = O.lm_solver(y, x, Sm, samp, T0, Nf, w, cT, 1e-3, (0.05, 0.02))
+    Tr, wr, cr, Ern = O.lm_solver(y, x, Sm, samp, T0, Nf, w, cT, 1e-3, (0.05, 0.02), nF=nF)
     assert np.allclose(wg, wr) and np.array_equal(cg, cr)
     assert np.abs(Tg - Tr).max() < 1e-5
     assert np.allclose(Eg, Ern, rtol=1e-5, atol=1e-12)

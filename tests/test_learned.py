@@ -222,7 +222,7 @@ def test_proposer_retires_after_a_round_that_keeps_nothing(monkeypatch):
         s[:, 7:13] = 1.0
         return s
 
-    def fake_energy(E, x, y, T, off):
+    def fake_energy(E, x, y, T, off, Mbe=None):
         calls["energy"] += 1
         return np.full(S, 0.5 if drop["on"] else 2.0)
 
@@ -251,3 +251,8 @@ def test_proposer_retires_after_a_round_that_keeps_nothing(monkeypatch):
     Q(L1, torch.zeros(1), T, cT)
     Q(L2, torch.zeros(1), T, cT)
     assert Q.proposed == 2 * S and not Q.retired
+    # a new solve (pyramid.solveXT calls reset first) proposes again
+    P.reset()
+    drop["on"] = True
+    _, take = P(L1, torch.zeros(1), T, cT)
+    assert take.all() and not P.retired and P.proposed == 3 * S

@@ -80,15 +80,14 @@ def test_quarter_turn_is_permutation():
     T = np.zeros((1, 6))
     T[0, 3] = np.pi / 2  # rotation in the (1,2) plane
     y = O.rigid_forward(x, T, n, n)[..., 0]
-    # find the index permutation of the first plane and check it is a permutation
-    err = []
-    cands = []
-    for f in (lambda a: np.rot90(a, 1, axes=(0, 1)), lambda a: np.rot90(a, -1, axes=(0, 1))):
-        for s1 in range(8):
-            for s2 in range(8):
-                cands.append(np.roll(f(x), (s1, s2), axis=(0, 1)))
-    err = min(np.abs(y - c).max() for c in cands)
-    assert err < 1e-10
+    # the sense derived in test_rotation_of_smooth_blob_matches_analytic:
+    # y(r1, r2) = x(r2, -r1) about the centre; np.rot90(x, 1)[i, j] =
+    # x[j, n - 1 - i] is that up to a circular shift; the other sense must
+    # not fit
+    for k, want in ((1, True), (-1, False)):
+        cands = [np.roll(np.rot90(x, k, axes=(0, 1)), (s1, s2), axis=(0, 1)) for s1 in range(8) for s2 in range(8)]
+        err = min(np.abs(y - c).max() for c in cands)
+        assert (err < 1e-10) == want, (k, err)
 
 
 def test_rotation_of_smooth_blob_matches_analytic():
@@ -107,10 +106,61 @@ def test_rotation_of_smooth_blob_matches_analytic():
     T = np.zeros((1, 6))
     T[0, 3] = th
     y = O.rigid_forward(x, T, n, n)[..., 0]
-    best = min(
-        np.abs(y - blob(np.cos(s * th) * g[0] + np.sin(s * th) * g[1], -np.sin(s * th) * g[0] + np.cos(s * th) * g[1])).max()
-        for s in (1, -1))
-    assert best < 1e-4
+    err = {s: np.abs(y - blob(np.cos(s * th) * g[0] + np.sin(s * th) * g[1],
+                              -np.sin(s * th) * g[0] + np.cos(s * th) * g[1])).max()
+           for s in (1, -1)}
+    # the sense follows from the shear factors of
+    # precomputeFactorsSincRigidTransform.m:58-60 with di = 1 (sigma = -1):
+    # e2 = exp(+i tan(th/2) r2 k1) moves dim 1 by +tan(th/2) r2
+    # (y(r) = x(r1 + tan(th/2) r2, r2)), e3 = exp(-i sin(th) r1 k2) gives
+    # y(r) = x(r1, r2 - sin(th) r1); shear-shear-shear composes to
+    # y(r) = x(cos th r1 + sin th r2, -sin th r1 + cos th r2)
+    assert err[1] < 1e-4 and err[-1] > 0.1, err
+
+
+def _rot(g, a, b, th):
+    """Coordinates after the rotation of plane (a, b) that the three shears
+    of sincRigidTransform.m:42-54 apply: (r_a, r_b) -> (cos th r_a +
+    sin th r_b, -sin th r_a + cos th r_b), as derived above for (1, 2)."""
+    out = list(g)
+    out[a] = np.cos(th) * g[a] + np.sin(th) * g[b]
+    out[b] = -np.sin(th) * g[a] + np.cos(th) * g[b]
+    return out
+
+
+def test_rotation_composition_order_and_translation():
+    """All three angles and a translation on an anisotropic blob against the
+    analytic result.  sincRigidTransform.m:42-72 applies parameter 4 (plane
+    (1,2): p1 = 1, p2 = 2), then 5 (plane (3,1)), then 6 (plane (2,3)),
+    then the translation, so the output at r samples the input at
+    R4 R5 R6 (r - t) (each R_m the coordinate map of _rot); the reversed
+    order and the opposite sense both miss by far more than the tolerance."""
+    n = (40, 40, 40)
+    g = np.meshgrid(*[np.arange(1, v + 1) - math.ceil((v + 1) / 2) for v in n], indexing="ij")
+    sig = (3.5, 4.5, 3.0)
+    c = (2.0, -1.5, 1.0)
+
+    def blob(r):
+        return np.exp(-sum((r[d] - c[d]) ** 2 / (2 * sig[d] ** 2) for d in range(3)))
+
+    x = blob(g).astype(np.complex128)
+    t = np.array([0.7, -1.2, 0.4])
+    th = (0.12, -0.09, 0.15)
+    T = np.array([[*t, *th]])
+    y = O.rigid_forward(x, T, n, n)[..., 0]
+    r = [g[d] - t[d] for d in range(3)]
+
+    def chain(order, sgn):
+        q = r
+        for m in order:  # innermost map first: R4 R5 R6 r applies R6 to r first
+            a, b = ((0, 1), (2, 0), (1, 2))[m]
+            q = _rot(q, a, b, sgn * th[m])
+        return blob(q)
+
+    good = np.abs(y - chain((2, 1, 0), 1)).max()
+    assert good < 5e-4, good
+    assert np.abs(y - chain((0, 1, 2), 1)).max() > 20 * good  # composition reversed
+    assert np.abs(y - chain((2, 1, 0), -1)).max() > 0.05      # sense reversed
 
 
 def test_flip_path_is_unitary_and_invertible():

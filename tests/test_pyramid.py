@@ -263,3 +263,39 @@ def test_pyramid_driver_accel_matches_oracle(dev):
     assert rel_l2(xg.cpu().numpy(), x_ref) < 1e-4, rel_l2(xg.cpu().numpy(), x_ref)
     assert np.abs(Tg[:, :3] - T_ref[:, :3]).max() < 0.01
     assert np.abs(Tg[:, 3:] - T_ref[:, 3:]).max() * 180 / np.pi < 0.01
+
+
+def test_cg_schedule_follows_solveXT_by_hand():
+    """nX of Reconstruction/solveXT.m:417-425 worked by hand for nIt(2) = 8
+    and for the reference's default nIt(2) = 1 (solveXT.m:178): levels
+    32, 64 (new resolutions) and two repeated full-resolution levels."""
+    from disorder_amd import pyramid as PY
+    sizes = [[32] * 3, [64] * 3, [64] * 3, [64] * 3]
+    want = {
+        8: {0: [8, 7, 6, 5, 4, 3, 2, 1, 1, 1], 1: [7, 6, 5, 4, 3, 2, 1, 1], 2: [0, 6, 5, 3, 1, 1],
+            3: [0, 6, 5, 3, 1]},
+        1: {0: [1, 1, 1], 1: [1, 1, 1], 2: [0, 1, 1, 1], 3: [0, 1]},
+        3: {0: [3, 2, 1, 1], 1: [2, 1, 1], 2: [0, 1, 1, 1], 3: [0, 1, 1]},
+    }
+    for mod in (PY, O):
+        for nIt2, per_level in want.items():
+            for li, seq in per_level.items():
+                got, nX = [], nIt2
+                for it in range(1, len(seq) + 1):
+                    nX = mod.cg_iterations(li, it, sizes, nX, nIt2)
+                    got.append(nX)
+                assert got == seq, (mod.__name__, nIt2, li, got, seq)
+
+
+def test_readout_roi_follows_solveXT():
+    """E.nF of solveXT.m:345 (1-based inclusive there, 0-based half-open
+    here): 'F' 1:floor((1-redFOV) n3), 'H' floor(1+redFOV n3):n3."""
+    from disorder_amd import pyramid as PY
+    for mod in (PY, O):
+        assert mod.readout_roi(256, "H") == (85, 256)      # 86:256
+        assert mod.readout_roi(256, "F") == (0, 170)       # 1:170
+        assert mod.readout_roi(64, "H") == (21, 64)        # floor(22.33) = 22 -> 22:64
+        assert mod.readout_roi(3, "F") == (0, 2)           # (2/3)*3 rounds up to 2.0000000000000004
+        assert mod.readout_roi(3, "H") == (1, 3)
+        assert mod.readout_roi(100, "AP") == (0, 100)
+        assert mod.readout_roi(90, "H", 0.5) == (45, 90)   # floor(46) = 46 -> 46:90
