@@ -7,10 +7,13 @@
 // walked in tiles of G dim-2 rows: each tile is multiplied by S_c, FFT'd along
 // dim 1 in LDS, and every thread accumulates the samples it owns (registers)
 // with the dim-2 DFT weight W2^{k2 i2}.  The adjoint (Reconstruction/
-// decode.m:43-84) scatters the weighted samples of each tile row back with
-// LDS float atomics, inverse-FFTs along dim 1, multiplies by conj(S_c) and
-// sums the coils in registers.  For E^H E both halves run in the same
-// workgroup, so the state's k-space never leaves the chip.
+// decode.m:43-84) stages the chunk's weighted sample values in LDS; each
+// tile element (row i2, spectrum column k1) then gathers the samples of its
+// k1 row from the state's CSR in a fixed order (no atomics, so sums are
+// reproducible run to run), the tile is inverse-FFT'd along dim 1,
+// multiplied by conj(S_c) and the coils are summed in registers.  For E^H E
+// both halves run in the same workgroup, so the state's k-space never
+// leaves the chip.
 #include <algorithm>
 
 #include "kernels.hpp"
@@ -29,6 +32,7 @@ __global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
   __shared__ float2 lds[kMaxTile];
   __shared__ float2 tw1[1024];
   __shared__ float2 tw2[1024];
+  __shared__ float2 wst[ADJ ? kMaxJ * kThreads : 1];  // adjoint: chunk's sample values of the current coil
   const int tid = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
   const int i3 = a.i3lo + blockIdx.x;
@@ -192,29 +196,37 @@ __global__ __launch_bounds__(kThreads) void coil_kernel(CoilArgs a) {
 #pragma unroll
           for (int i = 0; i < kTE; ++i) o[i] = make_float2(0.f, 0.f);
           for (int c = 0; c < cc; ++c) {
+            // stage this coil's sample values by slot (j - sc0)
+#pragma unroll
+            for (int i = 0; i < kMaxJ; ++i) {
+              float2 w = make_float2(0.f, 0.f);
+#pragma unroll
+              for (int cq = 0; cq < kCC; ++cq)
+                if (cq == c) w = W[i][cq];
+              wst[tid + i * kThreads] = w;
+            }
+            __syncthreads();
+            // element (p, k1) of the tile sums conj(W2^{k2 (r0 + p)}) w_j over
+            // the samples j of row k1 in this chunk, in CSR order; stored
+            // real/imag-swapped for the inverse FFT
+            const int* rp = a.rowptr + (long long)s * (n1 + 1);
 #pragma unroll
             for (int i = 0; i < kTE; ++i) {
               const int e = tid + i * kThreads;
-              if (e < tot) lds[e] = make_float2(0.f, 0.f);
-            }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < kMaxJ; ++i) {
-              if (jk1[i] >= 0) {
-                float2 w = make_float2(0.f, 0.f);
-#pragma unroll
-                for (int cq = 0; cq < kCC; ++cq)
-                  if (cq == c) w = W[i][cq];
-                int idx = (int)(((long long)jk2[i] * r0) % n2);
-                for (int p = 0; p < Gv; ++p) {
-                  // conj(W2^{k2 i2}) * w, stored real/imag-swapped for the inverse FFT
-                  const float2 v = cmulc(w, tw2[idx]);
-                  float* dst = reinterpret_cast<float*>(&lds[p * n1 + jk1[i]]);
-                  atomicAdd(dst, v.y);
-                  atomicAdd(dst + 1, v.x);
-                  idx += jk2[i];
-                  if (idx >= n2) idx -= n2;
+              if (e < tot) {
+                const int p = e / n1, k1 = e - p * n1;
+                float2 v = make_float2(0.f, 0.f);
+                if (p < Gv) {
+                  const int r = r0 + p;
+                  for (int q = rp[k1]; q < rp[k1 + 1]; ++q) {
+                    const int ent = a.csr[o0 + q];
+                    const int j = o0 + (ent & 0xffff);
+                    if (j < sc0 || j >= sc1) continue;
+                    const int k2 = ent >> 16;
+                    v = cadd(v, cmulc(wst[j - sc0], tw2[(k2 * r) % n2]));
+                  }
                 }
+                lds[e] = make_float2(v.y, v.x);
               }
             }
             __syncthreads();

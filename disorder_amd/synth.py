@@ -91,12 +91,19 @@ def random_walk_motion(S, seed, step_deg=0.5, step_vox=0.5, clip_deg=8.0, clip_v
     return T
 
 
-def uniform_motion(S, seed, rot_deg=5.0, tra=3.0):
-    """Rotations U(-rot,rot) degrees, translations U(-tra,tra) voxels."""
+def uniform_motion(S, seed, rot_deg=5.0, tra=3.0, remove_mean=True):
+    """Simulation/synthesizeT.m:36-40 for one motion state per shot: rotations
+    pi theta_r (U - 0.5) / 180 and translations theta_t (U - 0.5) with
+    theta_r = 2 rot_deg, theta_t = 2 tra (so U(-rot, rot) degrees and
+    U(-tra, tra) voxels), then the mean over shots subtracted from every
+    parameter (:38, as the reference does for the identifiability of the
+    common motion).  [S x 6] float64, radians for rotations."""
     rng = np.random.default_rng(seed)
     T = np.zeros((S, 6))
-    T[:, :3] = rng.uniform(-tra, tra, (S, 3))
-    T[:, 3:] = rng.uniform(-rot_deg, rot_deg, (S, 3)) * math.pi / 180
+    T[:, :3] = 2 * tra * (rng.uniform(0, 1, (S, 3)) - 0.5)
+    T[:, 3:] = math.pi * 2 * rot_deg * (rng.uniform(0, 1, (S, 3)) - 0.5) / 180
+    if remove_mean:
+        T -= T.mean(axis=0, keepdims=True)
     return T
 
 
@@ -112,10 +119,12 @@ def disorder_sampling(n1, n2, S, t1, t2, seed):
 def noise_level(x, Smaps, snr_db, unitary=True):
     """Simulation/SNRToLevels.m:23-33: sigma such that the SENSE-combined image
     noise power 2 sigma^2 mean(1/(sum|S|^2 + 1e-3)) / K equals the mean signal
-    power over 10^(dB/10).  K = numel(x) for the simulator's unnormalised
-    DFT (the reference's formula); K = 1 for data in the reconstruction
-    path's unitary convention (solveXT.m:342), where k-space noise of
-    variance 2 sigma^2 maps to image noise of the same variance per coil."""
+    power over 10^(dB/10).  unitary=False is the reference's formula,
+    K = numel(x), for data y = F S x with the simulator's unnormalised DFT.
+    unitary=True gives the same image SNR for data in the reconstruction
+    path's unitary convention (F / sqrt(numel), solveXT.m:342): those data
+    and their noise are the simulator's divided by sqrt(numel(x)), so the
+    level is the reference's divided by sqrt(numel(x)), i.e. K = 1."""
     meanS = (x.abs() ** 2).mean().item()
     meanN = meanS / (10 ** (snr_db / 10))
     P = (1.0 / ((Smaps.abs() ** 2).sum(dim=-1) + 1e-3)).mean().item() / (1 if unitary else x.numel())

@@ -225,6 +225,48 @@ def test_ehe_properties_config_grid(dev):
     assert abs(p - q) / abs(p) < 1e-5
 
 
+def test_ehe_two_shots_config_grid_matches_oracle(dev):
+    """E^H E of two shots of the 64-shot DISORDER pattern (768 lines each) at
+    the full 256x192x256 grid, 8 coils, against the FP64 oracle (rel-L2 <=
+    1e-5): an error common to E and E^H passes the property checks above but
+    not this one.  Smooth maps and a smooth volume plus noise, motion of a
+    few degrees and voxels (one shot with a rotation beyond 90 degrees would
+    not be a config case; both stay below)."""
+    import disorder_amd as D
+    n = (256, 192, 256)
+    C, S = 8, 64
+    rng = np.random.default_rng(3)
+    H = checkered_sampling(rng, n[0], n[1], S, 8, 8)
+    k1, k2, off, _ = O.build_harmonic_sampling(H, S)
+    pick = [5, 41]
+    sl = [np.arange(off[s], off[s + 1]) for s in pick]
+    k1s, k2s = np.concatenate([k1[i] for i in sl]), np.concatenate([k2[i] for i in sl])
+    offs = np.array([0, len(sl[0]), len(sl[0]) + len(sl[1])])
+    T = np.array([[1.3, -2.2, 0.7, 0.04, -0.03, 0.05], [-0.6, 1.1, 2.4, -0.05, 0.06, -0.02]])
+    g = torch.Generator(device=dev).manual_seed(11)
+    from disorder_amd import synth
+    Sm = synth.coil_maps(n, C, seed=12, device=dev)
+    x = D.colmajor(synth.phantom(n, 12, seed=13, device=dev)
+                   + 0.05 * torch.randn(n, dtype=torch.complex64, device=dev, generator=g))
+    E = D.Encoding(Sm, k1s, k2s, offs)
+    out = E.encodedecode(x, T).cpu().numpy()
+    yg = E.encode(x, T).cpu().numpy()
+    E.close()
+    xh = x.cpu().numpy().astype(np.complex128)
+    Sh = Sm.cpu().numpy().astype(np.complex128)
+    del Sm, x
+    torch.cuda.empty_cache()
+    O.set_fft_workers(16)
+    try:
+        samp = O.Sampling(k1s, k2s, offs, n[0], n[1])
+        yr = O.encode(xh, Sh, samp, T, n)
+        assert rel_l2(yg, yr) < TOL
+        ref = O.decode(yr, Sh, samp, T, n)
+    finally:
+        O.set_fft_workers(None)
+    assert rel_l2(out, ref) < TOL, rel_l2(out, ref)
+
+
 @pytest.mark.parametrize("tolType,nIt,lam,use_pre", [("RelativeResidual2Error", 6, 0.0, True),
                                                      ("Energy", 7, 0.05, True), ("None", 4, 0.0, False)])
 def test_cg_matches_oracle(dev, tolType, nIt, lam, use_pre):

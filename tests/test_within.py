@@ -76,23 +76,24 @@ def test_oracle_splits_the_moving_shot():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("schedule", ["fixed", "nX"])
-def test_driver_split_matches_oracle(dev, schedule):
-    """Same splits, image and motion as the oracle driver; with the nX
+@pytest.mark.parametrize("schedule,N", [("fixed", (32, 24, 16)), ("nX", (64, 24, 16))])
+def test_driver_split_matches_oracle(dev, schedule, N):
+    """Same split, image and motion as the oracle driver.  With the nX
     schedule the within-shot levels start with no CG (nX = 0,
-    solveXT.m:422) and several shots split."""
+    solveXT.m:422) and run 1-CG outer iterations, where an LM accept test
+    between energies equal to FP32 rounding can go either way; the scan is
+    then 64 x 24 x 16, whose levels (32 and 64 along dim 1) both run the
+    deterministic register coil path, and the oracle's robust-weight
+    decisions there sit 0.6 decades or more from the threshold."""
     from disorder_amd import pyramid as PY
     from tests.test_pyramid import _to_dev
-    x_gt, Sm, Sm32, y, k1, k2, off, N, st, bad = _scan()
+    x_gt, Sm, Sm32, y, k1, k2, off, N, st, bad = _scan(N=N)
     kw = dict(KW, cg_schedule=schedule)
     info_o = {}
     x_ref, T_ref = O.pyramid_solve(y, Sm32, k1, k2, off, N, traj=st, info=info_o, **kw)
     xg, Tg, info = PY.solveXT(_to_dev(y, dev), _to_dev(Sm, dev), k1, k2, off, N, traj=st, **kw)
     assert np.array_equal(info["sweep_of_state"], info_o["sweep_of_state"])
-    if schedule == "fixed":
-        assert [lv["states"] for lv in info["levels"]] == [16, 16, 16, 17]
-    else:
-        assert len(info["sweep_of_state"]) > 17
+    assert [lv["states"] for lv in info["levels"]] == [16, 16, 16, 17]
     assert rel_l2(xg.cpu().numpy(), x_ref) < 1e-4, rel_l2(xg.cpu().numpy(), x_ref)
     assert np.abs(Tg[:, :3] - T_ref[:, :3]).max() < 0.01
     assert np.abs(Tg[:, 3:] - T_ref[:, 3:]).max() * 180 / np.pi < 0.01
