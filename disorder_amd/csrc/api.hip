@@ -270,6 +270,8 @@ int dis_ctx_create(int device, dis_ctx** out) {
   }
   if (device < 0 || device >= ndev) return DIS_E_INVALID;
   if (hipSetDevice(device) != hipSuccess) return DIS_E_HIP;
+  // device address of the allocation-free Gram's static scratch (once per device)
+  if (dis::coil_gram_scratch_init(device) != DIS_OK) return DIS_E_HIP;
   dis_ctx* c = new dis_ctx();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <mutex>
 #include <vector>
 
 #include "../../include/disorder.h"
@@ -340,6 +341,31 @@ void hermitian_jacobi(std::vector<std::complex<double>>& A, int n, std::vector<s
 
 }  // namespace
 
+// ---- allocation-free Gram (dis_coil_gram_noalloc)
+// Block partials of the Gram kernel in a file-scope device array: each GPU
+// holds its own copy (kGramStaticBlocks x 64 x 64 complex128 = 16 MiB) until
+// the process ends.  Host code never passes `scratch` itself (in host code
+// the name is only a stand-in); it uses the address hipGetSymbolAddress
+// returned for the device, recorded once per device at context creation.
+constexpr int kGramStaticBlocks = 256;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double2 scratch[(size_t)kGramStaticBlocks * 4096];
+static double2* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+int coil_gram_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return DIS_E_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return DIS_OK;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return DIS_E_HIP;
+  }
+  g_scratch_dev[device] = (double2*)p;
+  return DIS_OK;
+}
+
 int coil_gram(dis_ctx* c, const float2* S, long long nv, int C, double2* P_dev_out) {
   const long long nb0 = (nv + 4095) / 4096;
   const int nb = (int)std::max<long long>(1, std::min<long long>(nb0, kGramBlocksMax));
@@ -422,6 +448,28 @@ int dis_coil_gram(dis_ctx* c, const void* S, int64_t nv, int32_t C, double* P) {
   if (hipMemcpyAsync(P, buf, sizeof(double2) * C * C, hipMemcpyDeviceToHost, ctx_stream(c)) != hipSuccess ||
       hipStreamSynchronize(ctx_stream(c)) != hipSuccess)
     return ctx_fail(c, DIS_E_HIP, "dis_coil_gram: copy of P failed");
+  return DIS_OK;
+}
+
+int dis_coil_gram_noalloc(dis_ctx* c, const void* S, int64_t nv, int32_t C, void* P_dev) {
+  if (!c) return DIS_E_INVALID;
+  if (!S || !P_dev || nv <= 0 || C <= 0) return ctx_fail(c, DIS_E_INVALID, "dis_coil_gram_noalloc: invalid argument");
+  if (C > kMaxCoils)
+    return ctx_fail(c, DIS_E_UNSUPPORTED, "dis_coil_gram_noalloc: more than 64 coils (the static scratch holds 64 x 64)");
+  const int dev = ctx_device(c);
+  double2* ws = (dev >= 0 && dev < kMaxDevices) ? g_scratch_dev[dev] : nullptr;
+  if (!ws) return ctx_fail(c, DIS_E_HIP, "dis_coil_gram_noalloc: static scratch not set up for this device");
+  if (ctx_set_device(c) != hipSuccess) return ctx_fail(c, DIS_E_HIP, "hipSetDevice failed");
+  // the same kernels as dis_coil_gram with at most kGramStaticBlocks blocks;
+  // every block writes its whole 64 x 64 partial, so nothing is cleared
+  const long long nb0 = (nv + 4095) / 4096;
+  const int nb = (int)std::max<long long>(1, std::min<long long>(nb0, kGramStaticBlocks));
+  const long long chunk = ((nv + nb - 1) / nb + kGramTile - 1) / kGramTile * kGramTile;
+  hipStream_t s = ctx_stream(c);
+  hipLaunchKernelGGL(gram_kernel, dim3(nb), dim3(256), 0, s, (const float2*)S, (long long)nv, (int)C, chunk, ws);
+  hipLaunchKernelGGL(gram_finalize_kernel, dim3(64), dim3(256), 0, s, (const double2*)ws, nb, (int)C,
+                     (double2*)P_dev);
+  if (hipGetLastError() != hipSuccess) return ctx_fail(c, DIS_E_HIP, "dis_coil_gram_noalloc: launch failed");
   return DIS_OK;
 }
 

@@ -87,6 +87,9 @@ void ctx_tape_free(dis_ctx* c);
 // Switch event profiling on (1) / off (0) / leave it (-1); returns the old state.
 bool ctx_profiling(dis_ctx* c, int on);
 int ctx_device(dis_ctx* c);
+// Records, once per device, the device address of the allocation-free
+// Gram's static scratch (coilcomp.hip); called with the device current.
+int coil_gram_scratch_init(int device);
 // Coil-stage schedule: 0 auto (work queue for uneven shots), 1 static grid,
 // 2 persistent work queue.
 int ctx_coil_schedule(dis_ctx* c);

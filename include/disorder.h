@@ -221,6 +221,18 @@ int dis_lm(dis_plan* plan, double* T, double* w, uint8_t* cT, double winit, cons
  * column-major C x C.  FP32 products, FP64 sums.  C <= 64. */
 int dis_coil_gram(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, double* P);
 
+/* dis_coil_gram that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The block
+ * partials live in a static per-GPU device array of 256 x 64 x 64 complex128
+ * (16 MiB, held until the process ends); C <= 64 (larger counts return
+ * DIS_E_UNSUPPORTED), any nv.  P_dev: DEVICE complex128 column-major C x C
+ * output.  Stream-ordered: returns once the kernels are enqueued on the
+ * context's stream (dis_sync waits).  Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (other contexts, other
+ * threads) race; calls on one context are ordered by its stream. */
+int dis_coil_gram_noalloc(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, void* P_dev);
+
 /* [U,D] = svd(P) and the component counts (compressCoils.m:93-111) on the
  * host in FP64.  P is Hermitian PSD, so U holds its eigenvectors (column
  * phases arbitrary, as for any SVD) and D [C] the |eigenvalues| descending.
