@@ -51,6 +51,8 @@ SIGNATURES = {
     "dis_plan_set_fold": (c_int, [c_void_p, c_int32, c_int32]),
     "dis_plan_set_readout_roi": (c_int, [c_void_p, c_int32, c_int32]),
     "dis_coil_gram_noalloc": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_void_p]),
+    "dis_stream_probe": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_int32, c_int32, c_int32,
+                                 c_double_p]),
     "dis_encode": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p]),
     "dis_decode": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p]),
     "dis_encodedecode": (c_int, [c_void_p, c_double_p, c_void_p, c_void_p]),

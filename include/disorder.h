@@ -349,6 +349,19 @@ int dis_cg_cs(dis_plan* plan, const double* T, const void* y, const float* Se, c
               dis_shearlet* sh, double sh_scale, void* x, int32_t x_init, int32_t nIt, int32_t tolType, double tol,
               int32_t* nn);
 
+/* Streaming-copy probe (no counterpart in the reference: a measurement of
+ * the GPU's sustainable HBM rate beside which the HBM-bound passes are
+ * judged).  Copies nbytes (a multiple of 32 KiB) from DEVICE in to DEVICE
+ * out reps times after one warm-up copy; ms receives the average time of one
+ * copy.  variant: 0 grid-stride 8-byte lanes, 1 16-byte lanes, 2 four
+ * 16-byte loads in flight per lane, 3 = 2 with nontemporal accesses, 4 the
+ * pencil passes' tile shape (32 KiB per workgroup, 16 8-byte loads per
+ * lane, then the stores), 5 = 4 with each workgroup looping over tiles and
+ * the next tile's loads issued before the current tile's stores.  blocks:
+ * grid size (0 = 8 per CU; variant 4: one per tile). */
+int dis_stream_probe(dis_ctx* ctx, const void* in, void* out, int64_t nbytes, int32_t variant, int32_t blocks,
+                     int32_t reps, double* ms);
+
 /* Kernel classes for dis_profile_read. */
 #define DIS_K_PASS 0   /* fused shear/translation pencil pass, one state per grid row */
 #define DIS_K_BCAST 1  /* first forward pass, one input volume to all states */
