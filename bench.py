@@ -508,7 +508,9 @@ def main():
             else:
                 k["algorithmic_GBps"] = kb / (kms * 1e6) if kms else None
             if name in measured and kms:
-                k["counter_traffic_GBps"] = measured[name] / (kms / nl * 1e6)
+                # the file holds the coil stage per E^H E (its two launches), the passes per launch
+                per = kms / args.steps if name == "coil_stage" else kms / nl
+                k["counter_traffic_GBps"] = measured[name] / (per * 1e6)
             kernels[name] = k
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     nl, kms, kb = prof[dom]
