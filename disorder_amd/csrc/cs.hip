@@ -429,7 +429,7 @@ int run_pass(dis_ctx* c, Shearlet* sh, const Pass& ps, long long* ntiles_out) {
   a.n = n[ax];
   a.L = ax == 0 ? 1 : ax == 1 ? n[0] : (long long)n[0] * n[1];
   a.H = ax == 0 ? (long long)n[1] * n[2] : ax == 1 ? n[2] : 1;
-  if (make_plan(a.n, &a.pl)) return ctx_fail(c, DIS_E_UNSUPPORTED, "shearlet: length with a prime factor above 7");
+  if (make_plan(a.n, &a.pl)) return ctx_fail(c, DIS_E_UNSUPPORTED, "shearlet: length above 1024");
   int st = ctx_twiddles(c, a.n, &a.tw);
   if (st) return st;
   a.scale = ps.scale;

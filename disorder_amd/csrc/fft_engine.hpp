@@ -2,8 +2,10 @@
 //
 // A workgroup of kThreads threads holds a tile of P pencils of length n
 // (P*n <= kMaxTile complex values) in LDS and transforms all of them at once.
-// Stages use radices {16, 8, 4, 2, 3, 5, 7}; the stage loop is runtime, each
-// radix butterfly is compile-time unrolled.  The inverse transform is the
+// Stages use radices {16, 8, 4, 2, 3, 5, 7} with compile-time unrolled
+// butterflies, and any other prime factor of n through a runtime-radix stage
+// (each output element is its butterfly's R-term sum); the stage loop is
+// runtime.  Any n <= 1024 is supported.  The inverse transform is the
 // forward transform applied to real/imaginary-swapped data (no separate
 // inverse kernels), unnormalised.
 //
@@ -263,6 +265,57 @@ __device__ __noinline__ void stockham_stage(lds_float2* buf, const lds_float2* t
   __syncthreads();
 }
 
+// Radix-R Stockham stage for any R (the prime factors above 7): output r of
+// butterfly b (k = b mod Ns) is
+//   sum_q in[b + q n/R] W_n^{q k n/(Ns R)} W_R^{q r} = sum_q in[b + q n/R] W_n^{q m},
+//   m = k n/(Ns R) + r n/R < n,
+// written to (b/Ns) Ns R + k + r Ns: the same stage as stockham_stage with
+// the R-point DFT evaluated directly (R^2 / R = R complex MACs per element).
+// Element u of the tile is (p, r, b) with b fastest along the lanes, so the
+// loads of consecutive lanes are consecutive in LDS.
+template <bool CONTIG>
+__device__ __noinline__ void generic_stage(lds_float2* buf, const lds_float2* tw, int n, int R, int Ns, int P,
+                                           int tid) {
+  constexpr int kE = kMaxTile / kThreads;
+  const int nb = n / R;
+  const int tot = P * n;
+  const int twstep = n / (Ns * R);
+  float2 out[kE];
+  int dst[kE];
+#pragma unroll
+  for (int i = 0; i < kE; ++i) {
+    const int u = tid + i * kThreads;
+    dst[i] = -1;
+    out[i] = make_float2(0.f, 0.f);
+    if (u < tot) {
+      int p, rem;
+      if (CONTIG) { p = u / n; rem = u - p * n; }
+      else { rem = u / P; p = u - rem * P; }
+      const int r = rem / nb, b = rem - r * nb;
+      const int k = b % Ns;
+      const int m = k * twstep + r * nb;
+      float2 acc = make_float2(0.f, 0.f);
+      int e = 0;
+      for (int q = 0; q < R; ++q) {
+        const int j = b + q * nb;
+        const float2 x = ld2(CONTIG ? buf + (p * n + j) : buf + (p + P * j));
+        const float2 w = ld2(tw + e);
+        acc = make_float2(acc.x + x.x * w.x - x.y * w.y, acc.y + x.x * w.y + x.y * w.x);
+        e += m;
+        if (e >= n) e -= n;
+      }
+      const int jd = (b / Ns) * Ns * R + k + r * Ns;
+      dst[i] = CONTIG ? (p * n + jd) : (p + P * jd);
+      out[i] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kE; ++i)
+    if (dst[i] >= 0) st2(buf + dst[i], out[i]);
+  __syncthreads();
+}
+
 // Forward unnormalised FFT of every pencil of the tile.  Caller must have
 // synchronised after filling the tile; returns synchronised.
 template <bool CONTIG>
@@ -280,7 +333,7 @@ __device__ void fft_tile(float2* gbuf, const float2* gtw, const FftPlan& pl, int
       case 3: stockham_stage<3, CONTIG>(buf, tw, pl.n, Ns, P, tid); break;
       case 5: stockham_stage<5, CONTIG>(buf, tw, pl.n, Ns, P, tid); break;
       case 7: stockham_stage<7, CONTIG>(buf, tw, pl.n, Ns, P, tid); break;
-      default: break;
+      default: generic_stage<CONTIG>(buf, tw, pl.n, R, Ns, P, tid); break;
     }
     Ns *= R;
   }

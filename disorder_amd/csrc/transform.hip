@@ -235,7 +235,15 @@ int make_plan(int n, FftPlan* pl, int maxradix) {
     m /= 2;
     ++a;
   }
-  if (m != 1) return DIS_E_UNSUPPORTED;
+  // any other prime factor takes a runtime-radix stage (last)
+  std::vector<int> big;
+  for (int p = 11; m > 1; p += 2) {
+    if ((long long)p * p > m) p = m;  // m itself is prime
+    while (m % p == 0) {
+      big.push_back(p);
+      m /= p;
+    }
+  }
   std::vector<int> two;
   while (a >= lr) {
     two.push_back(1 << lr);
@@ -245,6 +253,7 @@ int make_plan(int n, FftPlan* pl, int maxradix) {
   // powers of two first (the largest radix leads)
   std::vector<int> all(two.begin(), two.end());
   for (int s = 0; s < pl->nst; ++s) all.push_back(pl->radix[s]);
+  for (int r : big) all.push_back(r);
   if ((int)all.size() > kMaxStages) return DIS_E_UNSUPPORTED;
   pl->nst = (int)all.size();
   for (int s = 0; s < pl->nst; ++s) pl->radix[s] = all[s];
@@ -261,7 +270,7 @@ int make_grid(dis_ctx* c, const int32_t n[3], const int32_t Nfull[3], Grid* g) {
     const int nn[3] = {n[0], n[1], n[2]};
     if (ctx_tune_get(c, nn, d, &t)) g->tune[d] = t;
     if (make_plan(n[d], &g->plan[d], g->tune[d].valid ? g->tune[d].maxradix : 16) != DIS_OK)
-      return ctx_fail(c, DIS_E_UNSUPPORTED, "grid size must be <= 1024 with prime factors 2, 3, 5, 7");
+      return ctx_fail(c, DIS_E_UNSUPPORTED, "grid sizes must be at most 1024");
     int st = ctx_twiddles(c, n[d], &g->tw[d]);
     if (st) return st;
   }

@@ -261,7 +261,7 @@ int dis_compress_coils(dis_ctx* ctx, const void* S, int64_t nvS, int32_t C, cons
  * of `batch` volumes [nin] -> [nout] (device complex64, column-major, volume
  * stride prod(n)): per axis fft/nin, crop or zero-pad the spectrum about DC
  * keeping frequencies [-ceil((m-1)/2), floor((m-1)/2)], m = min(nin, nout),
- * then ifft*nout.  Sizes <= 1024 with prime factors <= 7. */
+ * then ifft*nout.  Sizes <= 1024 (any factors). */
 int dis_resample(dis_ctx* ctx, const int32_t nin[3], const int32_t nout[3], int64_t batch, const void* x, void* out);
 
 /* Compact k-space rows of a pyramid level: out[r + Ns_out*col] =

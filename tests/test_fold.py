@@ -109,7 +109,9 @@ GRIDS = [((40, 36, 16), (32, 30)), ((40, 36, 12), (35, 27)), ((64, 48, 16), (32,
          ((64, 48, 16), (64, 32)), ((64, 48, 12), (64, 24)), ((32, 40, 12), (32, 27)),
          ((64, 48, 12), (32, 48)), ((144, 36, 8), (48, 30)),
          # offset larger than the acquired length (64 -> 18: five-fold aliasing), and a larger grid
-         ((32, 64, 8), (32, 18)), ((128, 96, 24), (64, 72))]
+         ((32, 64, 8), (32, 18)), ((128, 96, 24), (64, 72)),
+         # acquired lengths with prime factors 11 and 13 (runtime-radix FFT stage)
+         ((26, 24, 8), (13, 22)), ((32, 30, 11), (22, 26))]
 
 
 @pytest.mark.gpu

@@ -31,7 +31,12 @@ def to_np(t):
                                      ((160, 20, 16), None), ((20, 224, 12), (40, 224, 24)), ((16, 18, 160), None),
                                      ((224, 16, 10), None),
                                      # odd radix-9 register path (144) on every axis
-                                     ((144, 12, 10), None), ((16, 144, 12), (32, 144, 24)), ((12, 16, 144), None)])
+                                     ((144, 12, 10), None), ((16, 144, 12), (32, 144, 24)), ((12, 16, 144), None),
+                                     # prime factors 11 and 13 (runtime-radix stage): 11, 13, 22, 26, 176
+                                     # on every axis, and a prime length 97
+                                     ((11, 13, 22), None), ((26, 11, 13), (52, 22, 26)), ((22, 26, 11), None),
+                                     ((176, 12, 10), None), ((12, 176, 11), (24, 176, 22)), ((13, 10, 176), None),
+                                     ((97, 6, 8), None)])
 def test_rigid_forward_inverse(dev, n, Nfull):
     import disorder_amd as D
     rng = np.random.default_rng(sum(n))
@@ -159,7 +164,14 @@ def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
                                                     ((64, 40, 4), 8, 2, None, False),
                                                     # register coil stage at n1 = 160 and 224
                                                     ((160, 16, 4), 4, 4, None, True),
-                                                    ((224, 12, 4), 3, 3, None, False)])
+                                                    ((224, 12, 4), 3, 3, None, False),
+                                                    # prime factors 11 and 13 on every axis (runtime-radix
+                                                    # stage in the passes and the generic coil stage)
+                                                    ((11, 13, 22), 3, 5, None, True),
+                                                    ((26, 22, 13), 2, 4, 3, False),
+                                                    ((176, 12, 11), 2, 4, None, True),
+                                                    ((13, 176, 6), 2, 5, None, False),
+                                                    ((12, 10, 176), 2, 4, None, True)])
 def test_encode_decode_ehe(dev, n, C, S, nmotion, weights):
     import disorder_amd as D
     rng = np.random.default_rng(11)

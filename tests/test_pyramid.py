@@ -79,7 +79,10 @@ def _to_dev(a, dev):
 @pytest.mark.parametrize("nin,nout,B", [((32, 24, 32), (16, 12, 16), 1), ((16, 12, 16), (32, 24, 32), 2),
                                         ((15, 14, 9), (8, 7, 5), 3), ((64, 48, 64), (32, 24, 32), 1),
                                         ((20, 18, 16), (20, 9, 16), 2), ((8, 6, 8), (8, 6, 8), 1),
-                                        ((256, 8, 4), (128, 8, 4), 1)])
+                                        ((256, 8, 4), (128, 8, 4), 1),
+                                        # pyramid of a 176 axis (176 -> 88 -> 44 -> 22 -> 11) and 13/26
+                                        ((176, 26, 13), (88, 13, 7), 1), ((22, 13, 176), (11, 7, 88), 2),
+                                        ((11, 13, 22), (22, 26, 44), 1)])
 def test_resample_matches_oracle(dev, nin, nout, B):
     import disorder_amd as D
     rng = np.random.default_rng(sum(nin) + B)
