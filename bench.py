@@ -118,46 +118,59 @@ def reconstruction_problem(dev, n=(256, 192, 256), C_in=32, C=8, S=64, snr_db=25
     return {"x_gt": x_gt, "S": Sc, "y": yc, "k1": k1, "k2": k2, "off": off, "T_gt": T_gt, "n": n, "C": C, "S_states": S}
 
 
-def _run_pyramid(P, args, proposer=None, x_ref=None):
+def _run_pyramid(P, args, proposer=None, x_ref=None, tolerSolve=1e-3, label=None):
     """One joint reconstruction with per-outer-iteration snapshots (taken
-    outside the timed solve).  Returns (x, T, info, curve)."""
+    outside the timed solve): NRMSE against the ground truth after removing
+    the common motion (the image moved by mean(T - T_gt), as
+    Simulation/pyramidDISORDER.m:124-131 removes it) and against x_ref.
+    Returns (x, T, info, curve)."""
+    import numpy as np
     import torch
     import disorder_amd as D
     from disorder_amd import pyramid as PY
     n = P["n"]
     xg = P["x_gt"]
     ng = torch.linalg.vector_norm(xg).item()
+    nref = torch.linalg.vector_norm(x_ref).item() if x_ref is not None else None
     snaps = []
+    name = label or ("learned" if proposer else "plain")
+
+    def aligned(xf, T):
+        d = np.mean(np.asarray(T) - P["T_gt"], axis=0, keepdims=True)
+        xa = D.sincRigidTransform(xf, d, di=1)[..., 0]
+        return torch.linalg.vector_norm(xa - xg).item() / ng
 
     def cb(level, it, x, T, t_solve):
         xf = D.resampling(x, n) if list(x.shape) != list(n) else x.clone()
-        e_gt = torch.linalg.vector_norm(xf - xg).item() / ng
-        e_ref = None
-        if x_ref is not None:
-            e_ref = torch.linalg.vector_norm(xf - x_ref).item() / torch.linalg.vector_norm(x_ref).item()
-        snaps.append({"level": level, "outer": it, "t_s": t_solve, "nrmse_vs_gt": e_gt, "nrmse_vs_ref": e_ref,
-                      "x": xf})
-        print(f"[pyramid] {'learned' if proposer else 'plain'} level {level} outer {it}: solve {t_solve:.2f} s, "
-              f"NRMSE vs GT {e_gt:.5f}", file=sys.stderr, flush=True)
+        e_ref = torch.linalg.vector_norm(xf - x_ref).item() / nref if x_ref is not None else None
+        snaps.append({"level": level, "outer": it, "t_s": t_solve, "nrmse_vs_gt_aligned": aligned(xf, T),
+                      "nrmse_vs_ref": e_ref})
+        print(f"[pyramid] {name} level {level} outer {it}: solve {t_solve:.2f} s, NRMSE vs GT (aligned) "
+              f"{snaps[-1]['nrmse_vs_gt_aligned']:.5f}" + (f", vs reference {e_ref:.2e}" if e_ref is not None else ""),
+              file=sys.stderr, flush=True)
 
     x, T, info = PY.solveXT(P["y"], P["S"], P["k1"], P["k2"], P["off"], n, nlevels=args.levels,
-                            n_outer=args.outer, n_cg=args.cg, final_cg=args.final_cg, callback=cb, proposer=proposer)
-    for sn in snaps:
-        if sn["x"] is not None:
-            sn["nrmse_vs_own_final"] = torch.linalg.vector_norm(sn["x"] - x).item() / torch.linalg.vector_norm(x).item()
-            sn["x"] = None
+                            n_outer=args.outer, n_cg=args.cg, final_cg=args.final_cg, callback=cb, proposer=proposer,
+                            tolerSolve=tolerSolve)
     return x, T, info, snaps
 
 
 def bench_pyramid(args):
     """Joint reconstruction at 256x192x256 (32 coils compressed to 8, 64
-    shots, 4 levels, 10 LM iterations per level, 8 CG iterations per outer
-    step, a level closed by CG to 'Energy' tol 1e-3 once every state
-    converged), plain and with the learned proposal.  value = the plain
-    method's solve time until its image is within 1e-3 NRMSE of its own
-    converged image; the learned run reports the same time for itself, its
-    ratio to the plain one, and the difference of the final NRMSE against the
-    ground truth.  Snapshots are taken outside the timed solve."""
+    shots, 4 levels, 10 LM iterations per level, the nX CG schedule of
+    solveXT.m:417-425 with nIt(2) = 8, readout ROI 'H', a level closed by CG
+    to 'Energy' tol 1e-3 once every state converged), plain and with the
+    learned proposal.  value = the plain method's solve time to converged
+    motion and its closing CG.  Reference: the plain method run with the
+    closing CG at 'Energy' tol 1e-6 (untimed); each run reports the time at
+    which its image first lies within 1e-3 NRMSE of it (null when it never
+    does: the default closing CG stops at 'Energy' 1e-3, and
+    final_nrmse_vs_ref says how close it ends).  The learned run's time
+    ratio is taken at its own convergence, with the NRMSE between the two
+    final images (the north star's criterion: the plain NRMSE within 1e-4).
+    NRMSE against the ground truth is given frame-aligned, with the motion
+    error once the common motion is removed.  Snapshots are taken outside
+    the timed solve."""
     import numpy as np
     import torch
     from disorder_amd import learned as LN
@@ -175,38 +188,53 @@ def bench_pyramid(args):
         T_err = T_err - T_err.mean(axis=0, keepdims=True)
         return {"solve_s": info["solve_s"], "levels": info["levels"], "lm_trials": info["lm_trials"],
                 "proposals_accepted": info["proposals_accepted"],
-                "final_nrmse_vs_gt": snaps[-1]["nrmse_vs_gt"],
+                "final_nrmse_vs_gt_aligned": snaps[-1]["nrmse_vs_gt_aligned"],
+                "final_nrmse_vs_ref": snaps[-1]["nrmse_vs_ref"],
                 "motion_err_max_vox": float(np.abs(T_err[:, :3]).max()),
-                "motion_err_max_deg": float(np.abs(T_err[:, 3:]).max() * 180 / math.pi)}
+                "motion_err_max_deg": float(np.abs(T_err[:, 3:]).max() * 180 / math.pi),
+                "motion_err_rms_vox": float(np.sqrt(np.mean(T_err[:, :3] ** 2))),
+                "motion_err_rms_deg": float(np.sqrt(np.mean(T_err[:, 3:] ** 2)) * 180 / math.pi)}
 
-    xp, Tp, ip, sp = _run_pyramid(P, args)
-    tts = next((sn["t_s"] for sn in sp if sn["nrmse_vs_own_final"] <= 1e-3), None)
+    xr, Tr, ir, sr = _run_pyramid(P, args, tolerSolve=1e-6, label="reference")
+    reference = summary(xr, Tr, ir, sr)
+    xp, Tp, ip, sp = _run_pyramid(P, args, x_ref=xr)
+    tts = next((sn["t_s"] for sn in sp if sn["nrmse_vs_ref"] <= 1e-3), None)
     plain = summary(xp, Tp, ip, sp)
-    plain["time_to_1e-3_own_final_s"] = tts
+    plain["time_to_1e-3_vs_ref_s"] = tts
     plain["curve"] = sp
-    line = {"metric": "time to 1e-3 NRMSE (plain LM), s", "value": tts, "unit": "s", "n_gpus": 1,
+    # known-motion CG on the same data for comparison (closing CG settings)
+    import disorder_amd as D
+    E = D.Encoding(P["S"], P["k1"], P["k2"], P["off"])
+    Se = D.coil_precond(P["S"])
+    xk, _, nk = D.CGsolver(P["y"], E, P["T_gt"], nIt=args.final_cg, tolType="Energy", tol=1e-3, Se=Se)
+    E.close()
+    known = torch.linalg.vector_norm(xk - P["x_gt"]).item() / torch.linalg.vector_norm(P["x_gt"]).item()
+    del xk, Se
+    line = {"metric": "joint CG+LM pyramid solve time to converged motion (plain LM), s", "value": ip["solve_s"],
+            "unit": "s", "n_gpus": 1,
             "higher_is_better": False, "dtype": "complex64",
             "data": "synthetic (seeded phantom, 32 Gaussian coils compressed to 8, DISORDER 8x8 sampling, "
                     "random-walk motion, 25 dB noise)",
             "config": {"workload": f"joint CG+LM pyramid at {n[0]}x{n[1]}x{n[2]}, 8 coils, {args.shots} shots, "
-                                   f"{args.levels} levels, {args.outer} outer, {args.cg} CG, final CG "
-                                   f"{args.final_cg} (Energy 1e-3)"},
-            "data_gen_s": gen_s, "plain": plain}
-    x_ref = xp
+                                   f"{args.levels} levels, {args.outer} outer, nX schedule nIt(2) = {args.cg}, "
+                                   f"readout ROI 'H', final CG {args.final_cg} (Energy 1e-3)",
+                       "reference": "the plain method with its closing CG at Energy 1e-6 (untimed)"},
+            "data_gen_s": gen_s, "plain": plain, "reference": reference,
+            "known_motion_cg_nrmse_vs_gt": known, "known_motion_cg_iterations": nk}
     wpath = args.learned if args.learned not in (None, "", "none") else None
     if wpath is not None and os.path.exists(wpath):
         prop = LN.Proposer(LN.MotionNet.load(wpath))
-        xl, Tl, il, sl = _run_pyramid(P, args, proposer=prop, x_ref=x_ref)
-        ttl = next((sn["t_s"] for sn in sl if sn["nrmse_vs_own_final"] <= 1e-3), None)
+        xl, Tl, il, sl = _run_pyramid(P, args, proposer=prop, x_ref=xr)
+        ttl = next((sn["t_s"] for sn in sl if sn["nrmse_vs_ref"] <= 1e-3), None)
         learned = summary(xl, Tl, il, sl)
-        learned["time_to_1e-3_own_final_s"] = ttl
-        learned["final_nrmse_vs_plain_final"] = sl[-1]["nrmse_vs_ref"]
+        learned["time_to_1e-3_vs_ref_s"] = ttl
         learned["proposals"] = {"proposed": prop.proposed, "accepted": prop.accepted, "retired": prop.retired}
         learned["curve"] = sl
         line["learned"] = learned
-        line["learned_nrmse_gt_minus_plain"] = learned["final_nrmse_vs_gt"] - plain["final_nrmse_vs_gt"]
-        if tts and ttl:
-            line["learned_vs_plain_time_ratio"] = ttl / tts
+        line["learned_nrmse_gt_minus_plain"] = learned["final_nrmse_vs_gt_aligned"] - plain["final_nrmse_vs_gt_aligned"]
+        line["learned_final_vs_plain_final_nrmse"] = (torch.linalg.vector_norm(xl - xp).item()
+                                                      / torch.linalg.vector_norm(xp).item())
+        line["learned_vs_plain_time_ratio"] = il["solve_s"] / ip["solve_s"]
     return line
 
 
@@ -557,11 +585,17 @@ def main():
             pl = bench_pyramid(args)
             lr = pl.get("learned", {})
             line["time_to_1e-3_nrmse"] = {
-                "plain_s": pl["value"], "learned_s": lr.get("time_to_1e-3_own_final_s"),
+                "plain_s": pl["plain"]["time_to_1e-3_vs_ref_s"], "learned_s": lr.get("time_to_1e-3_vs_ref_s"),
+                "learned_final_vs_plain_final_nrmse": pl.get("learned_final_vs_plain_final_nrmse"),
                 "learned_vs_plain_time_ratio": pl.get("learned_vs_plain_time_ratio"),
-                "plain_final_nrmse_vs_gt": pl["plain"]["final_nrmse_vs_gt"],
+                "plain_solve_s": pl["plain"]["solve_s"],
+                "plain_final_nrmse_vs_ref": pl["plain"]["final_nrmse_vs_ref"],
+                "plain_final_nrmse_vs_gt_aligned": pl["plain"]["final_nrmse_vs_gt_aligned"],
+                "known_motion_cg_nrmse_vs_gt": pl["known_motion_cg_nrmse_vs_gt"],
+                "plain_motion_err_max": [pl["plain"]["motion_err_max_vox"], pl["plain"]["motion_err_max_deg"]],
                 "learned_nrmse_gt_minus_plain": pl.get("learned_nrmse_gt_minus_plain"),
                 "learned_proposals": lr.get("proposals"),
+                "reference": pl["config"]["reference"],
                 "workload": pl["config"]["workload"], "unit": "s", "higher_is_better": False}
         except Exception as exc:  # reported, not fatal to the E^H E line
             line["time_to_1e-3_nrmse"] = {"error": repr(exc)}
@@ -612,6 +646,9 @@ def bench_small(args):
     ng = torch.linalg.vector_norm(x_gt).item()
     T_err = T_gt - T
     T_err = T_err - T_err.mean(axis=0, keepdims=True)
+    # the joint estimate is fixed up to a motion common to all shots: move it
+    # by mean(T - T_gt) before comparing with the ground truth
+    xa = D.sincRigidTransform(x, np.mean(T - T_gt, axis=0, keepdims=True), di=1)[..., 0]
     line = {"metric": "64^3 reference case: known-motion CG (20 it) + 3-level joint pyramid, wall s",
             "value": t_cg + t_pyr, "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "complex64",
             "data": "synthetic (seeded: 10 ellipsoids, 8 Gaussian coils, 4x4 random-checkered sampling, "
@@ -622,6 +659,7 @@ def bench_small(args):
             "cg20_s": t_cg, "pyramid_s": t_pyr, "levels": info["levels"],
             "cg_nrmse_vs_gt": torch.linalg.vector_norm(xk - x_gt).item() / ng,
             "pyramid_nrmse_vs_gt": torch.linalg.vector_norm(x - x_gt).item() / ng,
+            "pyramid_nrmse_vs_gt_aligned": torch.linalg.vector_norm(xa - x_gt).item() / ng,
             "motion_err_max_vox": float(np.abs(T_err[:, :3]).max()),
             "motion_err_max_deg": float(np.abs(T_err[:, 3:]).max() * 180 / math.pi)}
     if args.no_cpu_baseline:

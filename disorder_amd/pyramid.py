@@ -193,7 +193,8 @@ class Level:
 def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, traLimXT=(0.05, 0.02),
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
             robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None,
-            gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0):
+            gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0,
+            tolerSolve=1e-3):
     """Returns (x [N] on the device, T [states x 6], info).  callback(level,
     it, x, T, t_solve) runs after every outer iteration (and after a level's
     final CG, with outer = last + 1) outside the timed region
@@ -243,7 +244,9 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     (1/3, :97): the readout ROI E.nF of every LMsolver call and the readout
     weights EH.Mbe of the energies behind the robust weights
     (solveXT.m:345-346, :443); a direction other than 'F' or 'H' keeps
-    every readout plane."""
+    every readout plane.
+    tolerSolve: the 'Energy' tolerance of the closing CG of a level
+    (parXT.tolerSolve, IO/disorderAlgorithm.m:108; solveXT.m:177)."""
     import torch
     from . import trajectory as TJ
     from .sampling import harmonic_sampling_repeats
@@ -334,7 +337,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                 L.set_weights(RB.sample_weights(last_We, sweep_of_sample))
             if final_cg:
                 Ec, Tc, yc, Se = cg_ops(L, T, True)
-                x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
+                x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=tolerSolve, Se=Se)
             if robust and leff > 0:
                 We = weights(x, True)
                 subT = (We < 1.0)[sweep_of_state]
@@ -387,7 +390,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
                 if it < n_outer:  # the estT(l) = 0 iteration of :425-481
                     if final_cg:
                         Ec, Tc, yc, Se = cg_ops(L, T, True)
-                        x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=1e-3, Se=Se)
+                        x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=tolerSolve, Se=Se)
                     if robust and leff > 0:
                         We = weights(x, True)
                         subT = (We < 1.0)[sweep_of_state]

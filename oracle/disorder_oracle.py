@@ -1246,7 +1246,8 @@ def cg_iterations(li, it, sizes, nX_prev, nIt2):
 def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit=1e-3,
                   traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
                   info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None, n_within=0, traj=None,
-                  gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0):
+                  gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0,
+                  tolerSolve=1e-3):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -1379,7 +1380,7 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
                 samp_b, Tb = binned(T, samp.We)
                 c = ev[0]
                 x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=final_cg,
-                                    tolType="Energy", tol=1e-3, Se=Sef)
+                                    tolType="Energy", tol=tolerSolve, Se=Sef)
             if robust and leff > 0:
                 Wshot = weights(ev[0], True)
                 subT = [2 if Wshot[sw] < 1 else 1 for sw in sweep_of_state]
@@ -1418,7 +1419,7 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
                         samp_b, Tb = binned(T, samp.We)
                         c = ev[0]
                         x, _, _ = cg_solver(yl[..., :c], Sl[..., :c], samp_b, Tb, N, x0=x, nIt=final_cg,
-                                            tolType="Energy", tol=1e-3, Se=Sef)
+                                            tolType="Energy", tol=tolerSolve, Se=Sef)
                     if robust and leff > 0:
                         Wshot = weights(ev[0], True)
                         subT = [2 if Wshot[sw] < 1 else 1 for sw in sweep_of_state]
