@@ -44,6 +44,12 @@ from . import ops, robust as RB
 from .sampling import level_sampling
 
 
+class NotDisorder(RuntimeError):
+    """A motion state without samples at some pyramid level: the sampling is
+    probably not DISORDER, and solveXT.m:355 returns without performing
+    corrections."""
+
+
 def level_size(N, f):
     """ceil(N / 2^f) per dimension (downsampleOperators.m:34)."""
     return [int(math.ceil(v / 2 ** f)) for v in N]
@@ -307,6 +313,13 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
         subT = np.zeros(len(sweep_of_state), bool)
         nstates = len(sweep_of_state)
         ny = None if NY is None else level_size(NY[:2], max(0, nlevels - 1 - li))
+        # every state keeps samples at this level, else solveXT.m:355 stops
+        _, _, _, loff = level_sampling(k1c, k2c, offc, list(N[:2]) if NY is None else list(NY[:2]),
+                                       list(n[:2]) if ny is None else ny)
+        empty = np.flatnonzero(np.diff(loff) == 0)
+        if len(empty):
+            raise NotDisorder(f"level {li} ({n[0]}x{n[1]}x{n[2]}): states {empty.tolist()[:8]} have no samples; "
+                              "probably not DISORDER, returning without performing corrections")
         L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0, gibbs=gibbs, NY=NY, ny=ny,
                   nF=readout_roi(n[2], fat_shift_dir, redFOV))
         full_res = list(n) == list(N)

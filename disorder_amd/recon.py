@@ -22,12 +22,23 @@ from . import trajectory as TJ
 
 def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), nlevels=4, n_outer=10, n_cg=8,
                            final_cg=300, n_within=0, accel=0, gibbs=0.05, cs=False, uncorrected_cg=0,
-                           write=None, ctx=None, **solve_kw):
+                           write=None, ctx=None, maximum_dynamics=4, **solve_kw):
     """Returns a dict: x (corrected image, device), T [states x 6],
     sweep_of_state, eivaS, info (of pyramid.solveXT), x_cs (with cs),
     x_uncorrected (with uncorrected_cg > 0, CG without motion, solveXT.m:242),
-    files (with write = (path_out, name[, A]))."""
+    files (with write = (path_out, name[, A])).
+
+    The reference's early exits return instead {"fail": reason} and nothing
+    is reconstructed (rec.Fail = 1): as many repeats as
+    maximum_dynamics (parXT.maximumDynamics, disorderAlgorithm.m:117)
+    or more (solveXT.m:33), a separable y-z trajectory (kTraj given as the
+    pair (z2, z3) of different lengths, or one profile; :77), and a number
+    of profiles that is not a whole number of shots (:83).  A level at which
+    some motion state has no samples (:355, 'Probably not DISORDER') returns
+    {"nowithin": True, "message": ..., "x": the reconstruction without motion
+    correction (the closing CG with T = 0, :243), "T": zeros}."""
     import torch
+    from .trajectory import ReconstructionFail
     N = tuple(int(v) for v in S.shape[:3])
     if y.dim() not in (4, 5) or y.shape[0] > N[0] or y.shape[1] > N[1] or y.shape[2] != N[2]:
         raise ValueError("y must be [NY1, NY2, N3, C] or [NY1, NY2, N3, C, R] with NY1 <= N1, NY2 <= N2 "
@@ -39,6 +50,12 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
         raise ValueError("y and S must hold the same coils")
     if y.dim() == 4 and not ops.is_colmajor(y):  # the rows below are read through a column-major view
         raise ValueError("y must have column-major (MATLAB) strides; use disorder_amd.colmajor(y)")
+    if R >= maximum_dynamics:  # solveXT.m:33
+        return {"fail": f"Problem too big to fit in memory ({R} dynamics for a limit of {maximum_dynamics})"}
+    try:
+        TJ.trajectory_states(kTraj, kRange, n_echoes, n_repeats=R)  # host checks of :77, :83 before device work
+    except ReconstructionFail as exc:
+        return {"fail": str(exc)}
     if y.dim() == 5:  # repeats next to the phase encodes, coils last: [NY1, NY2, R, N3, C]
         yp = ops.empty_colmajor((NY[0], NY[1], R, N[2], int(y.shape[3])), y.device)
         yp.copy_(y.permute(0, 1, 4, 2, 3))
@@ -67,9 +84,21 @@ def aligned_reconstruction(y, S, kTraj, kRange, n_echoes, perc=(0.9, 0.9, 0.9), 
         out["x_uncorrected"], _, _ = ops.CGsolver(ycc, Ec, np.zeros((len(off) - 1, 6)), nIt=uncorrected_cg,
                                                   tolType="Energy", tol=1e-3, Se=Se)
         L.close()
-    x, T, info = PY.solveXT(yc, S, k1, k2, off, N, nlevels=nlevels, n_outer=n_outer, n_cg=n_cg, final_cg=final_cg,
-                            eivaS=eivaS, n_within=n_within, traj=st, gibbs=gibbs, accel=accel, ctx=ctx,
-                            NY=NY if folded else None, **solve_kw)
+    try:
+        x, T, info = PY.solveXT(yc, S, k1, k2, off, N, nlevels=nlevels, n_outer=n_outer, n_cg=n_cg,
+                                final_cg=final_cg, eivaS=eivaS, n_within=n_within, traj=st, gibbs=gibbs, accel=accel,
+                                ctx=ctx, NY=NY if folded else None, **solve_kw)
+    except PY.NotDisorder as exc:  # solveXT.m:355: the uncorrected reconstruction is what remains
+        if "x_uncorrected" not in out:
+            L = PY.Level(yc, S, k1, k2, off, N, N, ctx, lm_coils=eivaS[0], gibbs=gibbs, NY=NY, ny=NY)
+            Ec, _, ycc, Se = L.cg(eivaS[0], np.zeros((len(off) - 1, 6)))
+            out["x_uncorrected"], _, _ = ops.CGsolver(ycc, Ec, np.zeros((len(off) - 1, 6)), nIt=final_cg,
+                                                      tolType="Energy", tol=1e-3, Se=Se)
+            L.close()
+        out.update({"nowithin": True, "message": str(exc), "x": out["x_uncorrected"],
+                    "T": np.zeros((len(off) - 1, 6))})
+        torch.cuda.synchronize()
+        return out
     out.update({"x": x, "T": T, "sweep_of_state": info["sweep_of_state"], "info": info})
     if cs:
         from . import shearlet as SH

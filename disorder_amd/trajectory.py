@@ -39,6 +39,21 @@ import math
 import numpy as np
 
 
+class ReconstructionFail(RuntimeError):
+    """A scan the aligned reconstruction declines, with the reason the
+    reference prints when it sets rec.Fail = 1 and returns
+    (Reconstruction/solveXT.m:33, :77, :83)."""
+
+
+class InterruptedScan(ReconstructionFail, ValueError):
+    """The profiles are not a whole number of shots (solveXT.m:83)."""
+
+
+class SeparableTrajectory(ReconstructionFail, ValueError):
+    """rec.Assign.z{2} and z{3} of different lengths, or a single profile
+    (solveXT.m:77)."""
+
+
 def _mlgrid(K):
     """generateGrid(K, 0, K, ceil((K+1)/2)) per dimension: 1-c .. K-c."""
     return [np.arange(1, k + 1, dtype=np.int64) - (k + 2) // 2 for k in K]
@@ -208,6 +223,26 @@ def _dct2_ortho(x):
     return dct(np.asarray(x, np.float64), type=2, norm="ortho", axis=0)
 
 
+def trajectory_array(kTraj):
+    """[NProfs x 2] phase-encode coordinates from kTraj given as such an
+    array or as the pair (rec.Assign.z{2}, rec.Assign.z{3}); a pair of
+    different lengths, or a single profile, is a separable y-z trajectory,
+    which the aligned reconstruction does not handle (solveXT.m:77)."""
+    if isinstance(kTraj, (tuple, list)) and len(kTraj) == 2 and np.ndim(kTraj[0]) == 1:
+        z2, z3 = np.asarray(kTraj[0]).ravel(), np.asarray(kTraj[1]).ravel()
+        if len(z2) <= 1 or len(z2) != len(z3):
+            raise SeparableTrajectory(f"SEPARABLE Y-Z TRAJECTORIES ({len(z2)} and {len(z3)} profiles). "
+                                      "ALIGNED RECONSTRUCTION IS NOT PERFORMED")
+        kTraj = np.stack([z2, z3], axis=1)
+    kTraj = np.asarray(kTraj, np.int64)
+    if kTraj.ndim != 2 or kTraj.shape[1] != 2:
+        raise ValueError("kTraj must be [NProfs x 2] or the pair (z2, z3)")
+    if kTraj.shape[0] <= 1:
+        raise SeparableTrajectory("SEPARABLE Y-Z TRAJECTORIES (a single profile). ALIGNED RECONSTRUCTION IS NOT "
+                                  "PERFORMED")
+    return kTraj
+
+
 def trajectory_states(kTraj, kRange, n_echoes, group_sweeps=1, n_repeats=1):
     """solveXT.m:75-141, :349-350.
 
@@ -220,13 +255,12 @@ def trajectory_states(kTraj, kRange, n_echoes, group_sweeps=1, n_repeats=1):
     steady_state, repeats, sweep_of_profile [NProfs] (1-based), time_index
     and H, [NY1 x NY2] for one repeat or [NY1 x NY2 x R] (1-based profile /
     state 1..NSweeps of each position, 0 unsampled, DC first)."""
-    kTraj = np.asarray(kTraj, np.int64)
+    kTraj = trajectory_array(kTraj)
     NProfs = kTraj.shape[0]
-    if NProfs <= 1:
-        raise ValueError("a trajectory needs more than one profile")
     NEchos = NProfs if n_echoes == 1 else int(n_echoes)
     if NProfs % NEchos:
-        raise ValueError(f"{NProfs} profiles are not a whole number of {NEchos}-echo shots (interrupted scan?)")
+        raise InterruptedScan(f"NUMBER OF SHOTS: {NProfs / NEchos:.2f}, PROBABLY AN INTERRUPTED SCAN "
+                              f"({NProfs} profiles, {NEchos} echoes per shot)")
     NShots = NProfs // NEchos
     R = int(n_repeats)
     if R < 1:

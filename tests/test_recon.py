@@ -76,3 +76,80 @@ def test_entry_point_checks_data_before_touching_the_device():
         recon.aligned_reconstruction(colmajor(torch.zeros((8, 6, 4, 3), dtype=torch.complex64)), S, kTraj, kRange, 1)
     with pytest.raises(ValueError, match="NY1"):
         recon.aligned_reconstruction(colmajor(torch.zeros((8, 6, 4), dtype=torch.complex64)), S, kTraj, kRange, 1)
+
+
+def _small_scan(N=(16, 12, 6), S=4):
+    K = N[:2]
+    g1, g2 = np.meshgrid(np.arange(K[0]) - K[0] // 2, np.arange(K[1]) - K[1] // 2, indexing="ij")
+    k = np.stack([g1.ravel(order="F"), g2.ravel(order="F")], 1)
+    kRange = [[-(K[0] // 2), K[0] - K[0] // 2 - 1], [-(K[1] // 2), K[1] - K[1] // 2 - 1]]
+    return k, kRange
+
+
+def test_reference_early_exits_return_fail_before_device_work():
+    """solveXT.m:33 (too many dynamics), :77 (separable y-z trajectory) and
+    :83 (interrupted scan) set rec.Fail and return: the entry point returns
+    {"fail": reason} from host checks alone (CPU tensors here: nothing
+    touches a device)."""
+    import torch
+    from disorder_amd import recon
+    N = (16, 12, 6)
+    k, kRange = _small_scan(N)
+    S = torch.zeros(N + (2,), dtype=torch.complex64).permute(3, 2, 1, 0).contiguous().permute(3, 2, 1, 0)
+    y4 = torch.zeros(N[:2] + (N[2], 2), dtype=torch.complex64).permute(3, 2, 1, 0).contiguous().permute(3, 2, 1, 0)
+    y5 = torch.zeros(N[:2] + (N[2], 2, 4), dtype=torch.complex64)
+    out = recon.aligned_reconstruction(y5, S, k, kRange, len(k) // 4)
+    assert "too big" in out["fail"] and "4 dynamics" in out["fail"]
+    out = recon.aligned_reconstruction(y4, S, (k[:, 0], k[:-3, 1]), kRange, 1)
+    assert "SEPARABLE" in out["fail"]
+    out = recon.aligned_reconstruction(y4, S, k[:1], kRange, 1)
+    assert "SEPARABLE" in out["fail"]
+    out = recon.aligned_reconstruction(y4, S, k[:-5], kRange, 16)
+    assert "INTERRUPTED" in out["fail"]
+
+
+def test_trajectory_pair_equals_stacked_array():
+    k, kRange = _small_scan()
+    a = TR.trajectory_states(k, kRange, 48)
+    b = TR.trajectory_states((k[:, 0].tolist(), k[:, 1].tolist()), kRange, 48)
+    assert np.array_equal(a["H"], b["H"]) and np.array_equal(a["sweep_of_profile"], b["sweep_of_profile"])
+    with pytest.raises(TR.SeparableTrajectory):
+        TR.trajectory_states((k[:, 0], k[:-1, 1]), kRange, 48)
+    with pytest.raises(TR.InterruptedScan):
+        TR.trajectory_states(k[:-1], kRange, 48)
+
+
+@pytest.mark.gpu
+def test_not_disorder_returns_uncorrected_reconstruction(dev):
+    """Sequential shots (each shot a block of phase-encode rows) leave the
+    outer shots without samples in the coarse level's central k-space:
+    solveXT.m:355 returns without corrections.  The driver raises
+    NotDisorder; the entry point returns nowithin with the reconstruction
+    without motion correction, which equals a plain CG at T = 0."""
+    import torch
+    import disorder_amd as D
+    from disorder_amd import pyramid as PY, recon
+    from tests.test_pyramid import _coil_maps_np
+    rng = np.random.default_rng(3)
+    N, C, S = (32, 24, 8), 3, 4
+    K = N[:2]
+    # sequential order: shot s takes phase-encode rows k1 in [8 s - 16, 8 s - 8) (all k2)
+    g1, g2 = np.meshgrid(np.arange(K[0]) - K[0] // 2, np.arange(K[1]) - K[1] // 2, indexing="ij")
+    k = np.stack([g1.ravel(order="C"), g2.ravel(order="C")], 1)
+    kRange = [[-(K[0] // 2), K[0] - K[0] // 2 - 1], [-(K[1] // 2), K[1] - K[1] // 2 - 1]]
+    k1, k2, off, st = TR.sampling_from_trajectory(k, kRange, len(k) // S)
+    x = smooth_volume(rng, N, nblob=6)
+    Sm = _coil_maps_np(N, C, 4)
+    yc = O.encode(x, Sm, O.Sampling(k1, k2, off, N[0], N[1]), np.zeros((S, 6)), N)
+    to = lambda a: D.colmajor(torch.from_numpy(np.asfortranarray(a.astype(np.complex64))).to(dev))
+    with pytest.raises(PY.NotDisorder):
+        PY.solveXT(to(yc), to(Sm), k1, k2, off, N, nlevels=3, n_outer=2, n_cg=2, gibbs=0)
+    yfull = np.zeros(N[:2] + (N[2], C), np.complex128)
+    yfull[k1, k2] = yc
+    out = recon.aligned_reconstruction(to(yfull), to(Sm), k, kRange, len(k) // S, perc=None, nlevels=3, n_outer=2,
+                                       n_cg=2, final_cg=10, gibbs=0)
+    assert out["nowithin"] and "not DISORDER" in out["message"] and np.all(out["T"] == 0)
+    E = D.Encoding(to(Sm), k1, k2, off)
+    xr, _, _ = D.CGsolver(to(yc), E, np.zeros((S, 6)), nIt=10, tolType="Energy", tol=1e-3,
+                          Se=D.coil_precond(to(Sm)))
+    assert rel_l2(out["x"].cpu().numpy(), xr.cpu().numpy()) < 1e-5
