@@ -161,6 +161,163 @@ __global__ __launch_bounds__(256) void gram_finalize_kernel(const double2* __res
 
 // ------------------------------------------------------------------ apply
 template <int TERMS>
+__device__ __forceinline__ void split_terms(float a, short (&t)[3]);
+
+// ------------------------------------------------------------ Gram on MFMA
+// The Gram matrix as one real GEMM over the voxels, G = X Y^T with
+// X rows 2c + {re, im} of Shat and Y rows 2c + {re, im} of S (2C <= 128 rows,
+// K = voxels): P_re(i,j) = G[2i][2j] + G[2i+1][2j+1],
+// P_im(i,j) = G[2i+1][2j] - G[2i][2j+1].  A workgroup takes a chunk of
+// voxels in tiles of kGmKt: the 256 threads stage a tile (coalesced along
+// voxels per coil), form Shat with the voxel's squared coil norm, and store
+// X and Y split into bf16 terms in LDS; wave w then accumulates output rows
+// [32 w, 32 w + 32) x all 128 columns (16 tiles of 16 x 16) with
+// v_mfma_f32_16x16x16_bf16 (hi.hi + hi.lo + lo.hi; with TERMS = 6 also
+// mid terms) in FP32.  Each workgroup writes its FP32 128 x 128 partial; a
+// fixed-order FP64 sum over the partials gives P (deterministic).
+// Fragment layouts as in apply_kernel: lane l holds A[l&15][4(l>>4)+j],
+// B[4(l>>4)+j][l&15], D[4(l>>4)+j][l&15].
+constexpr int kGmKt = 32;             // voxels per staged tile (two MFMA K steps)
+constexpr int kGmLd = kGmKt + 4;      // padded row stride of the bf16 tiles
+constexpr int kGmRows = 2 * kMaxCoils;
+constexpr int kGmChunk = 4096;        // voxels per workgroup (FP32 accumulation length)
+constexpr int kGmBlocksMax = 8192;
+
+template <int TERMS>
+__global__ __launch_bounds__(256) void gram_mfma_kernel(const float2* __restrict__ S, long long nv, int C,
+                                                       long long chunk, float* __restrict__ partial) {
+  constexpr int NT = TERMS == 6 ? 3 : 2;  // bf16 terms per value
+  __shared__ __align__(16) short xs[NT][kGmRows][kGmLd];
+  __shared__ __align__(16) short ys[NT][kGmRows][kGmLd];
+  __shared__ float nrm[8][kGmKt];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int i16 = lane & 15, q = lane >> 4;
+  const long long v0 = (long long)blockIdx.x * chunk;
+  const long long v1 = v0 + chunk < nv ? v0 + chunk : nv;
+  f32x4 acc[2][8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int R = 2 * C;
+  const bool wave_live = 32 * w < R;  // rows of this wave hold coils
+  // staging role: voxel tv, coils 8 tg .. 8 tg + 7
+  const int tv = tid & (kGmKt - 1), tg = tid / kGmKt;
+  // the next tile's maps are loaded while this tile's MFMAs run
+  auto load_tile = [&](long long vb, float2 (&s)[8]) {
+    const long long v = vb + tv;
+    const bool vok = v < v1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int c = 8 * tg + k;
+      s[k] = (vok && c < C) ? S[v + nv * c] : make_float2(0.f, 0.f);
+    }
+  };
+  float2 s[8], sn[8];
+  load_tile(v0, s);
+  for (long long vb = v0; vb < v1; vb += kGmKt) {
+    float part = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) part += s[k].x * s[k].x + s[k].y * s[k].y;
+    nrm[tg][tv] = part;
+    __syncthreads();
+    float tot = 0.f;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) tot += nrm[g][tv];
+    const float inv = 1.0f / (tot + 1e-9f);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int r = 2 * (8 * tg + k);
+      const float vals[4] = {s[k].x * inv, s[k].y * inv, s[k].x, s[k].y};
+      short t[4][3];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) split_terms<TERMS>(vals[e], t[e]);
+#pragma unroll
+      for (int h = 0; h < NT; ++h) {
+        xs[h][r][tv] = t[0][h];
+        xs[h][r + 1][tv] = t[1][h];
+        ys[h][r][tv] = t[2][h];
+        ys[h][r + 1][tv] = t[3][h];
+      }
+    }
+    __syncthreads();
+    load_tile(vb + kGmKt, sn);  // past v1 the loads are predicated off
+    if (wave_live) {
+#pragma unroll
+      for (int ks = 0; ks < kGmKt / 16; ++ks) {
+        const int k0 = 16 * ks + 4 * q;
+        bf16x4 A[2][NT];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int h = 0; h < NT; ++h) A[mt][h] = *(const bf16x4*)&xs[h][32 * w + 16 * mt + i16][k0];
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+          if (16 * nt >= R) break;  // uniform: columns beyond the coils are zero
+          bf16x4 B[NT];
+#pragma unroll
+          for (int h = 0; h < NT; ++h) B[h] = *(const bf16x4*)&ys[h][16 * nt + i16][k0];
+#pragma unroll
+          for (int mt = 0; mt < 2; ++mt) {
+            f32x4 d = acc[mt][nt];
+            // smallest terms first
+            if (TERMS == 6) {
+              d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[mt][1], B[1], d, 0, 0, 0);
+              d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[mt][0], B[2], d, 0, 0, 0);
+              d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[mt][2], B[0], d, 0, 0, 0);
+            }
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[mt][0], B[1], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[mt][1], B[0], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A[mt][0], B[0], d, 0, 0, 0);
+            acc[mt][nt] = d;
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next tile overwrites the staging buffers
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = sn[k];
+  }
+  // partial[blk][row][col], row = 32 w + 16 mt + 4 q + j, col = 16 nt + i16
+  float* out = partial + (long long)blockIdx.x * (kGmRows * kGmRows);
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[(32 * w + 16 * mt + 4 * q + j) * kGmRows + 16 * nt + i16] = acc[mt][nt][j];
+}
+
+// P(i, j) from the FP32 block partials of G, summed in FP64 in a fixed
+// order: four strided partial sums per entry, then combined in order.
+__global__ __launch_bounds__(256) void gram_mfma_finalize_kernel(const float* __restrict__ partial, int nb, int C,
+                                                                double2* __restrict__ out) {
+  __shared__ double2 red[4][64];
+  const int e = blockIdx.x * 64 + (threadIdx.x & 63);  // e = i * 64 + j
+  const int qq = threadIdx.x >> 6;
+  const int i = e >> 6, j = e & 63;
+  double x = 0, y = 0;
+  if (i < C && j < C) {
+    for (int b = qq; b < nb; b += 4) {
+      const float* g = partial + (long long)b * (kGmRows * kGmRows);
+      x += (double)g[(2 * i) * kGmRows + 2 * j] + (double)g[(2 * i + 1) * kGmRows + 2 * j + 1];
+      y += (double)g[(2 * i + 1) * kGmRows + 2 * j] - (double)g[(2 * i) * kGmRows + 2 * j + 1];
+    }
+  }
+  red[qq][threadIdx.x & 63] = make_double2(x, y);
+  __syncthreads();
+  if (qq == 0 && i < C && j < C) {
+    double2 t = red[0][threadIdx.x];
+    for (int r = 1; r < 4; ++r) {
+      t.x += red[r][threadIdx.x].x;
+      t.y += red[r][threadIdx.x].y;
+    }
+    out[i + (long long)C * j] = t;  // column-major C x C
+  }
+}
+
+template <int TERMS>
 __device__ __forceinline__ void split_terms(float a, short (&t)[3]) {
   const unsigned short h = f2bf(a);
   const float r1 = a - bf2f(h);
@@ -367,16 +524,17 @@ int coil_gram_scratch_init(int device) {
 }
 
 int coil_gram(dis_ctx* c, const float2* S, long long nv, int C, double2* P_dev_out) {
-  const long long nb0 = (nv + 4095) / 4096;
-  const int nb = (int)std::max<long long>(1, std::min<long long>(nb0, kGramBlocksMax));
-  const long long chunk = ((nv + nb - 1) / nb + kGramTile - 1) / kGramTile * kGramTile;
+  // split-bf16 MFMA GEMM over voxel chunks of about kGmChunk, FP32 partials
+  const long long nb0 = (nv + kGmChunk - 1) / kGmChunk;
+  const int nb = (int)std::max<long long>(1, std::min<long long>(nb0, kGmBlocksMax));
+  const long long chunk = ((nv + nb - 1) / nb + kGmKt - 1) / kGmKt * kGmKt;
   void* ws = nullptr;
-  int st = ctx_workspace(c, sizeof(double2) * 4096 * (size_t)nb, &ws);
+  int st = ctx_workspace(c, sizeof(float) * kGmRows * kGmRows * (size_t)nb, &ws);
   if (st) return st;
   hipStream_t s = ctx_stream(c);
   const int ps = prof_begin(c, DIS_K_GRAM);
-  hipLaunchKernelGGL(gram_kernel, dim3(nb), dim3(256), 0, s, S, nv, C, chunk, (double2*)ws);
-  hipLaunchKernelGGL(gram_finalize_kernel, dim3(64), dim3(256), 0, s, (const double2*)ws, nb, C, P_dev_out);
+  hipLaunchKernelGGL(gram_mfma_kernel<3>, dim3(nb), dim3(256), 0, s, S, nv, C, chunk, (float*)ws);
+  hipLaunchKernelGGL(gram_mfma_finalize_kernel, dim3(64), dim3(256), 0, s, (const float*)ws, nb, C, P_dev_out);
   prof_end(c, ps, 8.0 * (double)nv * C);
   if (hipGetLastError() != hipSuccess) return ctx_fail(c, DIS_E_HIP, "coil Gram launch failed");
   return DIS_OK;

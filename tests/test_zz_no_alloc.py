@@ -1,13 +1,11 @@
 """dis_coil_gram_noalloc (the allocation-free variant of dis_coil_gram,
 whose block partials live in a static per-GPU device array) against
 dis_coil_gram through the C-ABI: two calls in a row, the second on a
-smaller input (fewer voxels and coils).  The kernels are shared: with up to
-256 x 4096 voxels both entry points cut the voxels into the same blocks and
-agree bit for bit; above that the static variant uses 256 larger blocks,
-whose FP32 partial sums (256 products each before the FP64 flush) round
-differently, so the bound is the Gram's FP32-product accuracy (1e-7 of the
-largest entry; the Gram itself is pinned to 1e-6 against FP64 in
-test_coilcomp.py)."""
+smaller input (fewer voxels and coils).  The static variant runs the VALU
+Gram kernel (FP32 products flushed into FP64 every 256 voxels, at most 256
+blocks so its partials fit the static array) and dis_coil_gram the
+split-bf16 MFMA kernel; both are pinned to rel-L2 1e-6 against FP64 in
+test_coilcomp.py, so they agree to 2e-6 relative."""
 import ctypes
 
 import numpy as np
@@ -44,10 +42,7 @@ def test_noalloc_gram_matches_allocating_gram(dev):
         S = torch.randn((C, nv), dtype=torch.complex64, device=dev, generator=g)  # row c = coil c: stride nv
         a = _gram_noalloc(ctx, S, nv, C, dev)
         b = _gram(ctx, S, nv, C)
-        if nv <= 256 * 4096:
-            assert np.array_equal(a, b), (nv, C, np.abs(a - b).max())
-        else:
-            assert np.abs(a - b).max() <= 1e-7 * np.abs(b).max(), (nv, C, np.abs(a - b).max())
+        assert np.linalg.norm(a - b) <= 2e-6 * np.linalg.norm(b), (nv, C, np.linalg.norm(a - b) / np.linalg.norm(b))
         assert np.allclose(a, a.conj().T, rtol=0, atol=1e-7 * np.abs(a).max())  # Hermitian
         assert np.all(np.abs(np.diag(a).imag) <= 1e-6 * np.abs(np.diag(a).real))
 
