@@ -71,8 +71,9 @@ int dis_rigid_transform(dis_ctx* ctx, const int32_t n[3], const int32_t Nfull[3]
 
 /* Encoding operator description (the E/EH structs of Reconstruction/solveXT.m
  * after Methods/buildHarmonicSampling.m):
- *   n        : level grid of x, also the phase-encode grid of y (E.Uf, SENSE
- *              folding, is not supported: NX must equal NY)
+ *   n        : level grid of x, also the phase-encode grid of y unless
+ *              SENSE folding is set (dis_plan_set_fold: y then lives on the
+ *              smaller acquired grid)
  *   Nfull    : full-resolution grid of the motion parameters
  *   ncoils   : C (E.Sf dim 4)
  *   nstates  : segments (E.NSe); the first nmotion (E.NMs) are rigidly
@@ -218,7 +219,9 @@ int dis_lm(dis_plan* plan, double* T, double* w, uint8_t* cT, double winit, cons
 
 /* Gram matrix P(i,j) = sum_v Shat_i(v) conj(S_j(v)), Shat = S/(sum_c|S_c|^2 +
  * 1e-9) (compressCoils.m:78-87).  P: host complex128 (re, im pairs),
- * column-major C x C.  FP32 products, FP64 sums.  C <= 64. */
+ * column-major C x C.  Split-bf16 MFMA products (three bf16 products per
+ * FP32 product) accumulated in FP32 over chunks of 4096 voxels, the chunk
+ * partials summed in FP64.  C <= 64. */
 int dis_coil_gram(dis_ctx* ctx, const void* S, int64_t nv, int32_t C, double* P);
 
 /* dis_coil_gram that allocates nothing.  No counterpart in the reference:
