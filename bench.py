@@ -16,7 +16,9 @@ Other workloads (--workload): pyramid (the joint CG+LM reconstruction, plain
 and with the learned proposal), compress (64 -> 8 coil compression),
 batched (six uneven volumes, hipGraph-captured CG), cs (the shearlet
 regulariser), small (the 64^3 reference case end to end, with the oracle's
-run beside it) and fold (E^H E with SENSE folding of the phase encodes).
+run beside it), fold (E^H E with SENSE folding of the phase encodes), stream
+(the device-copy ceiling) and proposal (how often the learned proposal
+lowers a shot's energy, against the distance from the solution).
 """
 from __future__ import annotations
 
@@ -41,12 +43,14 @@ def parse():
     ap.add_argument("--shots", type=int, default=64)
     ap.add_argument("--cpu-full", dest="cpu_full", action="store_true",
                     help="--workload small: run the oracle's whole 64^3 case (minutes), not a bounded sample")
-    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched", "cs", "small", "fold", "stream"],
+    ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched", "cs", "small", "fold", "stream",
+                                           "proposal"],
                     default="ehe",
                     help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid; "
                          "compress: 64 -> 8 coil compression of maps and data; batched: six uneven volumes in one "
                          "batched CG with hipGraph-captured iterations; small: the 64^3 reference case (CG with "
-                         "known motion, then the 3-level pyramid); fold: E^H E with SENSE folding")
+                         "known motion, then the 3-level pyramid); fold: E^H E with SENSE folding; stream: device-copy "
+                         "ceiling; proposal: learned-proposal acceptance against distance from the solution")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--outer", type=int, default=10)
     ap.add_argument("--cg", type=int, default=8)
@@ -455,6 +459,39 @@ def _cpu_baseline_cs(n):
                       f"256x192x256: {tb:.2f} s, FFT3 {tf:.2f} s; apps/s = 1/(50 band + 2 FFT3)"}
 
 
+def bench_proposal(args):
+    """Acceptance of the learned proposal against the distance of the
+    current estimate from the solution (learned.acceptance_curve): 3
+    simulated 64^3 scans of 32 shots and 8 coils (the training
+    distribution), per-shot errors N(0, d) for d from 0.05 to 4 voxels /
+    degrees, without and with a common offset N(0, 2) shared by the shots,
+    image from 4 CG iterations; the proposal (Newton step plus the
+    network's correction, centred) beside the diagonal Newton step alone.
+    Rows are averaged over the scans."""
+    import numpy as np
+    from disorder_amd import learned as LN
+    wpath = args.learned if args.learned not in (None, "", "none") else LN.WEIGHTS
+    net = LN.MotionNet.load(wpath)
+    dists = (0.05, 0.1, 0.2, 0.5, 1.0, 2.0, 4.0)
+    offsets = (0.0, 2.0)
+    t0 = time.perf_counter()
+    rows = LN.acceptance_curve(net, np.random.default_rng(11), dists, offsets=offsets, n_scans=3)
+    out = []
+    for o in offsets:
+        for d in dists:
+            rs = [r for r in rows if r["dist"] == d and r["offset"] == o]
+            row = {"dist": d, "offset": o, "err_before": float(np.mean([r["err_before"] for r in rs]))}
+            for k in ("net", "newton"):
+                row[k] = {m: float(np.mean([r[k][m] for r in rs])) for m in ("accept", "err_all", "err_accepted")}
+            out.append(row)
+    return {"metric": "learned-proposal acceptance against distance from the solution", "unit": "fraction of shots",
+            "n_gpus": 1, "data": "synthetic (learned.simulate_scan: 64^3, 8 coils, 32 shots, 3 scans)",
+            "config": {"workload": "per-shot errors N(0, d) voxel/degree, common offset N(0, o), image from 4 CG "
+                                   "iterations; err_* = rms per-shot error with the common motion removed"},
+            "weights": os.path.relpath(wpath, os.path.dirname(os.path.abspath(__file__))),
+            "elapsed_s": time.perf_counter() - t0, "rows": out}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -481,6 +518,9 @@ def main():
         return
     if args.workload == "stream":
         print(json.dumps(bench_stream(args)), flush=True)
+        return
+    if args.workload == "proposal":
+        print(json.dumps(bench_proposal(args)), flush=True)
         return
     import numpy as np
     import torch

@@ -10,10 +10,17 @@ of the error.
 Features per shot (13): from the LM partial sums at the current (x, T)
 (Reconstruction/LMsolver.m:82-115, via dis_lm_partials):
   [0:6]   diagonal Newton step -g_i / H_ii (translations in voxels,
-          rotations in degrees),
+          rotations in degrees, clipped to +-NEWTON_CLIP),
   [6:12]  log(H_ii / mean over shots of H_ii),
   [12]    log(residual energy / mean over shots).
-Target: T_true - T_current in the same units.
+Proposal: the diagonal Newton step plus the network's output, less its mean
+over shots.  Training target of the network: the relative update
+(T_true - T_current) minus its mean over shots (the motion is determined
+only up to a rigid motion shared by every shot, and the image at the
+current estimate sits in the frame of the shots' mean error, so a shot's
+energy with that image falls towards the relative update, not the absolute
+one), less the Newton step of the features: the network learns the
+correction to the diagonal Newton step.
 
 Network, training and inference run in libdisorder_hip.so
 (dis_mlp_forward / dis_mlp_backward / dis_mlp_train_step): MLP
@@ -34,6 +41,7 @@ from ._lib import check
 
 NIN, NHID, NOUT, NK = 13, 256, 6, 5
 DEG = 180.0 / math.pi
+NEWTON_CLIP = 20.0  # voxels / degrees
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "motion_net.npy")
 
 
@@ -131,17 +139,41 @@ def features(sums):
     F = np.zeros((s.shape[0], NIN))
     step = -g / hd
     step[:, 3:] *= DEG
-    F[:, 0:6] = step
+    F[:, 0:6] = np.clip(np.nan_to_num(step, nan=0.0), -NEWTON_CLIP, NEWTON_CLIP)
     F[:, 6:12] = np.log(hd / np.maximum(hd.mean(axis=0, keepdims=True), tiny))
     F[:, 12] = np.log(np.maximum(en, tiny) / max(en.mean(), tiny))
     return np.nan_to_num(F, nan=0.0, posinf=0.0, neginf=0.0).astype(np.float32)
 
 
 def to_update(Y):
-    """Network output (vox, deg) -> motion update (vox, rad)."""
+    """Update in (vox, deg) -> motion update (vox, rad)."""
     d = np.asarray(Y, np.float64).copy()
     d[:, 3:] /= DEG
     return d
+
+
+def relative_target(T_true, Tc):
+    """Training target before the Newton step is taken off: (T_true - Tc)
+    less its mean over shots, translations in voxels, rotations in
+    degrees."""
+    t = np.asarray(T_true, np.float64) - np.asarray(Tc, np.float64)
+    t -= t.mean(axis=0, keepdims=True)
+    t[:, 3:] *= DEG
+    return t.astype(np.float32)
+
+
+def net_target(F, t):
+    """What the network is trained to output for features F and relative
+    target t: the correction to the diagonal Newton step."""
+    return (np.asarray(t, np.float32) - np.asarray(F, np.float32)[:, 0:6]).astype(np.float32)
+
+
+def propose(net, F, dev):
+    """Proposed update (vox, deg) for one scan's features [S x 13]: the
+    Newton step plus the network's correction, less its mean over shots."""
+    Y = net.forward(torch.from_numpy(np.ascontiguousarray(F, np.float32)).to(dev), [0, F.shape[0]]).cpu().numpy()
+    u = np.asarray(F, np.float64)[:, 0:6] + Y
+    return u - u.mean(axis=0, keepdims=True)
 
 
 def lm_sums(y, E, x, T):
@@ -207,10 +239,9 @@ class Proposer:
         scale = level_scale(L)
         Fh = features(sums)
         Fh[:, 0:3] /= scale
-        F = torch.from_numpy(Fh).to(x.device)
-        Y = self.net.forward(F, [0, F.shape[0]]).cpu().numpy()
-        Y[:, 0:3] *= scale
-        d = to_update(Y)
+        u = propose(self.net, Fh, x.device)
+        u[:, 0:3] *= scale
+        d = to_update(u)
         Tn = T.copy()
         take = none.copy()
         for a in self.alphas:
@@ -231,11 +262,70 @@ class Proposer:
         return Tn, take
 
 
+def relative_error(T, T_true):
+    """Per-shot motion error once the common motion is removed (the method
+    determines the motion only up to a rigid motion shared by every shot,
+    Simulation/synthesizeT.m:38, pyramidDISORDER.m:124-131): rows of
+    (T - T_true) - mean over shots, translations in voxels, rotations in
+    degrees."""
+    d = np.asarray(T, np.float64) - np.asarray(T_true, np.float64)
+    d = d - d.mean(axis=0, keepdims=True)
+    d[:, 3:] *= DEG
+    return d
+
+
+def acceptance_curve(net, rng, dists, offsets=(0.0,), n=(64, 64, 64), S=32, C=8, n_scans=3, n_cg=4):
+    """How often a proposal lowers its shot's energy (the Proposer's test),
+    against the distance of the current estimate from the solution.
+
+    For each simulated scan (simulate_scan), distance d and common offset o:
+    T_c = T_true + e + c with e ~ N(0, d) per shot and parameter (voxels,
+    degrees) and c ~ N(0, o) one draw shared by every shot; the image from
+    n_cg CG iterations at T_c; features at (x, T_c).  Candidate updates:
+      net          the proposal (propose: Newton step plus the network's
+                   correction, centred over the shots),
+      newton       the diagonal Newton step (features 0:6) alone.
+    For each: the fraction of shots whose energy decreases, and the rms
+    relative error (relative_error, all shots, 6 parameters in voxels and
+    degrees) before, with every shot moved, and with only the shots that
+    lowered their energy moved.  Returns a list of rows."""
+    dev = net.p.device
+    rows = []
+    for _ in range(n_scans):
+        E, x, y, T_true, off, Sm = simulate_scan(rng, n, S, C, dev)
+        for o in offsets:
+            for d in dists:
+                e = rng.normal(0.0, d, (S, 6)) + rng.normal(0.0, o, (1, 6))
+                e[:, 3:] /= DEG
+                Tc = T_true + e
+                xc, _, _ = ops.CGsolver(y, E, Tc, x=None, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0)
+                sums = lm_sums(y, E, xc, Tc)
+                F = features(sums)
+                cands = {"net": to_update(propose(net, F, dev)), "newton": to_update(F[:, 0:6])}
+                before = float(np.sqrt(np.mean(relative_error(Tc, T_true) ** 2)))
+                row = {"dist": d, "offset": o, "err_before": before}
+                for name, u in cands.items():
+                    Tp = Tc + u
+                    Tp[:, 3:] = (Tp[:, 3:] + math.pi) % (2 * math.pi) - math.pi
+                    ok = state_energy(E, xc, y, Tp, off) < sums[:, 0]
+                    Tk = Tc.copy()
+                    Tk[ok] = Tp[ok]
+                    row[name] = {"accept": float(ok.mean()),
+                                 "err_all": float(np.sqrt(np.mean(relative_error(Tp, T_true) ** 2))),
+                                 "err_accepted": float(np.sqrt(np.mean(relative_error(Tk, T_true) ** 2)))}
+                rows.append(row)
+                del xc
+        E.close()
+        del x, y, Sm
+    return rows
+
+
 # ------------------------------------------------------------------ training data
-def simulate_scan(rng, n, S, C, dev, snr_db=30.0):
+def simulate_scan(rng, n, S, C, dev, snr_db=30.0, sampling=False):
     """One simulated scan (the method's own synthesis): phantom, coils,
     DISORDER sampling, random-walk motion with step sigma U(0.2, 1.5) deg /
-    voxel plus 5 % abrupt jumps up to 10 deg / 10 voxels, noise."""
+    voxel plus 5 % abrupt jumps up to 10 deg / 10 voxels, noise.  Returns
+    (E, x, y, T, off, Sm), and (k1, k2) after them with `sampling`."""
     seed = int(rng.integers(1 << 30))
     t1 = int(2 ** math.floor(math.log2(math.sqrt(S))))
     t2 = S // t1
@@ -254,7 +344,50 @@ def simulate_scan(rng, n, S, C, dev, snr_db=30.0):
     E = ops.Encoding(Sm, k1, k2, off)
     y = E.encode(x, T)
     y = synth.add_noise(y, synth.noise_level(x, Sm, snr_db), seed=seed + 3)
+    if sampling:
+        return E, x, y, T, off, Sm, k1, k2
     return E, x, y, T, off, Sm
+
+
+class StateRecorder:
+    """A proposer for pyramid.solveXT that proposes nothing and records what
+    the Proposer would see at every outer iteration of every level: the
+    features at (x, T) and the relative target, translations of both in
+    voxels of the level grid."""
+
+    def __init__(self, T_true):
+        self.T_true = np.asarray(T_true, np.float64)
+        self.rows = []
+
+    def reset(self):
+        pass
+
+    def __call__(self, L, x, T, cT):
+        if (~cT).any():
+            scale = level_scale(L)
+            F = features(lm_sums(L.y, L.E, x, T))
+            F[:, 0:3] /= scale
+            t = relative_target(self.T_true, T)
+            t[:, 0:3] /= scale
+            self.rows.append((F, t))
+        return T, np.zeros(len(T), bool)
+
+
+def pyramid_examples(rng, n, S, C, dev, nlevels, n_outer=10, n_cg=8):
+    """Training examples from the states the plain pyramid visits on one
+    simulated scan (simulate_scan): solveXT with a StateRecorder, the
+    default nX schedule and readout ROI, no closing CG.  Returns the rows
+    and the largest relative error (voxels / degrees) of the solve's final
+    motion, by which scans the plain method does not solve can be left
+    out."""
+    from . import pyramid
+    E, x, y, T, off, Sm, k1, k2 = simulate_scan(rng, n, S, C, dev, sampling=True)
+    E.close()
+    rec = StateRecorder(T)
+    _, Tf, _ = pyramid.solveXT(y, Sm, k1, k2, off, n, nlevels=nlevels, n_outer=n_outer, n_cg=n_cg, final_cg=0,
+                               proposer=rec)
+    del x, y, Sm
+    return rec.rows, float(np.abs(relative_error(Tf, T)).max())
 
 
 def training_examples(rng, E, x, y, T_true, off, n_cg=8, n_states=3):
@@ -278,16 +411,15 @@ def training_examples(rng, E, x, y, T_true, off, n_cg=8, n_states=3):
             Tc = rng.uniform(0.0, 1.0) * T_true + e
         xc, _, _ = ops.CGsolver(y, E, Tc, x=None, nIt=n_cg, tolType="RelativeResidual2Error", tol=0.0)
         sums = lm_sums(y, E, xc, Tc)
-        F = features(sums)
-        tgt = T_true - Tc
-        tgt[:, 3:] *= DEG
-        out.append((F, tgt.astype(np.float32)))
+        out.append((features(sums), relative_target(T_true, Tc)))
     return out
 
 
-def train(net: MotionNet, data, epochs=30, batch=512, lr=1e-3, seed=0, log=None):
-    """Adam on the L2 loss to the true update.  data: list of (F [S x 13],
-    target [S x 6]) scans; a batch holds whole scans up to `batch` rows."""
+def train(net: MotionNet, data, epochs=30, batch=512, lr=1e-3, seed=0, log=None, residual=True):
+    """Adam on the L2 loss to the target.  data: list of (F [S x 13],
+    t [S x 6]) scans; a batch holds whole scans up to `batch` rows.  With
+    `residual` the network's target is net_target(F, t) (t less the Newton
+    step), otherwise t itself."""
     rng = np.random.default_rng(seed)
     dev = net.p.device
     losses = []
@@ -300,7 +432,7 @@ def train(net: MotionNet, data, epochs=30, batch=512, lr=1e-3, seed=0, log=None)
             while i < len(order) and (so[-1] == 0 or so[-1] + data[order[i]][0].shape[0] <= batch):
                 F, t = data[order[i]]
                 Fs.append(F)
-                Ts.append(t)
+                Ts.append(net_target(F, t) if residual else t)
                 so.append(so[-1] + F.shape[0])
                 i += 1
             X = torch.from_numpy(np.concatenate(Fs)).to(dev)

@@ -1,13 +1,23 @@
 """Train the learned motion proposal on scans made by the method's own
-simulator (disorder_amd.learned.simulate_scan): phantoms at 64^3 and
+simulator (disorder_amd.learned.simulate_scan; with --pyramid, the states the
+plain pyramid visits on those scans, learned.pyramid_examples): phantoms at 64^3 and
 128^3, 8 coils, 32-64 DISORDER shots, random-walk motion with step sigma
 0.2-1.5 deg / voxel plus 5 % abrupt jumps up to 10 deg / 10 voxels.  Every
-scan contributes the features and true updates at `--states` current
-estimates.  Training: Adam, lr 1e-3, batches of up to 512 shot rows of whole
-scans, L2 loss to the true update; all on the GPU through
-libdisorder_hip.so.
+scan contributes the features and relative updates (the true update less
+its mean over shots) at `--states` current estimates.  Training: Adam, lr
+1e-3, batches of up to 512 shot rows of whole scans, L2 loss of the Newton
+step plus the network's output to the relative update (learned.net_target);
+all on the GPU through libdisorder_hip.so.  The held-out errors compare the
+proposal with no update and with the Newton step alone.
 
-  python -m disorder_amd.train_proposal --scans 20000 --out disorder_amd/weights/motion_net.npy
+The committed weights (disorder_amd/weights/motion_net.npy) come from the
+states of 2,000 pyramid solves, kept where the plain solve ends within 1
+voxel / 1 degree of the true motion:
+
+  python -m disorder_amd.train_proposal --pyramid --scans 1000 --seed 202 --save-data p2.npz
+  python -m disorder_amd.train_proposal --pyramid --scans 1000 --seed 203 --save-data p3.npz
+  python -m disorder_amd.train_proposal --load-data p2.npz p3.npz --max-err 1.0 \
+      --out disorder_amd/weights/motion_net.npy
 """
 from __future__ import annotations
 
@@ -38,33 +48,60 @@ def main(argv=None):
     ap.add_argument("--save-data", default=None, help="write the simulated examples (.npz, float16) and stop")
     ap.add_argument("--load-data", nargs="*", default=None, help="train on saved examples instead of simulating")
     ap.add_argument("--init", default=None, help="start from these weights (fine-tuning)")
+    ap.add_argument("--max-err", dest="max_err", type=float, default=None,
+                    help="train only on pyramid examples of scans whose final motion error (voxel / degree, common "
+                         "motion removed) is at most this")
+    ap.add_argument("--pyramid", action="store_true",
+                    help="examples from the states the plain pyramid visits (every level and outer iteration) "
+                         "instead of estimates drawn around the solution")
     a = ap.parse_args(argv)
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(a.seed)
     data = []
+    errs = []  # per example: final motion error of the scan's plain solve (0 for drawn estimates)
     t0 = time.perf_counter()
     if a.load_data:
         for path in a.load_data:
             z = np.load(path, allow_pickle=False)
             F, T, so = z["F"].astype(np.float32), z["T"].astype(np.float32), z["off"]
+            er = z["err"] if "err" in z.files else np.zeros(len(so) - 1)
             data += [(F[so[i]:so[i + 1]], T[so[i]:so[i + 1]]) for i in range(len(so) - 1)]
+            errs += [float(e) for e in er]
+        if a.max_err is not None:
+            keep = [i for i, e in enumerate(errs) if e <= a.max_err]
+            print(f"[train] {len(keep)} of {len(data)} examples from scans solved within {a.max_err}", flush=True)
+            data = [data[i] for i in keep]
+            errs = [errs[i] for i in keep]
         print(f"[train] loaded {len(data)} examples", flush=True)
     else:
         n128 = int(round(a.scans * a.frac128))
+        failed = 0
         for i in range(a.scans):
             n = (128, 128, 128) if i < n128 else (64, 64, 64)
             S = int(rng.choice([32, 64]))
-            E, x, y, T, off, Sm = LN.simulate_scan(rng, n, S, 8, dev)
-            data += LN.training_examples(rng, E, x, y, T, off, n_cg=a.cg, n_states=a.states)
-            E.close()
-            del x, y, Sm
-            if (i + 1) % 500 == 0 or i + 1 == a.scans:
+            if a.pyramid:
+                try:
+                    rows, err = LN.pyramid_examples(rng, n, S, 8, dev, nlevels=3 if n[0] == 64 else 4)
+                    data += rows
+                    errs += [err] * len(rows)
+                except RuntimeError as exc:  # a solve that ends with an error status: the scan is skipped
+                    failed += 1
+                    print(f"[train] scan {i}: {exc}", flush=True)
+            else:
+                E, x, y, T, off, Sm = LN.simulate_scan(rng, n, S, 8, dev)
+                rows = LN.training_examples(rng, E, x, y, T, off, n_cg=a.cg, n_states=a.states)
+                data += rows
+                errs += [0.0] * len(rows)
+                E.close()
+                del x, y, Sm
+            if (i + 1) % (100 if a.pyramid else 500) == 0 or i + 1 == a.scans:
                 print(f"[train] simulated {i + 1}/{a.scans} scans ({time.perf_counter() - t0:.0f} s)", flush=True)
     gen_s = time.perf_counter() - t0
     if a.save_data:
         so = np.concatenate([[0], np.cumsum([d[0].shape[0] for d in data])]).astype(np.int64)
         np.savez_compressed(a.save_data, F=np.concatenate([d[0] for d in data]).astype(np.float16),
-                            T=np.concatenate([d[1] for d in data]).astype(np.float16), off=so)
+                            T=np.concatenate([d[1] for d in data]).astype(np.float16), off=so,
+                            err=np.asarray(errs, np.float32))
         print(json.dumps({"scans": a.scans, "examples": len(data), "rows": int(so[-1]), "gen_s": gen_s}), flush=True)
         return 0
     order = rng.permutation(len(data))
@@ -87,7 +124,7 @@ def main(argv=None):
            "gen_s": gen_s, "train_s": train_s, "epochs": a.epochs, "loss_first": losses[0], "loss_last": losses[-1]}
     if len(Fv):
         Yv = net.forward(torch.from_numpy(Fv).to(dev), so).cpu().numpy()
-        res.update({"val_mse": float(np.mean((Yv - Tv) ** 2)), "val_mse_zero": float(np.mean(Tv ** 2)),
+        res.update({"val_mse": float(np.mean((Fv[:, :6] + Yv - Tv) ** 2)), "val_mse_zero": float(np.mean(Tv ** 2)),
                     "val_mse_newton": float(np.mean((Fv[:, :6] - Tv) ** 2))})
     net.save(a.out)
     print(json.dumps(res), flush=True)

@@ -72,6 +72,7 @@ def test_features_diagonal_newton_step():
     rng = np.random.default_rng(2)
     h = rng.uniform(1, 10, (S, 6))
     d = rng.normal(0, 1, (S, 6))          # true update (vox, rad)
+    d[:, 3:] *= 0.1                       # within the +-20 degree clip
     g = -h * d                            # gradient of a diagonal quadratic at the current point
     sums = np.zeros((S, 28))
     sums[:, 0] = rng.uniform(1, 2, S)
@@ -81,6 +82,29 @@ def test_features_diagonal_newton_step():
     assert np.allclose(F[:, :3], d[:, :3], rtol=1e-5)
     assert np.allclose(F[:, 3:6], d[:, 3:] * 180 / math.pi, rtol=1e-5)
     assert np.allclose(F[:, 6:12].mean(axis=0), np.log(h / h.mean(0)).mean(0), atol=1e-5)
+    # a near-flat direction (tiny curvature) gives a step clipped to +-NEWTON_CLIP
+    sums[0, 7] = 1e-12
+    sums[0, 1] = -1.0
+    assert features(sums)[0, 0] == pytest.approx(20.0)
+
+
+def test_relative_target_and_proposal_centring():
+    """The training target is the true update less its mean over shots (a
+    common offset of the estimate is not in it), and the network learns the
+    target less the Newton step of the features."""
+    from disorder_amd.learned import net_target, relative_target
+    rng = np.random.default_rng(4)
+    S = 9
+    T_true = rng.normal(0, 1, (S, 6))
+    e = rng.normal(0, 0.3, (S, 6))
+    t1 = relative_target(T_true, T_true + e)
+    t2 = relative_target(T_true, T_true + e + rng.normal(0, 5, (1, 6)))  # plus a common offset
+    np.testing.assert_allclose(t1, t2, atol=1e-4)
+    np.testing.assert_allclose(t1.mean(axis=0), 0.0, atol=1e-5)
+    np.testing.assert_allclose(t1[:, :3], -(e[:, :3] - e[:, :3].mean(0)), atol=1e-5)
+    np.testing.assert_allclose(t1[:, 3:], -(e[:, 3:] - e[:, 3:].mean(0)) * 180 / math.pi, atol=1e-4)
+    F = rng.normal(0, 1, (S, 13)).astype(np.float32)
+    np.testing.assert_allclose(net_target(F, t1) + F[:, :6], t1, atol=1e-6)
 
 
 # ---------------------------------------------------------------- GPU
@@ -155,51 +179,74 @@ def test_training_reduces_loss(dev):
         t[1:-1] += 0.25 * (F[:-2, :6] + F[2:, :6])
         data.append((F, t.astype(np.float32)))
     net = MotionNet(seed=0)
-    losses = train(net, data, epochs=40, batch=512, lr=1e-3)
+    losses = train(net, data, epochs=40, batch=512, lr=1e-3, residual=False)
     assert losses[-1] < 0.2 * losses[0], losses[::8]
     assert all(b <= a * 1.05 for a, b in zip(losses[::8], losses[8::8]))
 
 
 @pytest.mark.gpu
 def test_trained_proposal_against_plain_pyramid(dev, capsys):
-    """What the committed trained network gains over the plain method on a
-    small joint reconstruction: proposals are kept only where the shot
-    energy decreases and LM handles the rest, so learned+LM must reach the
-    plain method's NRMSE (no worse than 0.1 % above it: a kept proposal
-    moves the path of the later CG, LM and robust-weight steps, and after
-    four outer iterations both runs still carry the frame offset that the
-    common motion leaves unobservable) and, when no proposal was kept, the
-    plain motion within 0.01 deg / 0.01 voxel; the accepted-proposal count
-    and the time ratio are printed."""
+    """The committed trained network inside the joint pyramid on the
+    reference's 64^3 case (the inputs of tests/golden/small_case.py: 8
+    coils, 16 shots, motion U(+-5 deg, +-3 voxel), 30 dB, 3 levels), plain
+    and with the proposer (first two outer iterations of every level, no
+    retirement).  Measured and printed, by the learned-proposal criterion:
+    outer iterations, LM trials and solve time until the motion error (common
+    motion removed) first lies below 0.1 voxel and 0.1 degree, and the final
+    frame-aligned NRMSE against the ground truth.  Asserted: proposals are
+    kept for some shots (the comparison is not vacuous), both runs reach the
+    motion criterion, and learned+LM ends within 1e-3 of the plain NRMSE
+    (a kept proposal moves the path of every later CG and LM step; the
+    energy test keeps it from degrading the fit)."""
     import os
     import time
     import torch
     import disorder_amd as D
     from disorder_amd import learned as LN
     from disorder_amd import pyramid as PY
-    from disorder_amd import synth
+    from tests.golden import small_case as SC
     assert os.path.exists(LN.WEIGHTS), "trained weights missing"
-    n, S = (64, 48, 64), 16
-    _, k1, k2, off = synth.disorder_sampling(n[0], n[1], S, 4, 4, seed=7)
-    x = synth.phantom(n, 12, seed=8, device=dev)
-    Sm = synth.coil_maps(n, 8, seed=9, device=dev)
-    T = synth.random_walk_motion(S, seed=10, step_deg=1.0, step_vox=1.0)
-    E = D.Encoding(Sm, k1, k2, off)
-    y = synth.add_noise(E.encode(x, T), synth.noise_level(x, Sm, 30.0), seed=11)
-    ng = torch.linalg.vector_norm(x).item()
-    out = {}
-    for name, prop in (("plain", None), ("learned", LN.Proposer(LN.MotionNet.load()))):
+    d = SC.make_inputs()
+
+    def to_dev(a):
+        return D.colmajor(torch.from_numpy(np.asfortranarray(a.astype(np.complex64))).to(dev))
+    y, Sm = to_dev(d["y"]), to_dev(d["Sm"])
+
+    def tr(v, Tm):
+        return D.sincRigidTransform(to_dev(v), Tm, di=1)[..., 0].cpu().numpy()
+
+    def run(prop):
+        hit = {}
+        count = {"outer": 0}
+
+        def cb(li, it, x, T, t):
+            count["outer"] += 1
+            e = LN.relative_error(T, d["T_gt"])
+            if "t" not in hit and np.abs(e[:, :3]).max() < 0.1 and np.abs(e[:, 3:]).max() < 0.1:
+                hit.update(t=t, outer=count["outer"])
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        xr, Tr, info = PY.solveXT(y, Sm, k1, k2, off, n, nlevels=2, n_outer=4, n_cg=6, final_cg=50, proposer=prop)
-        out[name] = (torch.linalg.vector_norm(xr - x).item() / ng, Tr, time.perf_counter() - t0, info, prop)
-    ep, Tp, tp, _, _ = out["plain"]
-    el, Tl, tl, il, prop = out["learned"]
+        x, T, info = PY.solveXT(y, Sm, d["k1"], d["k2"], d["off"], SC.N, callback=cb, proposer=prop, **SC.PYR)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return {"nrmse": SC.aligned_nrmse(x.cpu().numpy(), T, d["x_gt"], d["T_gt"], tr), "wall": wall,
+                "outer": [lv["outer"] for lv in info["levels"]], "lm_trials": info["lm_trials"],
+                "t_crit": hit.get("t"), "outer_crit": hit.get("outer")}
+    def _s(t):
+        return "never" if t is None else f"{t:.3f} s"
+    plain = run(None)
+    prop = LN.Proposer(LN.MotionNet.load(), retire=False)
+    learned = run(prop)
     with capsys.disabled():
-        print(f"\n[learned proposal] accepted {prop.accepted}/{prop.proposed}; NRMSE plain {ep:.6f} learned {el:.6f}; "
-              f"time learned/plain {tl / tp:.2f}")
-    assert el <= ep * (1 + 1e-3)
-    assert np.abs(Tl[:, :3] - Tp[:, :3]).max() <= 0.01 or prop.accepted > 0
-    assert np.abs(Tl[:, 3:] - Tp[:, 3:]).max() * 180 / np.pi <= 0.01 or prop.accepted > 0
+        print(f"\n[learned proposal, 64^3 reference case] kept {prop.accepted}/{prop.proposed}; "
+              f"plain: outer {plain['outer']}, LM trials {plain['lm_trials']}, motion < 0.1 after "
+              f"{plain['outer_crit']} outer / {_s(plain['t_crit'])}, NRMSE {plain['nrmse']:.5f}, "
+              f"{plain['wall']:.3f} s; learned: outer {learned['outer']}, LM trials {learned['lm_trials']}, motion "
+              f"< 0.1 after {learned['outer_crit']} outer / {_s(learned['t_crit'])}, NRMSE {learned['nrmse']:.5f}, "
+              f"{learned['wall']:.3f} s")
+    assert prop.accepted > 0
+    assert plain["t_crit"] is not None and learned["t_crit"] is not None
+    assert learned["nrmse"] <= plain["nrmse"] + 1e-3
 
 
 def test_proposer_retires_after_a_round_that_keeps_nothing(monkeypatch):
