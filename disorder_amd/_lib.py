@@ -50,6 +50,7 @@ SIGNATURES = {
     "dis_plan_set_weights": (c_int, [c_void_p, c_float_p]),
     "dis_plan_set_fold": (c_int, [c_void_p, c_int32, c_int32]),
     "dis_plan_set_readout_roi": (c_int, [c_void_p, c_int32, c_int32]),
+    "dis_plan_set_lm_filter": (c_int, [c_void_p, c_float_p, c_int32]),
     "dis_coil_gram_noalloc": (c_int, [c_void_p, c_void_p, ctypes.c_int64, c_int32, c_void_p]),
     "dis_stream_probe": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int64, c_int32, c_int32, c_int32,
                                  c_double_p]),

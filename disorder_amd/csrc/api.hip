@@ -62,6 +62,8 @@ struct dis_plan {
   std::vector<int> hoff;
   int fold[2] = {0, 0};   // acquired phase-encode grid under SENSE folding
   int roi[2] = {0, 0};    // readout range [lo, hi) of the LM sums (E.nF); {0, 0} = all
+  float* Fd = nullptr;    // [Ns, fdn] fractional finite-difference weights of LMsolver (E.Fd), or null
+  int fdn = 0;
 };
 
 namespace {
@@ -486,6 +488,7 @@ int dis_plan_destroy(dis_plan* p) {
   if (p->rowptr) (void)hipFree(p->rowptr);
   if (p->csr) (void)hipFree(p->csr);
   if (p->sord) (void)hipFree(p->sord);
+  if (p->Fd) (void)hipFree(p->Fd);
   delete p;
   return DIS_OK;
 }
@@ -530,6 +533,30 @@ int dis_plan_set_readout_roi(dis_plan* p, int32_t lo, int32_t hi) {
   return DIS_OK;
 }
 
+int dis_plan_set_lm_filter(dis_plan* p, const float* Fd, int32_t nF) {
+  if (!p) return DIS_E_INVALID;
+  dis_ctx* c = p->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!Fd) {
+    if (p->Fd) HIPCHK(c, hipFree(p->Fd));
+    p->Fd = nullptr;
+    p->fdn = 0;
+    return DIS_OK;
+  }
+  if (nF <= 0 || nF > p->g.n[2])
+    return fail(c, DIS_E_INVALID, "dis_plan_set_lm_filter: need 0 < nF <= n3");
+  const size_t cnt = (size_t)std::max(1, p->g.nsamples) * nF;
+  if (p->Fd && p->fdn != nF) {
+    HIPCHK(c, hipFree(p->Fd));
+    p->Fd = nullptr;
+  }
+  if (!p->Fd) HIPCHK(c, hipMalloc(&p->Fd, cnt * sizeof(float)));
+  p->fdn = nF;
+  if (p->g.nsamples > 0)
+    HIPCHK(c, hipMemcpy(p->Fd, Fd, (size_t)p->g.nsamples * nF * sizeof(float), hipMemcpyHostToDevice));
+  return DIS_OK;
+}
+
 int dis_plan_set_weights(dis_plan* p, const float* We) {
   if (!p) return DIS_E_INVALID;
   dis_ctx* c = p->ctx;
@@ -565,6 +592,8 @@ OpView plan_view(dis_plan* p) {
   v.fold[1] = p->fold[1];
   v.roi[0] = p->roi[0];
   v.roi[1] = p->roi[1];
+  v.Fd = p->Fd;
+  v.fdn = p->fdn;
   return v;
 }
 

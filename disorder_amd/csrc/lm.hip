@@ -1,7 +1,10 @@
 // Device kernels of the LM motion step (Reconstruction/LMsolver.m:52-141):
-// per-state residual energy, gradient and Gauss-Newton Hessian reductions,
-// and the damped 6x6 normal-equation solves, all in FP64.
+// per-state residual energy, gradient and Gauss-Newton Hessian reductions
+// and the damped 6x6 normal-equation solves, in FP64; and the fractional
+// finite-difference readout weighting (E.Fd, :88-91), in FP32.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 #include "lm.hpp"
@@ -40,7 +43,7 @@ __global__ __launch_bounds__(kThreads) void lm_reduce_kernel(LmReduceArgs a) {
     const long long c = rest / a.i3n;
     const long long i3 = a.i3lo + (rest - c * a.i3n);
     const long long yi = j + (long long)a.Ns * (i3 + (long long)a.n3 * c);
-    const float2 Y = a.Yr[yi], d = a.y[yi];
+    const float2 Y = a.Yr[yi], d = a.y ? a.y[yi] : make_float2(0.f, 0.f);
     const float2 r = make_float2(Y.x - d.x, Y.y - d.y);
     acc[0] += (double)r.x * r.x + (double)r.y * r.y;
     float2 J[6];
@@ -128,7 +131,83 @@ __global__ void lm_solve_kernel(const double* sums, const double* w, int nlist, 
   for (int i = 0; i < 6; ++i) step[(long long)sl * 6 + i] = sc * x[i];
 }
 
+// grid (tiles of P samples, nlist): the tile's P consecutive samples of one
+// listed state, every coil in turn, staged in LDS with the samples fastest
+// (coalesced loads along the compact rows) and transformed along the readout
+// (Reconstruction/LMsolver.m:88-91 with fftGPU(., 3), Methods/fftGPU.m:28-45)
+__global__ __launch_bounds__(kThreads) void readout_filter_kernel(ReadoutFilterArgs a, int P) {
+  constexpr int kE = kMaxTile / kThreads;
+  __shared__ float2 lds[kMaxTile];
+  __shared__ float2 tw[1024];
+  const int tid = threadIdx.x;
+  const int s = a.slist[blockIdx.y];
+  const int o0 = a.off[s], o1 = a.off[s + 1];
+  const int j0 = o0 + blockIdx.x * P;
+  if (j0 >= o1) return;  // uniform: the whole tile lies past the state's samples
+  const int n = a.nF, tot = P * n;
+  const int Pv = min(P, o1 - j0);
+  for (int k = tid; k < n; k += kThreads) tw[k] = a.tw[k];
+  float en[kE];
+#pragma unroll
+  for (int i = 0; i < kE; ++i) en[i] = 0.f;
+  for (int c = 0; c < a.C; ++c) {
+    const long long base = (long long)j0 + (long long)a.Ns * (a.lo + (long long)a.n3 * c);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) {
+        const int k = e / P, p = e - k * P;
+        float2 v = make_float2(0.f, 0.f);
+        if (p < Pv) {
+          const long long idx = base + p + (long long)a.Ns * k;
+          v = a.Yr[idx];
+          if (a.y) {
+            const float2 d = a.y[idx];
+            v = make_float2(v.x - d.x, v.y - d.y);
+          }
+        }
+        lds[e] = v;
+      }
+    }
+    __syncthreads();
+    fft_tile<false>(lds, tw, a.plan, P, tid);
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) {
+        const int k = e / P, p = e - k * P;
+        if (p < Pv) {
+          const float f = a.Fd[j0 + p + (long long)a.Ns * k];
+          const float2 v = make_float2(lds[e].x * f, lds[e].y * f);
+          a.Yr[base + p + (long long)a.Ns * k] = v;
+          en[i] += v.x * v.x + v.y * v.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (a.epart) {
+#pragma unroll
+    for (int i = 0; i < kE; ++i) {
+      const int e = tid + i * kThreads;
+      if (e < tot) {
+        const int k = e / P, p = e - k * P;
+        if (p < Pv) a.epart[j0 + p + (long long)a.Ns * (a.lo + k)] = en[i];
+      }
+    }
+  }
+}
+
 }  // namespace
+
+hipError_t launch_readout_filter(ReadoutFilterArgs a, int nlist, int maxnec, hipStream_t st) {
+  if (a.nF <= 0 || a.nF > 1024 || a.lo < 0 || a.lo + a.nF > a.n3 || nlist <= 0) return hipErrorInvalidValue;
+  int P = 1;
+  while (2 * P * a.nF <= kMaxTile && 2 * P <= 64) P *= 2;
+  const int tiles = std::max(1, (maxnec + P - 1) / P);
+  hipLaunchKernelGGL(readout_filter_kernel, dim3(tiles, nlist), dim3(kThreads), 0, st, a, P);
+  return hipGetLastError();
+}
 
 hipError_t launch_lm_reduce(const LmReduceArgs& a, int nlist, double* partial_out, double* sums_out,
                             hipStream_t st) {

@@ -748,6 +748,20 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
   // readout ROI E.nF (LMsolver.m:33,68,98): sums over planes [ro_lo, ro_hi)
   const int ro_lo = v.roi[1] > 0 ? v.roi[0] : 0;
   const int ro_hi = v.roi[1] > 0 ? v.roi[1] : g.n[2];
+  // fractional finite-difference weights E.Fd (LMsolver.m:88-91,101-104,186-189):
+  // residuals, Jacobians and trial residuals are weighted along the ROI's
+  // readout DFT before their sums
+  const bool filt = v.Fd != nullptr;
+  FftPlan fd_plan{};
+  const float2* fd_tw = nullptr;
+  if (filt) {
+    if (v.fold[0] > 0) return ctx_fail(c, DIS_E_UNSUPPORTED, "LMsolver: fractional finite-difference weights on a folded plan");
+    if (v.fdn != ro_hi - ro_lo)
+      return ctx_fail(c, DIS_E_INVALID, "LMsolver: the fractional finite-difference weights do not match the readout ROI");
+    if (make_plan(v.fdn, &fd_plan) != DIS_OK) return ctx_fail(c, DIS_E_UNSUPPORTED, "LMsolver: readout ROI above 1024 planes");
+    rc = ctx_twiddles(c, v.fdn, &fd_tw);
+    if (rc) return rc;
+  }
   {
     const int mc = lm_check_motion(NM, T, w);
     if (mc) return ctx_fail(c, DIS_E_INVALID, lm_status_text(mc));
@@ -803,6 +817,25 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
     rc = fold_setup(c, v, maxnec, &fold_a);
     if (rc) return rc;
   }
+  int maxnec_all = 0;
+  for (int s = 0; s < g.nstates; ++s) maxnec_all = std::max(maxnec_all, v.hoff[s + 1] - v.hoff[s]);
+  auto readout_filter = [&](float2* Y, const float2* ysub, float* ep, const int* dlist) {
+    ReadoutFilterArgs fa{};
+    fa.Yr = Y;
+    fa.y = ysub;
+    fa.Fd = v.Fd;
+    fa.epart = ep;
+    fa.off = v.off;
+    fa.slist = dlist;
+    fa.Ns = g.nsamples;
+    fa.n3 = g.n[2];
+    fa.C = g.ncoils;
+    fa.lo = ro_lo;
+    fa.nF = ro_hi - ro_lo;
+    fa.tw = fd_tw;
+    fa.plan = fd_plan;
+    return fa;
+  };
   auto coil_args = [&](const int* dlist, const float2* U, int mode) {
     CoilArgs a = fold_a;
     a.U = U;
@@ -861,9 +894,15 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
       ag.yout = J[gi];
       DCHK(c, launch_coil(ag, COIL_ENCODE, sb, stream));
     }
+    if (filt) {  // residual and Jacobians through fftGPU(., 3) .* E.Fd (LMsolver.m:88-91, 101-104)
+      for (int q = 0; q < 7; ++q) {
+        ReadoutFilterArgs fa = readout_filter(q == 0 ? Yr : J[q - 1], q == 0 ? y : nullptr, nullptr, dlist);
+        DCHK(c, launch_readout_filter(fa, sb, maxnec_all, stream));
+      }
+    }
     LmReduceArgs ra{};
     ra.Yr = Yr;
-    ra.y = y;
+    ra.y = filt ? nullptr : y;
     for (int gi = 0; gi < 6; ++gi) ra.J[gi] = J[gi];
     ra.off = v.off;
     ra.slist = dlist;
@@ -925,8 +964,16 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
       const unsigned char* d = nullptr;
       r = ctx_upload_bytes(c, st, &d);
       if (r) return r;
-      CoilArgs a = coil_args((const int*)(d + ol), Tx, COIL_ENERGY);
-      DCHK(c, launch_coil(a, COIL_ENERGY, sb, stream));
+      if (filt) {  // encode, then the weighted residual's energies (LMsolver.m:186-189)
+        CoilArgs a = coil_args((const int*)(d + ol), Tx, COIL_ENCODE);
+        a.yout = Yr;
+        DCHK(c, launch_coil(a, COIL_ENCODE, sb, stream));
+        ReadoutFilterArgs fa = readout_filter(Yr, y, epart, (const int*)(d + ol));
+        DCHK(c, launch_readout_filter(fa, sb, maxnec_all, stream));
+      } else {
+        CoilArgs a = coil_args((const int*)(d + ol), Tx, COIL_ENERGY);
+        DCHK(c, launch_coil(a, COIL_ENERGY, sb, stream));
+      }
       DCHK(c, launch_energy_reduce(epart, g.nsamples, g.n[2], nullptr, ro_lo, ro_hi, esamp, stream));
       std::vector<double> he(std::max(1, g.nsamples));
       DCHK(c, hipMemcpyAsync(he.data(), esamp, sizeof(double) * g.nsamples, hipMemcpyDeviceToHost, stream));

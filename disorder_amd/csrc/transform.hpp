@@ -43,6 +43,8 @@ struct OpView {
   const int* hoff;  // host copy of the offsets
   int fold[2];      // acquired phase-encode grid when SENSE folding is on, else {0, 0}
   int roi[2];       // readout planes [roi[0], roi[1]) of LMsolver (E.nF), or {0, 0} for all
+  const float* Fd = nullptr;  // [Ns, fdn] fractional finite-difference weights of LMsolver (E.Fd), or null
+  int fdn = 0;
 };
 
 // Fills the SENSE-folding fields of a coil-stage argument block (no-op

@@ -155,6 +155,31 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     check(b->ctx, dis_plan_set_fold(b->plan, static_cast<int32_t>(v[0]), static_cast<int32_t>(v[1])));
     return;
   }
+  if (cmd == "plan_set_readout_roi") {  // E.nF: disorder_mex('plan_set_readout_roi', plan, E.nF)
+    PlanBox* b = handle<PlanBox>(prhs[1]);
+    int32_t lo = 0, hi = 0;
+    const size_t m = mxGetNumberOfElements(prhs[2]);
+    if (m > 0) {  // E.nF is one contiguous 1-based range (solveXT.m:345)
+      const double* v = mxGetPr(prhs[2]);
+      lo = static_cast<int32_t>(v[0]) - 1;
+      hi = static_cast<int32_t>(v[m - 1]);
+      if (hi - lo != static_cast<int32_t>(m)) mexErrMsgIdAndTxt("disorder:arg", "E.nF must be a contiguous range");
+    }
+    check(b->ctx, dis_plan_set_readout_roi(b->plan, lo, hi));
+    return;
+  }
+  if (cmd == "plan_set_lm_filter") {  // E.Fd: disorder_mex('plan_set_lm_filter', plan, single(GRes))
+    PlanBox* b = handle<PlanBox>(prhs[1]);
+    if (mxIsEmpty(prhs[2])) {
+      check(b->ctx, dis_plan_set_lm_filter(b->plan, nullptr, 0));
+      return;
+    }
+    if (!mxIsSingle(prhs[2]) || mxIsComplex(prhs[2]) || static_cast<int64_t>(mxGetM(prhs[2])) != b->Ns)
+      mexErrMsgIdAndTxt("disorder:arg", "E.Fd must be real single [nsamples x 1 x nF]");
+    const int32_t nF = static_cast<int32_t>(mxGetNumberOfElements(prhs[2]) / mxGetM(prhs[2]));
+    check(b->ctx, dis_plan_set_lm_filter(b->plan, static_cast<const float*>(mxGetData(prhs[2])), nF));
+    return;
+  }
   if (cmd == "plan_destroy") {
     PlanBox* b = handle<PlanBox>(prhs[1]);
     check(b->ctx, dis_plan_destroy(b->plan));

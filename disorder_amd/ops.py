@@ -241,6 +241,22 @@ class Encoding:
         check(self.ctx.lib.dis_plan_set_readout_roi(self.h, lo, hi), self.ctx.h)
         self.nF = None if nF is None else (lo, hi)
 
+    def set_lm_filter(self, Fd):
+        """E.Fd (Reconstruction/LMsolver.m:88-91,101-104,186-189): per-sample
+        fractional finite-difference weights [Ns x nF] along the readout DFT
+        of the ROI (nF = its length, n3 without one), applied by LMsolver
+        and lm_partials; None clears."""
+        if Fd is None:
+            check(self.ctx.lib.dis_plan_set_lm_filter(self.h, None, 0), self.ctx.h)
+            self.Fd = None
+            return
+        Fd = np.asfortranarray(np.asarray(Fd, dtype=np.float32))
+        if Fd.ndim != 2 or Fd.shape[0] != self.Ns:
+            raise ValueError("Fd must be [nsamples x nF]")
+        self.Fd = Fd
+        check(self.ctx.lib.dis_plan_set_lm_filter(self.h, Fd.ctypes.data_as(_lib.c_float_p), Fd.shape[1]),
+              self.ctx.h)
+
     def _T(self, T):
         if self.nmotion == 0:
             return np.zeros(6), None

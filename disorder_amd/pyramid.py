@@ -68,6 +68,48 @@ def readout_roi(n3, fat_shift_dir="H", redFOV=1.0 / 3.0):
     return 0, int(n3)
 
 
+def fractional_weights(n, nF, sp, order, k1, k2):
+    """E.Fd of Reconstruction/solveXT.m:353-354 at the samples: the
+    'FractionalFiniteDiscreteIso' filter of Methods/buildFilter.m:58-68 on
+    the grid [n1, n2, nF] (level phase-encode grid, readout ROI length) with
+    spacing sp = N ./ n: k_d = 2 pi m_d / (sp_d N_d) for the FFT-order
+    frequency m_d (generateGrid centred at ceil((N+1)/2), then ifftshift,
+    :120), radius |k| clipped at pi, (2 |sin(|k|/2)|)^order, and the zero
+    frequency set to the largest value of the rest of the grid (:134);
+    buildHarmonicSampling (Methods/buildHarmonicSampling.m:35,42) takes the
+    (k1, k2) row of every sample.  The largest off-zero value comes from the
+    radius extreme that the order favours (the filter is monotone in the
+    radius up to the clip): the grid corner for order > 0, the nearest
+    neighbour of the zero frequency for order < 0.  Returns [Ns, nF]
+    float32 (sample fastest when stored column-major)."""
+    Nd = [int(n[0]), int(n[1]), int(nF)]
+    k1 = np.asarray(k1, np.int64)
+    k2 = np.asarray(k2, np.int64)
+    sc = [2.0 * math.pi / (float(sp[d]) * Nd[d]) for d in range(3)]
+
+    def freqs(m):
+        i = np.arange(m)
+        return np.where(i < (m + 1) // 2, i, i - m).astype(np.float64)
+
+    def g(r):
+        with np.errstate(divide="ignore"):
+            return (2.0 * np.abs(np.sin(np.minimum(r, math.pi) / 2.0))) ** order
+
+    a1 = (freqs(Nd[0]) * sc[0])[k1]
+    a2 = (freqs(Nd[1]) * sc[1])[k2]
+    a3 = freqs(Nd[2]) * sc[2]
+    F = g(np.sqrt(a1[:, None] ** 2 + a2[:, None] ** 2 + a3[None, :] ** 2))
+    live = [d for d in range(3) if Nd[d] > 1]
+    if order > 0:
+        dc = g(math.sqrt(sum((Nd[d] // 2 * sc[d]) ** 2 for d in live)))
+    elif order < 0 and live:
+        dc = g(min(sc[d] for d in live))
+    else:
+        dc = 1.0
+    F[(k1 == 0) & (k2 == 0), 0] = dc
+    return F.astype(np.float32)
+
+
 def cg_iterations(li, it, sizes, nX_prev, nIt2):
     """CG iterations nX of an outer iteration while motion is estimated
     (Reconstruction/solveXT.m:417-425): nIt(2) on the first outer iteration
@@ -110,7 +152,7 @@ class Level:
     motion, data and preconditioner."""
 
     def __init__(self, y, S, k1, k2, off, N, n, ctx, lm_coils=None, rows0=None, gibbs=0.0, NY=None, ny=None,
-                 nF=None):
+                 nF=None, fraction_order=0.0):
         self.n = list(n)
         # acquired phase-encode grid (full and at this level); below the
         # image grid the level's encodings fold (SENSE, E.Uf / EH.Ub)
@@ -138,6 +180,13 @@ class Level:
         # of the energies (EH.Mbe, solveXT.m:345-346)
         self.nF = None if nF is None or tuple(nF) == (0, n[2]) else (int(nF[0]), int(nF[1]))
         self.E.set_readout_roi(self.nF)
+        if fraction_order:  # E.Fd of the LMsolver pair (solveXT.m:353-354, :506)
+            if self.ny is not None:
+                raise ValueError("fractional finite-difference motion estimation needs unfolded data "
+                                 "(buildHarmonicSampling reshapes the image-grid filter onto the acquired grid)")
+            lo, hi = (0, n[2]) if self.nF is None else self.nF
+            sp = [N[d] / n[d] for d in range(3)]
+            self.E.set_lm_filter(fractional_weights(n, hi - lo, sp, fraction_order, l1, l2))
         self.Mbe = None
         if self.nF is not None:
             self.Mbe = np.zeros(n[2], np.float32)
@@ -200,7 +249,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
             robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None,
             gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0,
-            tolerSolve=1e-3):
+            tolerSolve=1e-3, fraction_order=0.0):
     """Returns (x [N] on the device, T [states x 6], info).  callback(level,
     it, x, T, t_solve) runs after every outer iteration (and after a level's
     final CG, with outer = last + 1) outside the timed region
@@ -252,7 +301,11 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     (solveXT.m:345-346, :443); a direction other than 'F' or 'H' keeps
     every readout plane.
     tolerSolve: the 'Energy' tolerance of the closing CG of a level
-    (parXT.tolerSolve, IO/disorderAlgorithm.m:108; solveXT.m:177)."""
+    (parXT.tolerSolve, IO/disorderAlgorithm.m:108; solveXT.m:177).
+    fraction_order: parXT.fractionOrder (0, off, by default,
+    IO/disorderAlgorithm.m:114): nonzero gives every LMsolver call the
+    fractional finite-difference weights E.Fd of that order
+    (fractional_weights, solveXT.m:353-354, :506); needs unfolded data."""
     import torch
     from . import trajectory as TJ
     from .sampling import harmonic_sampling_repeats
@@ -321,7 +374,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             raise NotDisorder(f"level {li} ({n[0]}x{n[1]}x{n[2]}): states {empty.tolist()[:8]} have no samples; "
                               "probably not DISORDER, returning without performing corrections")
         L = Level(y, S, k1c, k2c, offc, N, n, ctx, lm_coils=ev[1], rows0=rows0, gibbs=gibbs, NY=NY, ny=ny,
-                  nF=readout_roi(n[2], fat_shift_dir, redFOV))
+                  nF=readout_roi(n[2], fat_shift_dir, redFOV), fraction_order=fraction_order)
         full_res = list(n) == list(N)
         sweep_of_sample = sweep_of_state[np.repeat(np.arange(nstates), np.diff(L.off))]
         if x is None:

@@ -116,6 +116,23 @@ int dis_plan_set_fold(dis_plan* plan, int32_t ny1, int32_t ny2);
  * Only a contiguous range is supported (solveXT builds E.nF as one). */
 int dis_plan_set_readout_roi(dis_plan* plan, int32_t lo, int32_t hi);
 
+/* Fractional finite-difference motion estimation (E.Fd of
+ * Reconstruction/LMsolver.m:88-91,101-104,186-189; built per level by
+ * Reconstruction/solveXT.m:353-354 and handed to LMsolver at :506 when
+ * parXT.fractionOrder ~= 0): with weights set, dis_lm and dis_lm_partials
+ * pass the residual of every state, its six Jacobians and every trial
+ * residual through the unnormalised nF-point DFT along the readout planes
+ * of the ROI (fftGPU along dim 3) and multiply them by Fd before the
+ * energies, gradients and Hessians are summed.  Fd: host [nsamples x nF]
+ * float, sample fastest, in the plan's sample order (the filter of
+ * buildFilter(..., 'FractionalFiniteDiscreteIso', ...) at each sample's
+ * phase-encode position, as buildHarmonicSampling samples it); nF must equal
+ * the readout ROI's length (n3 without one) when LM runs.  NULL clears.
+ * Encode, decode, E^H E, energies and CG are unaffected.  Not available on a
+ * SENSE-folded plan (the reference's buildHarmonicSampling cannot reshape
+ * an image-grid filter onto a smaller acquired grid either). */
+int dis_plan_set_lm_filter(dis_plan* plan, const float* Fd, int32_t nF);
+
 /* y = E x  (Reconstruction/encode.m:1).  T: host [nmotion x 6]. */
 int dis_encode(dis_plan* plan, const double* T, const void* x, void* y);
 /* x = E^H (We .* y)  (Reconstruction/decode.m:1). */

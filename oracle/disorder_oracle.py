@@ -784,13 +784,54 @@ def _crop_readout(a, nF, axis):
     return a[tuple(idx)]
 
 
-def lm_partials(y, x, Smaps, samp, T, Nfull, states=None, nF=None):
+def fractional_filter(N, sp, order):
+    """Methods/buildFilter.m 'FractionalFiniteDiscreteIso' on the grid N
+    (restated literally over the whole grid): kGrid = generateGrid(N, gpu,
+    2 pi i ./ sp, ceil((N+1)/2)) (:30), x = sum over dimensions with N > 1
+    of |kGrid|^2 (:59-61), sqrt (:62), clipped at pi (:64), 2 |sin(x/2)|
+    (:65-66), x.^order (:68), ifftshift (:120), then x(1) = max(x(2:end))
+    (:134: the zero frequency takes the largest value of the rest)."""
+    N = [int(v) for v in N]
+    x = np.zeros(tuple(N))
+    for m, n in enumerate(N):
+        if n > 1:
+            g = generate_grid([n], [2 * np.pi / sp[m]], [math.ceil((n + 1) / 2)])[0]  # |2 pi i / sp| scale
+            x = x + np.abs(g.reshape([-1 if d == m else 1 for d in range(len(N))])) ** 2
+    x = np.sqrt(x)
+    x[x > np.pi] = np.pi
+    x = 2 * np.abs(np.sin(x / 2))
+    with np.errstate(divide="ignore"):
+        x = x ** order
+    x = np.fft.ifftshift(x)
+    flat = x.reshape(-1)
+    flat[0] = np.max(flat[1:])
+    return flat.reshape(x.shape)
+
+
+def fractional_samples(F3, samp):
+    """GSt of Methods/buildHarmonicSampling.m:35,42: the filter rows at the
+    samples' phase-encode positions, [Ns, nF]."""
+    return F3[np.asarray(samp.k1), np.asarray(samp.k2), :]
+
+
+def _weighted(a, Fd_rows):
+    """fftGPU(a, 3) .* E.Fd (Reconstruction/LMsolver.m:89-90, 102-103):
+    unnormalised DFT along the readout of [samples, nF, C] rows, times the
+    per-sample weights [samples, nF]."""
+    if Fd_rows is None:
+        return a
+    return fft_dim(a, 1) * Fd_rows[:, :, None]
+
+
+def lm_partials(y, x, Smaps, samp, T, Nfull, states=None, nF=None, Fd=None):
     """Per-state residual energy, gradient dG and Gauss-Newton Hessian dH
     (Reconstruction/LMsolver.m:52-122), all states transformed (NMs = S).
     nF: readout ROI (lo, hi) or None: the maps and data are cropped to it
     (:33), and so are the transformed image (:68) and its derivatives (:98)
     before encoding; energies are still divided by numel(y) of the uncropped
-    data (:32).  Returns EnPrev (S,), dG (S,6), dH (S,21)."""
+    data (:32).  Fd: E.Fd [Ns, nF] or None: residual and Jacobians pass
+    through the readout DFT and the weights first (:88-91, :101-104).
+    Returns EnPrev (S,), dG (S,6), dH (S,21)."""
     T = np.atleast_2d(np.asarray(T, np.float64))
     S = T.shape[0]
     states = range(S) if states is None else states
@@ -807,9 +848,11 @@ def lm_partials(y, x, Smaps, samp, T, Nfull, states=None, nF=None):
         G = sinc_rigid_transform_gradient(xB, f)
         sub = _one_state(samp, s)
         Tz = np.zeros((1, 6))
-        r = encode(_crop_readout(xt[..., 0], nF, 2), Smaps, sub, Tz, Nfull) - y[samp.state(s)[0]]
+        rows = samp.state(s)[0]
+        Fr = None if Fd is None else Fd[rows]
+        r = _weighted(encode(_crop_readout(xt[..., 0], nF, 2), Smaps, sub, Tz, Nfull) - y[rows], Fr)
         En[s] = float(np.sum(np.abs(r) ** 2)) / NElY
-        J = [encode(_crop_readout(G[g][..., 0], nF, 2), Smaps, sub, Tz, Nfull) for g in range(6)]
+        J = [_weighted(encode(_crop_readout(G[g][..., 0], nF, 2), Smaps, sub, Tz, Nfull), Fr) for g in range(6)]
         for g in range(6):
             dG[s, g] = float(np.sum(np.real(J[g] * np.conj(r))))
         for h, (i, j) in enumerate(HA):
@@ -823,9 +866,10 @@ def _one_state(samp, s):
                     None if samp.We is None else samp.We[sl])
 
 
-def lm_state_energy(y, x, Smaps, samp, T, Nfull, states, nF=None):
+def lm_state_energy(y, x, Smaps, samp, T, Nfull, states, nF=None, Fd=None):
     """Trial energies of LMsolver.m:151-192: transform (:166), readout crop
-    (:167), encode minus the cropped data (:171), over numel(y) (:32)."""
+    (:167), encode minus the cropped data (:171), with E.Fd the readout DFT
+    and weights (:186-189), over numel(y) (:32)."""
     T = np.atleast_2d(np.asarray(T, np.float64))
     NElY = y.size
     Sc, yc = _crop_readout(Smaps, nF, 2), _crop_readout(y, nF, 1)
@@ -833,7 +877,9 @@ def lm_state_energy(y, x, Smaps, samp, T, Nfull, states, nF=None):
     for s in states:
         sub = _one_state(samp, s)
         xt = rigid_forward(x, T[s:s + 1], x.shape, Nfull)[..., 0] if np.any(T[s] != 0) else x
-        r = encode(_crop_readout(xt, nF, 2), Sc, sub, np.zeros((1, 6)), Nfull) - yc[samp.state(s)[0]]
+        rows = samp.state(s)[0]
+        r = encode(_crop_readout(xt, nF, 2), Sc, sub, np.zeros((1, 6)), Nfull) - yc[rows]
+        r = _weighted(r, None if Fd is None else Fd[rows])
         out[s] = float(np.sum(np.abs(r) ** 2)) / NElY
     return out
 
@@ -855,15 +901,16 @@ def lm_step(dH, dG, w, winit):
     return step if np.all(np.isfinite(step)) else np.zeros(6)
 
 
-def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None, nF=None):
+def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None, nF=None, Fd=None):
     """Reconstruction/LMsolver.m:43-225 for one outer call, with the readout
-    ROI E.nF = nF (lo, hi) or None.  Returns the new (T, w, cT, Enmin)."""
+    ROI E.nF = nF (lo, hi) or None and the fractional finite-difference
+    weights E.Fd = Fd [Ns, nF] or None.  Returns the new (T, w, cT, Enmin)."""
     T = np.array(np.atleast_2d(T), np.float64, copy=True)
     w = np.array(w, np.float64, copy=True)
     cT = np.array(cT, bool, copy=True)
     S = T.shape[0]
     est = [s for s in range(S) if not cT[s]]
-    EnPrev, dG, dH = lm_partials(y, x, Smaps, samp, T, Nfull, est, nF=nF)
+    EnPrev, dG, dH = lm_partials(y, x, Smaps, samp, T, Nfull, est, nF=nF, Fd=Fd)
     En = EnPrev.copy()
     dGEff = np.zeros((S, 6))
     flagw = np.zeros(S, bool)
@@ -874,7 +921,7 @@ def lm_solver(y, x, Smaps, samp, T, Nfull, w, cT, winit, tL, log=None, nF=None):
         Tup = restrict_transform(T + dGEff)
         cfull = cT | flagw
         todo = [s for s in range(S) if not cfull[s]]
-        for s, e in lm_state_energy(y, x, Smaps, samp, Tup, Nfull, todo, nF=nF).items():
+        for s, e in lm_state_energy(y, x, Smaps, samp, Tup, Nfull, todo, nF=nF, Fd=Fd).items():
             En[s] = e
         big = w > 1e10
         En[big] = EnPrev[big]
@@ -1247,7 +1294,7 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
                   traLimXT=(0.05, 0.02), echoesMin=4, final_cg=0, T0=None, callback=None, robust=True,
                   info=None, bin_motion=True, traLimX=(0.05, 0.02), eivaS=None, n_within=0, traj=None,
                   gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0,
-                  tolerSolve=1e-3):
+                  tolerSolve=1e-3, fraction_order=0.0):
     """Joint reconstruction over a spatial pyramid (solveXT.m:266-531) with
     the loop semantics kept: per level the operators are downsampled
     (downsampleOperators.m:34-60), x is resampled from the previous level,
@@ -1287,7 +1334,10 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
     first outer iteration of a repeated full-resolution level), 'fixed' runs
     n_cg every time.  fat_shift_dir, redFOV: the readout ROI E.nF of LMsolver
     and the readout weights EH.Mbe of the energies behind the robust weights
-    (solveXT.m:345-346, :443).
+    (solveXT.m:345-346, :443).  fraction_order: parXT.fractionOrder; nonzero
+    gives LMsolver the weights E.Fd = buildFilter([n(1:2) numel(E.nF)],
+    'FractionalFiniteDiscreteIso', N ./ n, order) at the samples
+    (solveXT.m:353-354, :506).
     Returns (x at full size, T)."""
     from oracle import trajectory_oracle as TO
     NSw = len(off) - 1
@@ -1351,6 +1401,12 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
             return Sampling(l1, l2, offb, ny[0], ny[1], We), Tb
 
         nF = readout_roi(n[2], fat_shift_dir, redFOV)
+        Fd = None
+        if fraction_order:
+            if list(ny) != list(n[:2]):
+                raise ValueError("fractional finite differences need unfolded data")
+            F3 = fractional_filter([n[0], n[1], nF[1] - nF[0]], [N[d] / n[d] for d in range(3)], fraction_order)
+            Fd = fractional_samples(F3, samp)
         Mbe = np.zeros(n[2])
         Mbe[nF[0]:nF[1]] = 1.0
 
@@ -1410,7 +1466,7 @@ def pyramid_solve(yc, Smaps, k1, k2, off, N, nlevels=3, n_outer=3, n_cg=4, winit
             if robust and full_res:
                 weights(c, leff > 0)
             T, w, cT, _ = lm_solver(yl[..., :ev[1]], x, Sl[..., :ev[1]], samp, T, N, w, cT, winit, tL,
-                                    nF=None if nF == (0, n[2]) else nF)
+                                    nF=None if nF == (0, n[2]) else nF, Fd=Fd)
             if callback is not None:
                 callback(li, it, x, T)
             if cT.all():
