@@ -17,8 +17,10 @@ and with the learned proposal), compress (64 -> 8 coil compression),
 batched (six uneven volumes, hipGraph-captured CG), cs (the shearlet
 regulariser), small (the 64^3 reference case end to end, with the oracle's
 run beside it), fold (E^H E with SENSE folding of the phase encodes), stream
-(the device-copy ceiling) and proposal (how often the learned proposal
-lowers a shot's energy, against the distance from the solution).
+(the device-copy ceiling), proposal (how often the learned proposal
+lowers a shot's energy, against the distance from the solution) and lm (one
+LMsolver call and its partial derivatives at 256x192x256, plain and with
+fractional finite-difference weights).
 """
 from __future__ import annotations
 
@@ -44,13 +46,14 @@ def parse():
     ap.add_argument("--cpu-full", dest="cpu_full", action="store_true",
                     help="--workload small: run the oracle's whole 64^3 case (minutes), not a bounded sample")
     ap.add_argument("--workload", choices=["ehe", "pyramid", "compress", "batched", "cs", "small", "fold", "stream",
-                                           "proposal"],
+                                           "proposal", "lm"],
                     default="ehe",
                     help="ehe: E^H E apps/s (default); pyramid: time to 1e-3 NRMSE of the joint CG+LM pyramid; "
                          "compress: 64 -> 8 coil compression of maps and data; batched: six uneven volumes in one "
                          "batched CG with hipGraph-captured iterations; small: the 64^3 reference case (CG with "
                          "known motion, then the 3-level pyramid); fold: E^H E with SENSE folding; stream: device-copy "
-                         "ceiling; proposal: learned-proposal acceptance against distance from the solution")
+                         "ceiling; proposal: learned-proposal acceptance against distance from the solution; lm: "
+                         "LMsolver and its partials at 256x192x256")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--outer", type=int, default=10)
     ap.add_argument("--cg", type=int, default=8)
@@ -240,6 +243,76 @@ def bench_pyramid(args):
                                                       / torch.linalg.vector_norm(xp).item())
         line["learned_vs_plain_time_ratio"] = il["solve_s"] / ip["solve_s"]
     return line
+
+
+def bench_lm(args):
+    """LMsolver at 256x192x256, 8 coils, 64 shots of 768 lines, readout ROI
+    'H' (planes 85..255, solveXT.m:345): the partial derivatives
+    (dis_lm_partials: transform and its gradient's DUAL passes, seven coil
+    encodes of the ROI, the 28 FP64 sums per state) and one update
+    (dis_lm: partials, FP64 6x6 solves, trial transforms and energies),
+    plain and with fractional finite-difference weights of order 1 (E.Fd:
+    the readout DFT and weighting of the residual and the six Jacobians).
+    The sums read Yr, y and the six Jacobians over the ROI once:
+    8 Ns nF C 8 bytes per call."""
+    import numpy as np
+    import torch
+    import disorder_amd as D
+    from disorder_amd import pyramid as PY
+    from disorder_amd import synth
+
+    dev = torch.device("cuda", 0)
+    n, C, S = (256, 192, 256), 8, args.shots
+    t = int(round(math.sqrt(S)))
+    _, k1, k2, off = synth.disorder_sampling(n[0], n[1], S, t, S // t, seed=1)
+    x = synth.phantom(n, 40, seed=2, device=dev)
+    Sm = synth.coil_maps(n, C, seed=3, device=dev)
+    T_gt = synth.random_walk_motion(S, seed=4)
+    ctx = D.Context(0)
+    E = D.Encoding(Sm, k1, k2, off, ctx=ctx)
+    y = E.encode(x, T_gt)
+    rng = np.random.default_rng(5)
+    T0 = T_gt + rng.normal(0.0, 1.0, (S, 6)) * np.array([0.3, 0.3, 0.3, 0.005, 0.005, 0.005])
+    nF = PY.readout_roi(n[2], "H")
+    E.set_readout_roi(nF)
+    Ns, L = int(off[-1]), nF[1] - nF[0]
+
+    def partials(reps):
+        D.lm_partials(y, E, x, T0)
+        torch.cuda.synchronize()
+        ctx.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            D.lm_partials(y, E, x, T0)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / reps
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        nl, kms, kb = prof["pass"]
+        return ms, {"launches": nl / reps, "ms": kms / reps, "algorithmic_GBps": kb / (kms * 1e6) if kms else None}
+
+    def update():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, cT, _, ntr = D.LMsolver(y, E, x, T0, np.full(S, 1e-3), np.zeros(S, bool))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0, int(ntr), int(np.sum(cT))
+
+    out = {}
+    for label, order in (("plain", 0.0), ("fractional_order_1", 1.0)):
+        E.set_lm_filter(PY.fractional_weights(n, L, (1, 1, 1), order, k1, k2) if order else None)
+        ms_p, dual = partials(max(1, args.steps // 2))
+        ms_u, ntr, nconv = update()
+        out[label] = {"lm_partials_ms": ms_p, "dual_passes": dual, "lm_update_ms": ms_u, "trial_rounds": ntr,
+                      "converged_states": nconv}
+    E.close()
+    return {"metric": "LMsolver partial derivatives per call (64 states), ms", "value": out["plain"]["lm_partials_ms"],
+            "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "complex64",
+            "data": "synthetic (seeded phantom, 8 Gaussian coil maps, DISORDER 8x8 sampling, random-walk motion, "
+                    "estimate 0.3 vox / 0.3 deg off)",
+            "config": {"workload": f"LMsolver at {n[0]}x{n[1]}x{n[2]}, {C} coils, {S} shots, readout ROI 'H' "
+                                   f"({L} of {n[2]} planes)"},
+            "lm_reduce_bytes_per_call": 8 * Ns * L * C * 8, **out}
 
 
 def bench_compress(args):
@@ -521,6 +594,9 @@ def main():
         return
     if args.workload == "proposal":
         print(json.dumps(bench_proposal(args)), flush=True)
+        return
+    if args.workload == "lm":
+        print(json.dumps(bench_lm(args)), flush=True)
         return
     import numpy as np
     import torch
