@@ -249,7 +249,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             echoesMin=4, final_cg=0, T0=None, x0=None, callback=None, proposer=None, ctx=None,
             robust=True, bin_motion=True, traLimX=MB.TRA_LIM_X, eivaS=None, n_within=0, traj=None,
             gibbs=0.05, accel=0, NY=None, cg_schedule="nX", fat_shift_dir="H", redFOV=1.0 / 3.0,
-            tolerSolve=1e-3, fraction_order=0.0):
+            tolerSolve=1e-3, fraction_order=0.0, closing_probe=None):
     """Returns (x [N] on the device, T [states x 6], info).  callback(level,
     it, x, T, t_solve) runs after every outer iteration (and after a level's
     final CG, with outer = last + 1) outside the timed region
@@ -305,7 +305,17 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
     fraction_order: parXT.fractionOrder (0, off, by default,
     IO/disorderAlgorithm.m:114): nonzero gives every LMsolver call the
     fractional finite-difference weights E.Fd of that order
-    (fractional_weights, solveXT.m:353-354, :506); needs unfolded data."""
+    (fractional_weights, solveXT.m:353-354, :506); needs unfolded data.
+    closing_probe(level, run, t_solve): called outside the timed region
+    right before every closing CG; run(m) returns, as a new full-size
+    image, the result of the first m iterations of that closing CG (the
+    same operator, weights and preconditioner, from the same image; CG's
+    iterates do not depend on its stop rule; run(m, energy_rule=True) does
+    the closing CG's per-iteration energy checks too, for timing), so a
+    caller can locate a crossing inside the closing CG without changing the
+    solve.
+    info['closing_cg'] lists per closing CG its level, iterations and the
+    solve clock before and after it."""
     import torch
     from . import trajectory as TJ
     from .sampling import harmonic_sampling_repeats
@@ -348,6 +358,34 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
 
     def cg_ops(L, T, final):
         return L.cg(ev[0] if final else ev[2], T, *((res_fine, traLimX) if bin_motion else ()))
+
+    info["closing_cg"] = []
+
+    def closing_cg(li, L, T, x):
+        """The estT(l) = 0 CG of a level (solveXT.m:425-431, nIt(1), 'Energy' tolerSolve)."""
+        nonlocal cb_time
+        Ec, Tc, yc, Se = cg_ops(L, T, True)
+        if closing_probe is not None:
+            torch.cuda.synchronize()
+            tc = time.perf_counter()
+
+            def run(m, energy_rule=False):
+                # energy_rule: the closing CG's own 'Energy' rule with tol 0 (its per-iteration work, no early stop)
+                tt = ("Energy", 0.0) if energy_rule else ("RelativeResidual2Error", 0.0)
+                xm = x.clone() if m <= 0 else ops.CGsolver(yc, Ec, Tc, x=x, nIt=int(m), tolType=tt[0], tol=tt[1],
+                                                           Se=Se)[0]
+                return xm if list(xm.shape) == list(N) else ops.resampling(xm, N, ctx)
+
+            closing_probe(li, run, tc - t0 - cb_time)
+            cb_time += time.perf_counter() - tc
+            torch.cuda.synchronize()
+        tb = time.perf_counter() - t0 - cb_time
+        x, _, nn = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=tolerSolve, Se=Se)
+        if closing_probe is not None:
+            torch.cuda.synchronize()
+        info["closing_cg"].append({"level": li, "iterations": int(nn), "t_before": tb,
+                                   "t_after": time.perf_counter() - t0 - cb_time})
+        return x
 
     levels = pyramid_levels(N, nlevels) + [(list(N), res_fine)] * int(n_within)
     last = len(levels) - 1  # L of :383, lowered to a level that flags no shot (:474)
@@ -402,8 +440,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             if robust and leff > 0 and last_We is not None:
                 L.set_weights(RB.sample_weights(last_We, sweep_of_sample))
             if final_cg:
-                Ec, Tc, yc, Se = cg_ops(L, T, True)
-                x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=tolerSolve, Se=Se)
+                x = closing_cg(li, L, T, x)
             if robust and leff > 0:
                 We = weights(x, True)
                 subT = (We < 1.0)[sweep_of_state]
@@ -455,8 +492,7 @@ def solveXT(y, S, k1, k2, off, N, nlevels=4, n_outer=10, n_cg=8, winit=1e-3, tra
             if cT.all():
                 if it < n_outer:  # the estT(l) = 0 iteration of :425-481
                     if final_cg:
-                        Ec, Tc, yc, Se = cg_ops(L, T, True)
-                        x, _, _ = ops.CGsolver(yc, Ec, Tc, x=x, nIt=final_cg, tolType="Energy", tol=tolerSolve, Se=Se)
+                        x = closing_cg(li, L, T, x)
                     if robust and leff > 0:
                         We = weights(x, True)
                         subT = (We < 1.0)[sweep_of_state]
