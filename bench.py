@@ -949,20 +949,24 @@ def bench_stream(args, nbytes=2 << 30):
     rows = []
     names = {0: "grid-stride 8 B lanes", 1: "grid-stride 16 B lanes", 2: "grid-stride 4x16 B in flight",
              3: "grid-stride 4x16 B nontemporal", 4: "pass tile (32 KiB per workgroup)",
-             5: "pass tile, next tile's loads before the stores"}
+             5: "pass tile, next tile's loads before the stores",
+             6: "readout-axis pass pattern (256x192x256 volumes)", 7: "dim-2 pass pattern (256x192x256 volumes)"}
+    vol = 256 * 192 * 256 * 8
     for v, blocks in [(0, ncu * 8), (1, ncu * 8), (2, ncu * 4), (2, ncu * 8), (2, ncu * 16), (3, ncu * 8),
-                      (3, ncu * 16), (4, 0), (5, ncu * 4), (5, ncu * 8), (5, ncu * 16)]:
+                      (3, ncu * 16), (4, 0), (5, ncu * 4), (5, ncu * 8), (5, ncu * 16), (6, 0), (7, 0)]:
         ms = ctypes.c_double(0)
         _lib.check(ctx.lib.dis_stream_probe(ctx.h, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
                                             nbytes, v, blocks, max(args.steps, 5), ctypes.byref(ms)), ctx.h)
+        moved = 2.0 * (nbytes // vol) * vol if v >= 6 else 2.0 * nbytes
         rows.append({"variant": v, "shape": names[v], "blocks": blocks, "ms": ms.value,
-                     "GBps": 2.0 * nbytes / (ms.value * 1e6)})
+                     "GBps": moved / (ms.value * 1e6)})
         print(f"[stream] {names[v]}, {blocks} blocks: {rows[-1]['GBps']:.0f} GB/s", file=sys.stderr, flush=True)
     tc = stream_copy_rate(dev, nbytes)
-    best = max(rows, key=lambda r: r["GBps"])
+    best = max((r for r in rows if r["variant"] < 6), key=lambda r: r["GBps"])
     return {"metric": "streaming copy rate (read + write)", "value": best["GBps"], "unit": "GB/s", "n_gpus": 1,
             "higher_is_better": True, "dtype": "bytes", "data": "2 GiB device buffers",
-            "config": {"workload": "device-to-device copy of 2 GiB, best of the probe shapes"},
+            "config": {"workload": "device-to-device copy of 2 GiB, best of the flat probe shapes (the two pass "
+                                   "patterns are reported beside them)"},
             "best": best, "frac_of_peak": best["GBps"] / HBM_PEAK_GBS, "probes": rows, "torch_copy": tc}
 
 

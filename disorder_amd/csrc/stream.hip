@@ -11,6 +11,11 @@
 //              16 float2 loads per lane, then 16 stores
 //   variant 5: variant 4 with each workgroup looping over tiles and the loads
 //              of the next tile issued before the stores of the current one
+//   variant 6: the readout-axis register pass's access pattern on whole
+//              256 x 192 x 256 complex64 volumes (16 consecutive dim-1
+//              positions at one dim-2 position, every readout plane)
+//   variant 7: the dim-2 register pass's pattern on the same volumes (16
+//              consecutive dim-1 positions, every dim-2 row of one plane)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -94,6 +99,29 @@ __global__ __launch_bounds__(256) void copy_tile_pipe(const float2* __restrict__
   }
 }
 
+// variants 6 and 7: lane (p = tid % 16, t = tid / 16) copies element
+// t + 16 r (r < R) of the pencil at dim-1 position i10 + p; the tile of
+// workgroup b is dealt XCD-contiguously as in the passes (grid sizes 3072
+// and 4096, multiples of 8)
+constexpr int kPN1 = 256, kPN2 = 192, kPN3 = 256;
+constexpr long long kPV = (long long)kPN1 * kPN2 * kPN3;
+
+template <bool READOUT>
+__global__ __launch_bounds__(256) void copy_pencils(const float2* __restrict__ in, float2* __restrict__ out) {
+  constexpr int R = READOUT ? kPN3 / 16 : kPN2 / 16;
+  const int n = (int)gridDim.x, b = (int)blockIdx.x;
+  const int tile = (b & 7) * (n >> 3) + (b >> 3);
+  const int line = tile / (kPN1 / 16), i10 = (tile % (kPN1 / 16)) * 16;
+  const int p = threadIdx.x & 15, t = threadIdx.x >> 4;
+  const long long js = READOUT ? (long long)kPN1 * kPN2 : kPN1;
+  const long long base = (long long)blockIdx.y * kPV + i10 + p + (READOUT ? (long long)kPN1 * line : (long long)kPN1 * kPN2 * line);
+  float2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = in[base + (long long)(t + 16 * r) * js];
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[base + (long long)(t + 16 * r) * js] = v[r];
+}
+
 }  // namespace
 
 }  // namespace dis
@@ -103,8 +131,11 @@ using namespace dis;
 extern "C" int dis_stream_probe(dis_ctx* c, const void* in, void* out, int64_t nbytes, int32_t variant,
                                 int32_t blocks, int32_t reps, double* ms) {
   if (!c) return DIS_E_INVALID;
-  if (!in || !out || nbytes <= 0 || (nbytes & 32767) || variant < 0 || variant > 5 || reps <= 0 || !ms)
-    return ctx_fail(c, DIS_E_INVALID, "dis_stream_probe: need nbytes a positive multiple of 32 KiB, variant 0..5");
+  if (!in || !out || nbytes <= 0 || (nbytes & 32767) || variant < 0 || variant > 7 || reps <= 0 || !ms)
+    return ctx_fail(c, DIS_E_INVALID, "dis_stream_probe: need nbytes a positive multiple of 32 KiB, variant 0..7");
+  const long long nvol = nbytes / (long long)(kPV * sizeof(float2));
+  if (variant >= 6 && (nvol < 1 || nvol > 65535))
+    return ctx_fail(c, DIS_E_INVALID, "dis_stream_probe: variants 6 and 7 need at least one 256 x 192 x 256 volume");
   if (ctx_set_device(c) != hipSuccess) return ctx_fail(c, DIS_E_HIP, "hipSetDevice failed");
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -120,7 +151,9 @@ extern "C" int dis_stream_probe(dis_ctx* c, const void* in, void* out, int64_t n
       case 2: hipLaunchKernelGGL(copy_f4x4<false>, dim3(nb), dim3(256), 0, s, (const vf4*)in, (vf4*)out, nbytes / 16); break;
       case 3: hipLaunchKernelGGL(copy_f4x4<true>, dim3(nb), dim3(256), 0, s, (const vf4*)in, (vf4*)out, nbytes / 16); break;
       case 4: hipLaunchKernelGGL(copy_tile, dim3((unsigned)std::max<long long>(nb, ntiles)), dim3(256), 0, s, (const float2*)in, (float2*)out, ntiles); break;
-      default: hipLaunchKernelGGL(copy_tile_pipe, dim3(nb), dim3(256), 0, s, (const float2*)in, (float2*)out, ntiles); break;
+      case 5: hipLaunchKernelGGL(copy_tile_pipe, dim3(nb), dim3(256), 0, s, (const float2*)in, (float2*)out, ntiles); break;
+      case 6: hipLaunchKernelGGL(copy_pencils<true>, dim3((kPN1 / 16) * kPN2, (unsigned)nvol), dim3(256), 0, s, (const float2*)in, (float2*)out); break;
+      default: hipLaunchKernelGGL(copy_pencils<false>, dim3((kPN1 / 16) * kPN3, (unsigned)nvol), dim3(256), 0, s, (const float2*)in, (float2*)out); break;
     }
   };
   hipEvent_t e0, e1;

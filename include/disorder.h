@@ -377,8 +377,14 @@ int dis_cg_cs(dis_plan* plan, const double* T, const void* y, const float* Se, c
  * 16-byte loads in flight per lane, 3 = 2 with nontemporal accesses, 4 the
  * pencil passes' tile shape (32 KiB per workgroup, 16 8-byte loads per
  * lane, then the stores), 5 = 4 with each workgroup looping over tiles and
- * the next tile's loads issued before the current tile's stores.  blocks:
- * grid size (0 = 8 per CU; variant 4: one per tile). */
+ * the next tile's loads issued before the current tile's stores, 6 and 7
+ * the strided register passes' access patterns on floor(nbytes / V) whole
+ * 256 x 192 x 256 complex64 volumes (V = 100,663,296 bytes; nbytes >= V):
+ * per workgroup the pencils of 16 consecutive dim-1 positions, lane
+ * (p, t) element t + 16 r, along the readout axis at one dim-2 position (6,
+ * elements n1 n2 apart) or along dim 2 within one readout plane (7, n1
+ * apart), XCD-contiguous tile ranges.  blocks: grid size (0 = 8 per CU;
+ * variants 4, 6 and 7: one per tile). */
 int dis_stream_probe(dis_ctx* ctx, const void* in, void* out, int64_t nbytes, int32_t variant, int32_t blocks,
                      int32_t reps, double* ms);
 
