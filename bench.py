@@ -750,7 +750,8 @@ def main():
                                f"of {off[1] - off[0]} lines", "grid": list(n), "coils": C, "shots": S},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": "profiles/ehe_traffic.json (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
+                     "traffic_source": "profiles/ehe_traffic.json (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1: L2 fills "
+                                       "and write-backs, served by the MALL or HBM)",
                      "algorithmic_bytes_per_launch": kb / nl, "avg_launch_ms": kms / nl},
         "pass_model_v1": {"bytes_per_app": passmodel, "GBps": passmodel / (ms * 1e6),
                           "frac_of_peak": passmodel / (ms * 1e6) / HBM_PEAK_GBS},

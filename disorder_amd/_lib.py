@@ -74,6 +74,7 @@ SIGNATURES = {
                           ctypes.c_double, c_void_p, c_int32, c_int32, c_int32, ctypes.c_double, c_int32_p]),
     "dis_autotune_clear": (c_int, [c_void_p]),
     "dis_set_coil_schedule": (c_int, [c_void_p, c_int32]),
+    "dis_set_cache_budget": (c_int, [c_void_p, ctypes.c_int64]),
     "dis_set_pass_shape": (c_int, [c_void_p, c_int32_p, c_int32, c_int32, c_int32, c_int32]),
     "dis_cg_batched": (c_int, [c_void_p, c_int32, ctypes.POINTER(c_void_p), ctypes.POINTER(c_double_p),
                                ctypes.POINTER(c_void_p), ctypes.POINTER(c_float_p), ctypes.POINTER(c_void_p), c_int32,

@@ -34,6 +34,7 @@ struct dis_ctx {
   // buffers without copies or synchronisation (for graph capture)
   std::map<std::array<int, 4>, PassTune> tune;  // tuned pass shapes per (n1, n2, n3, axis)
   int coil_schedule = 0;  // 0 auto, 1 static grid, 2 persistent work queue
+  long long cache_budget = 208ll << 20;  // bytes of state volumes a transform keeps in flight (0 = all states)
   int tape_mode = 0;  // 0 off, 1 record, 2 replay
   std::vector<std::pair<void*, size_t>> tape;
   size_t tape_pos = 0;
@@ -213,6 +214,7 @@ bool ctx_profiling(dis_ctx* c, int on) {
 int ctx_device(dis_ctx* c) { return c->device; }
 
 int ctx_coil_schedule(dis_ctx* c) { return c->coil_schedule; }
+long long ctx_cache_budget(dis_ctx* c) { return c->cache_budget; }
 
 int prof_begin(dis_ctx* c, int kclass) {
   if (!c->prof) return -1;
@@ -707,6 +709,13 @@ extern "C" int dis_set_pass_shape(dis_ctx* c, const int32_t n[3], int32_t axis, 
   t.maxradix = maxradix;
   const int nn[3] = {n[0], n[1], n[2]};
   ctx_tune_set(c, nn, axis, t);
+  return DIS_OK;
+}
+
+extern "C" int dis_set_cache_budget(dis_ctx* c, int64_t bytes) {
+  if (!c) return DIS_E_INVALID;
+  if (bytes < 0) return fail(c, DIS_E_INVALID, "dis_set_cache_budget: bytes must be >= 0");
+  c->cache_budget = bytes;
   return DIS_OK;
 }
 

@@ -170,48 +170,83 @@ PassArgs pass_args(const Grid& gr, const Step& stp, const unsigned char* dbase, 
   return a;
 }
 
+// States per batch of a transform: as many state volumes as the cache budget
+// holds (at least one).  With several batches, every pass of a batch runs
+// before the next batch starts, so the batch's intermediate volumes are read
+// back from the last-level cache (MALL, 256 MB on MI355X) instead of HBM
+// (E^H E at 256x192x256, 64 states: two states per batch, the 18 in-place
+// passes 43.5 -> 38.9 ms).
+int state_batch(dis_ctx* c, long long nvox, int S) {
+  const long long budget = ctx_cache_budget(c);
+  if (budget <= 0) return S;
+  const long long per = std::max(1ll, budget / (8ll * nvox));
+  return (int)std::min<long long>(S, per);
+}
+
 int run_steps(dis_ctx* c, const Grid& gr, const std::vector<Step>& steps, const unsigned char* dbase, int S,
               const float2* in, long long in_ss, float2* W, bool bcast_first, float2* final_out, int final_mode,
               bool accumulate) {
   hipStream_t stream = ctx_stream(c);
   const long long nv = gr.nvox;
-  for (size_t i = 0; i < steps.size(); ++i) {
+  const double V = 8.0 * (double)nv;  // bytes of one complex64 volume
+  // flips address states through lists: a sequence with flips runs unbatched
+  bool flips = false;
+  for (const Step& stp : steps) flips = flips || stp.flip_m >= 0;
+  const int B = flips ? S : state_batch(c, nv, S);
+  const size_t ns = steps.size();
+  // the broadcast first pass and the state-summing last pass stay single
+  // launches over every state: batching them would re-read the input or
+  // re-accumulate the output per batch, and the flat summing pass keeps the
+  // order in which the states' contributions are added
+  const bool flat_first = B < S && bcast_first;
+  const bool flat_last = B < S && final_mode == MODE_ACCUM;
+  // one launch of step i for states [b0, b0 + nb)
+  auto step = [&](size_t i, int b0, int nb) -> int {
     const Step& stp = steps[i];
     PassArgs a = pass_args(gr, stp, dbase, S);
-    const bool first = i == 0, last = i + 1 == steps.size();
-    a.in = first ? in : W;
+    if (a.coefA) a.coefA += b0;
+    if (a.coefB) a.coefB += b0;
+    a.S = nb;
+    const bool first = i == 0, last = i + 1 == ns;
+    a.in = first ? in + b0 * in_ss : W + b0 * nv;
     a.in_sstride = first ? in_ss : nv;
-    a.out = W;
+    a.out = W + b0 * nv;
     a.out_sstride = nv;
-    const double V = 8.0 * (double)nv;  // bytes of one complex64 volume
     if (stp.flip_m >= 0) {
       a.in = W;
       a.in_sstride = nv;
+      a.out = W;
       const int ps = prof_begin(c, DIS_K_FLIP);
       DCHK(c, launch_pass(a, MODE_FLIP, stp.nlist, stream));
       prof_end(c, ps, 2.0 * V * stp.nlist);
-      continue;
-    }
-    if (first && bcast_first) {
+    } else if (first && bcast_first) {
       const int ps = prof_begin(c, DIS_K_BCAST);
       DCHK(c, launch_pass(a, MODE_BCAST, 1, stream));
-      prof_end(c, ps, V * (1.0 + S));
+      prof_end(c, ps, V * (1.0 + nb));
     } else if (last && final_mode == MODE_ACCUM) {
       a.out = final_out;
       a.out_sstride = 0;
-      a.accumulate = accumulate ? 1 : 0;
+      const bool acc = accumulate || b0 > 0;  // later batches add to the earlier ones' sum
+      a.accumulate = acc ? 1 : 0;
       const int ps = prof_begin(c, DIS_K_ACCUM);
       DCHK(c, launch_pass(a, MODE_ACCUM, 1, stream));
-      prof_end(c, ps, V * (S + (accumulate ? 2.0 : 1.0)));
+      prof_end(c, ps, V * (nb + (acc ? 2.0 : 1.0)));
     } else {
-      if (last && final_out) {
-        a.out = final_out;
-      }
+      if (last && final_out) a.out = final_out + b0 * nv;
       const int ps = prof_begin(c, DIS_K_PASS);
-      DCHK(c, launch_pass(a, MODE_MAP, S, stream));
-      prof_end(c, ps, 2.0 * V * S);
+      DCHK(c, launch_pass(a, MODE_MAP, nb, stream));
+      prof_end(c, ps, 2.0 * V * nb);
     }
+    return DIS_OK;
+  };
+  int rc = DIS_OK;
+  if (flat_first && (rc = step(0, 0, S))) return rc;
+  for (int b0 = 0; b0 < S; b0 += B) {
+    const int nb = std::min(B, S - b0);
+    for (size_t i = flat_first ? 1 : 0; i < ns - (flat_last ? 1 : 0); ++i)
+      if ((rc = step(i, b0, nb))) return rc;
   }
+  if (flat_last && (rc = step(ns - 1, 0, S))) return rc;
   return DIS_OK;
 }
 

@@ -95,6 +95,7 @@ int coil_gram_scratch_init(int device);
 // Coil-stage schedule: 0 auto (work queue for uneven shots), 1 static grid,
 // 2 persistent work queue.
 int ctx_coil_schedule(dis_ctx* c);
+long long ctx_cache_budget(dis_ctx* c);
 
 // Event timing around a launch when profiling is enabled (no-ops otherwise).
 int prof_begin(dis_ctx* c, int kclass);  // returns a slot or -1

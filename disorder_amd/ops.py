@@ -97,6 +97,11 @@ class Context:
     def sync(self):
         check(self.lib.dis_sync(self.h), self.h)
 
+    def set_cache_budget(self, nbytes):
+        """Bytes of state volumes a rigid transform keeps in flight
+        (dis_set_cache_budget; 0 runs every state in every pass launch)."""
+        check(self.lib.dis_set_cache_budget(self.h, ctypes.c_int64(int(nbytes))), self.h)
+
     def profile(self, on=True):
         check(self.lib.dis_profile_enable(self.h, 1 if on else 0), self.h)
 

@@ -198,6 +198,15 @@ int dis_set_pass_shape(dis_ctx* ctx, const int32_t n[3], int32_t axis, int32_t g
  * (shot, readout-plane) work queues.  Results are identical; only the
  * order in which items run changes. */
 int dis_set_coil_schedule(dis_ctx* ctx, int32_t mode);
+/* Bytes of state volumes a rigid transform keeps in flight (default 208
+ * MiB; 0 = every state in every launch).  The passes of a transform (of
+ * encode, decode, E^H E, the CG and LM trial transforms) run state batch by
+ * state batch, each batch as many states as the budget holds, every pass of
+ * a batch before the next batch starts, so a batch's intermediate volumes
+ * stay in the last-level cache (MALL) between passes instead of streaming
+ * through HBM.  Results are unchanged except for the order in which the
+ * adjoint adds the states' contributions (one fixed order per budget). */
+int dis_set_cache_budget(dis_ctx* ctx, int64_t bytes);
 /* Forget every tuned shape (back to the built-in choice). */
 int dis_autotune_clear(dis_ctx* ctx);
 

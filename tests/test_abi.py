@@ -49,6 +49,8 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
     lib = _lib.load()
     assert lib.dis_mlp_param_count() == 256 * 13 + 256 + 256 * 256 + 256 + 6 * 256 + 6 + 6 * 6 * 5 + 6
     assert lib.dis_set_coil_schedule(None, 0) == _lib.DIS_E_INVALID
+    assert lib.dis_set_cache_budget(None, 0) == _lib.DIS_E_INVALID
+    assert lib.dis_plan_set_lm_filter(None, None, 0) == _lib.DIS_E_INVALID
     assert lib.dis_autotune(None, None, None, 1, None, None) == _lib.DIS_E_INVALID
     assert lib.dis_autotune_clear(None) == _lib.DIS_E_INVALID
     assert lib.dis_set_pass_shape(None, None, 0, 0, 0, 16) == _lib.DIS_E_INVALID

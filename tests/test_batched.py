@@ -11,7 +11,7 @@ import pytest
 
 from disorder_amd import sampling as SA
 from oracle import disorder_oracle as O
-from tests.helpers import rel_l2
+from tests.helpers import crandn, random_motion, rel_l2, smooth_volume
 
 # phase-encode sizes on the register coil path (deterministic, no atomics):
 # results are bit-identical however the iterations are launched
@@ -127,3 +127,40 @@ def test_persistent_work_queue_matches_static_grid(dev, n, tiles, C):
     D._lib.check(ctx.lib.dis_set_coil_schedule(ctx.h, 0), ctx.h)
     for a, b in zip(out[1], out[2]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("budget_vols", [1, 2, 3])
+def test_state_batches_match_one_launch_per_pass(dev, budget_vols):
+    """Transforms run state batch by state batch (the cache budget holding
+    1, 2 or 3 volumes of a 48x40x24 grid, 7 states, so batches of uneven
+    size) give encode, decode, E^H E, energies and the rigid transform
+    bit-identical to every state in every pass launch (budget 0); flips
+    (rotations beyond 90 degrees) run unbatched."""
+    import disorder_amd as D
+    from tests.test_gpu_parity import to_dev
+    n, C, S = (48, 40, 24), 3, 7
+    rng = np.random.default_rng(17 + budget_vols)
+    x = smooth_volume(rng, n) + 0.1 * crandn(rng, n)
+    Sm = crandn(rng, n + (C,)) * 0.5
+    H = rng.integers(1, S + 1, size=(n[0], n[1]))
+    k1, k2, off, _ = O.build_harmonic_sampling(H, S)
+    T = random_motion(rng, S)
+    T[2, 3] = 2.2  # a rotation beyond 90 degrees: that state's transform flips
+    ctx = D.Context(0)
+    E = D.Encoding(to_dev(Sm, dev), k1, k2, off[:S + 1], ctx=ctx)
+    xd = to_dev(x, dev)
+    y = E.encode(xd, T)
+    out = {}
+    for budget in (0, budget_vols * 8 * n[0] * n[1] * n[2]):
+        ctx.set_cache_budget(budget)
+        for Tc in (T, np.where(np.arange(S)[:, None] == 2, 0.0, T)):  # with and without the flip
+            key = (budget, Tc is T)
+            out[key] = (E.encode(xd, Tc).cpu().numpy(), E.decode(y, Tc).cpu().numpy(),
+                        E.encodedecode(xd, Tc).cpu().numpy(), E.energy(xd, y, Tc, per_sample=True),
+                        D.sincRigidTransform(xd, Tc, di=1, ctx=ctx).cpu().numpy())
+    ctx.set_cache_budget(208 << 20)
+    for flip in (True, False):
+        for a, b in zip(out[(0, flip)], out[(budget_vols * 8 * n[0] * n[1] * n[2], flip)]):
+            assert np.array_equal(a, b), flip
+    E.close()
