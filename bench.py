@@ -238,7 +238,7 @@ def bench_pyramid(args):
     up to twice the plain method's iterations and is null when its image
     converges elsewhere.  The learned run's solve-time ratio is also taken
     at its own convergence, with the NRMSE between the two final images
-    (the north star's criterion: the plain NRMSE within 1e-4).
+    (the criterion of equal quality: the plain NRMSE within 1e-4).
     NRMSE against the ground truth is given frame-aligned, with the motion
     error once the common motion is removed.  Snapshots are taken outside
     the timed solve."""
@@ -638,8 +638,8 @@ def bench_proposal(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (multi-GPU sharding of shots "
-              "is a design note in DESIGN.md, not part of this build)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (sharding the shots over GPUs "
+              "is designed, not built)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "pyramid":
         print(json.dumps(bench_pyramid(args)), flush=True)

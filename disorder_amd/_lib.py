@@ -1,7 +1,7 @@
 """ctypes binding of libdisorder_hip.so (the C-ABI declared in include/disorder.h).
 
-The library is built in-tree by ``__graft_entry__.build()`` (make in
-``disorder_amd/csrc``).  Loading fails loudly when it is missing: there is no
+The library is built in-tree (``make -C disorder_amd/csrc``, hipcc for
+gfx950).  Loading fails loudly when it is missing: there is no
 CPU fallback anywhere in the product path.
 """
 from __future__ import annotations
@@ -124,7 +124,7 @@ def load():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+        raise ImportError(f"{LIB_PATH} is missing: build it with make -C disorder_amd/csrc "
                           "(there is no CPU fallback)")
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():

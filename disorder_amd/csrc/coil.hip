@@ -11,9 +11,11 @@
 // tile element (row i2, spectrum column k1) then gathers the samples of its
 // k1 row from the state's CSR in a fixed order (no atomics, so sums are
 // reproducible run to run), the tile is inverse-FFT'd along dim 1,
-// multiplied by conj(S_c) and the coils are summed in registers.  For E^H E
-// both halves run in the same workgroup, so the state's k-space never
-// leaves the chip.
+// multiplied by conj(S_c) and the coils are summed in registers.  E^H E
+// runs the forward half into a k-space scratch [Ns x n3 x C] and the adjoint
+// half from it, as two launches; the one-launch form (MODE = COIL_EHE, both
+// halves in one workgroup, the k-space kept on chip) is kept in the source
+// but measured slower (its registers are live across both halves).
 #include <algorithm>
 
 #include "kernels.hpp"

@@ -1,6 +1,6 @@
 // Streaming-copy probes: the HBM read+write rate this GPU sustains for a
-// plain device copy under several access shapes, the measured ceiling beside
-// which the HBM-bound pencil passes are judged.  No counterpart in the
+// plain device copy under several access shapes, the measured ceiling the
+// HBM-bound pencil passes are compared with.  No counterpart in the
 // reference (it is a measurement of the hardware, not of the method).
 //
 //   variant 0: grid-stride, one float2 (8 B) per lane per iteration

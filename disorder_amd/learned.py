@@ -4,8 +4,8 @@ A small network proposes per-shot motion updates from the shots'
 sincRigidTransformGradient-based features at the current estimate; the
 proposal of a shot is kept only where computeEnergy of that shot decreases,
 and LMsolver (Reconstruction/LMsolver.m:1) runs as before on every shot whose
-proposal was rejected, so the plain method stays the fallback and the judge
-of the error.
+proposal was rejected, so the plain method stays the fallback and decides
+what is kept.
 
 Features per shot (13): from the LM partial sums at the current (x, T)
 (Reconstruction/LMsolver.m:82-115, via dis_lm_partials):

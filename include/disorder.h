@@ -379,8 +379,8 @@ int dis_cg_cs(dis_plan* plan, const double* T, const void* y, const float* Se, c
               int32_t* nn);
 
 /* Streaming-copy probe (no counterpart in the reference: a measurement of
- * the GPU's sustainable HBM rate beside which the HBM-bound passes are
- * judged).  Copies nbytes (a multiple of 32 KiB) from DEVICE in to DEVICE
+ * the GPU's sustainable HBM rate that the HBM-bound passes are compared
+ * with).  Copies nbytes (a multiple of 32 KiB) from DEVICE in to DEVICE
  * out reps times after one warm-up copy; ms receives the average time of one
  * copy.  variant: 0 grid-stride 8-byte lanes, 1 16-byte lanes, 2 four
  * 16-byte loads in flight per lane, 3 = 2 with nontemporal accesses, 4 the
