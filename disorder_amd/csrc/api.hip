@@ -44,9 +44,12 @@ struct dis_ctx {
     int kclass;
     hipEvent_t a, b;
     double bytes;
+    int launches;
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
+  int group_depth = 0;  // nesting of ProfGroup scopes
+  int open = -1;        // record of a group whose end event is not yet recorded
 };
 
 struct dis_plan {
@@ -216,8 +219,29 @@ int ctx_device(dis_ctx* c) { return c->device; }
 int ctx_coil_schedule(dis_ctx* c) { return c->coil_schedule; }
 long long ctx_cache_budget(dis_ctx* c) { return c->cache_budget; }
 
+namespace {
+void prof_close(dis_ctx* c) {
+  if (c->open < 0) return;
+  (void)hipEventRecord(c->recs[c->open].b, c->stream);
+  c->open = -1;
+}
+}  // namespace
+
+ProfGroup::ProfGroup(dis_ctx* ctx) : c(ctx) { ++c->group_depth; }
+
+ProfGroup::~ProfGroup() {
+  if (--c->group_depth == 0) prof_close(c);
+}
+
 int prof_begin(dis_ctx* c, int kclass) {
   if (!c->prof) return -1;
+  if (c->open >= 0) {
+    if (c->recs[c->open].kclass == kclass) {
+      ++c->recs[c->open].launches;
+      return c->open;
+    }
+    prof_close(c);
+  }
   auto get = [&]() {
     hipEvent_t e = nullptr;
     if (!c->pool.empty()) {
@@ -233,6 +257,7 @@ int prof_begin(dis_ctx* c, int kclass) {
   r.a = get();
   r.b = get();
   r.bytes = 0;
+  r.launches = 1;
   if (!r.a || !r.b) return -1;
   (void)hipEventRecord(r.a, c->stream);
   c->recs.push_back(r);
@@ -241,8 +266,11 @@ int prof_begin(dis_ctx* c, int kclass) {
 
 void prof_end(dis_ctx* c, int slot, double bytes) {
   if (slot < 0 || !c->prof) return;
-  c->recs[slot].bytes = bytes;
-  (void)hipEventRecord(c->recs[slot].b, c->stream);
+  c->recs[slot].bytes += bytes;
+  if (c->group_depth > 0)
+    c->open = slot;  // the end event waits for the group's last launch
+  else
+    (void)hipEventRecord(c->recs[slot].b, c->stream);
 }
 
 }  // namespace dis
@@ -330,6 +358,7 @@ int dis_profile_enable(dis_ctx* c, int32_t on) {
     c->pool.push_back(r.b);
   }
   c->recs.clear();
+  c->open = -1;
   c->prof = on != 0;
   return DIS_OK;
 }
@@ -346,7 +375,7 @@ int dis_profile_read(dis_ctx* c, int32_t kclass, int64_t* launches, double* ms_t
     HIPCHK(c, hipEventElapsedTime(&t, r.a, r.b));
     ms += t;
     by += r.bytes;
-    ++n;
+    n += r.launches;
   }
   if (launches) *launches = n;
   if (ms_total) *ms_total = ms;

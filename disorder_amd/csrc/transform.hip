@@ -240,6 +240,7 @@ int run_steps(dis_ctx* c, const Grid& gr, const std::vector<Step>& steps, const 
     return DIS_OK;
   };
   int rc = DIS_OK;
+  ProfGroup group(c);
   if (flat_first && (rc = step(0, 0, S))) return rc;
   for (int b0 = 0; b0 < S; b0 += B) {
     const int nb = std::min(B, S - b0);

@@ -100,6 +100,19 @@ long long ctx_cache_budget(dis_ctx* c);
 // Event timing around a launch when profiling is enabled (no-ops otherwise).
 int prof_begin(dis_ctx* c, int kclass);  // returns a slot or -1
 void prof_end(dis_ctx* c, int slot, double bytes);
+// Inside a group, launches of one class issued back to back with nothing
+// else on the stream between them share one event pair (the first launch's
+// start, the last one's end) and count as that many launches; a class change
+// or the end of the outermost group records the end event.  Two events per
+// run of same-class launches instead of two per launch: a transform of 64
+// states in two-state batches records a handful of events, not a thousand.
+struct ProfGroup {
+  explicit ProfGroup(dis_ctx* c);
+  ~ProfGroup();
+  ProfGroup(const ProfGroup&) = delete;
+  ProfGroup& operator=(const ProfGroup&) = delete;
+  dis_ctx* c;
+};
 
 int make_plan(int n, FftPlan* pl, int maxradix = 16);
 // Tuned pass shapes per (n1, n2, n3, axis), kept by the context.

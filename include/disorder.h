@@ -408,11 +408,14 @@ int dis_stream_probe(dis_ctx* ctx, const void* in, void* out, int64_t nbytes, in
 #define DIS_K_SHEAR 7  /* shearlet FFT-axis pass (CS reconstruction) */
 #define DIS_K_COUNT 8
 
-/* Per-kernel-class timing with HIP events recorded around every launch on the
- * context's stream (no counterpart in the reference, which only prints
- * tic/toc totals, Reconstruction/solveXT.m:184,263).  Enabling resets the
- * counters.  bytes_total is the algorithmic HBM traffic of the launches
- * (each pass reads and writes every state volume once). */
+/* Per-kernel-class timing with HIP events on the context's stream (no
+ * counterpart in the reference, which only prints tic/toc totals,
+ * Reconstruction/solveXT.m:184,263).  Events bracket every launch, except
+ * that the passes of one rigid transform issued back to back share one event
+ * pair per run of same-class launches (ms_total then also holds the few
+ * microseconds between those launches).  Enabling resets the counters.
+ * bytes_total is the algorithmic HBM traffic of the launches (each pass
+ * reads and writes every state volume once). */
 int dis_profile_enable(dis_ctx* ctx, int32_t on);
 int dis_profile_read(dis_ctx* ctx, int32_t kernel_class, int64_t* launches, double* ms_total, double* bytes_total);
 

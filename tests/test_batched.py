@@ -164,3 +164,42 @@ def test_state_batches_match_one_launch_per_pass(dev, budget_vols):
         for a, b in zip(out[(0, flip)], out[(budget_vols * 8 * n[0] * n[1] * n[2], flip)]):
             assert np.array_equal(a, b), flip
     E.close()
+
+
+@pytest.mark.gpu
+def test_grouped_profile_counts_every_launch(dev):
+    """Profiling E^H E with one state per batch (cache budget of one volume)
+    counts S times the pass launches of the unbatched run (budget 0), the
+    same algorithmic bytes and the same single broadcast and accumulate
+    launch per transform, although the batched passes of each transform share
+    one event pair; every class's time is positive."""
+    import disorder_amd as D
+    from tests.test_gpu_parity import to_dev
+    n, C, S = (40, 32, 16), 2, 5
+    rng = np.random.default_rng(23)
+    x = smooth_volume(rng, n)
+    Sm = crandn(rng, n + (C,)) * 0.5
+    H = rng.integers(1, S + 1, size=(n[0], n[1]))
+    k1, k2, off, _ = O.build_harmonic_sampling(H, S)
+    T = random_motion(rng, S)
+    ctx = D.Context(0)
+    E = D.Encoding(to_dev(Sm, dev), k1, k2, off[:S + 1], ctx=ctx)
+    xd = to_dev(x, dev)
+    prof = {}
+    for budget in (0, 8 * n[0] * n[1] * n[2]):
+        ctx.set_cache_budget(budget)
+        E.encodedecode(xd, T)  # warm-up outside the profile
+        ctx.profile(True)
+        E.encodedecode(xd, T)
+        prof[budget] = ctx.profile_read()
+        ctx.profile(False)
+    ctx.set_cache_budget(208 << 20)
+    flat, one = prof[0], prof[8 * n[0] * n[1] * n[2]]
+    assert flat["pass"][0] > 0 and one["pass"][0] == S * flat["pass"][0]
+    assert one["pass"][2] == pytest.approx(flat["pass"][2], rel=1e-12)
+    for k in ("bcast_pass", "accum_pass", "coil_stage"):
+        assert one[k][0] == flat[k][0] >= 1, k
+        assert one[k][2] == pytest.approx(flat[k][2], rel=1e-12), k
+    for p in (flat, one):
+        assert all(v[1] > 0 for v in p.values() if v[0]), p
+    E.close()
