@@ -304,7 +304,12 @@ hipError_t launch_coil_generic(const CoilArgs& a, int mode, int nstates, hipStre
 // Register path for n1 = 16 * R1.  Sixteen dim-2 rows per tile, sixteen
 // lanes per row (regfft.hpp).  Workgroups are numbered so that the S states
 // of one readout plane run back to back on one XCD: they share that plane's
-// coil maps through the XCD's L2.
+// coil maps through the XCD's L2.  The adjoint adds each sample into the 16
+// rows of a tile with a geometric recurrence over the rows; the lanes own
+// either one phase-encode row each (samples of a row summed in CSR order)
+// or, in adjoint-only launches of plans with a lane table (lanetab.hpp),
+// equal runs of row-sorted samples, so that no wave waits on its most-sampled
+// row.
 constexpr int kRJ = 3;  // samples per thread per chunk on the register path
 static_assert(kRJ * kThreads == kCoilRegChunk, "slot tables are built per chunk of kCoilRegChunk samples");
 constexpr int kHornerBatch = 16;  // rows of a full group whose LDS reads may be in flight together
@@ -321,8 +326,12 @@ __device__ __forceinline__ int queue_next(int* queue, int home, int per_queue) {
   return -1;
 }
 
-template <int R1, int MODE, bool PERSIST = false>
+// BAL: adjoint-only instance driven by the plan's lane table (a.adjtab set,
+// no fold along dim 1); kept a separate instance so that each kernel holds
+// one adjoint loop.
+template <int R1, int MODE, bool PERSIST = false, bool BAL = false>
 __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nstates) {
+  static_assert(!BAL || MODE == COIL_DECODE, "the lane-table adjoint is an adjoint-only launch");
   constexpr bool FWD = MODE == COIL_ENCODE || MODE == COIL_EHE || MODE == COIL_ENERGY;
   constexpr bool ADJ = MODE == COIL_DECODE || MODE == COIL_EHE;
   constexpr int N = R1 * 16;
@@ -330,8 +339,16 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   constexpr int XZ = (16 * XB > 16 * N) ? 16 * XB : 16 * N;
   constexpr int kEntCap = 2048;
   __shared__ fvec2 xzraw[XZ];                 // FFT exchange, aliased with the 16-row tile
-  __shared__ fvec2 wcraw[kRJ * kThreads];     // adjoint: sample values of the current coil
-  __shared__ int csrraw[N + 1 + kEntCap];     // adjoint: the state's CSR by k1
+  // adjoint scratch: the row-owner loop's sample values of the current coil
+  // and the state's CSR by k1, or (adjoint-only launches) the lane table's
+  // carries [16][kThreads]
+  constexpr int kWcElems = kRJ * kThreads;
+  constexpr int kCsrInts = N + 1 + kEntCap;
+  constexpr int kRowOwnerElems = kWcElems + (kCsrInts + 1) / 2;
+  constexpr int kCarryElems = BAL ? 16 * kThreads : 0;
+  __shared__ fvec2 adjraw[kRowOwnerElems > kCarryElems ? kRowOwnerElems : kCarryElems];
+  fvec2* const wcraw = adjraw;
+  int* const csrraw = (int*)(adjraw + kWcElems);
   __shared__ fvec2 t1raw[N];
   __shared__ fvec2 t2raw[1024];
   const int tid = threadIdx.x;
@@ -387,22 +404,44 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   const double cf = a.tfwd ? a.tfwd[s] * a.invN1 : 0.0;
   const double ca = a.tadj ? a.tadj[s] * a.invN1 : 0.0;
 
+  // adjoint-only launches with a lane table: every lane takes cs <= 3
+  // row-sorted samples (one chunk per state)
+  constexpr bool bal = BAL;
+  int binfo = 0;
   bool first_out = true;
   for (int sc0 = o0; sc0 < o1 || (sc0 == o0 && ADJ); sc0 += kRJ * kThreads) {
     const int sc1 = min(o1, sc0 + kRJ * kThreads);
     int jk1[kRJ], jk2[kRJ], jstep[kRJ], jg[kRJ];
     float jw[kRJ];
+    int bent[kRJ];
+    if (bal) {
+      const int4 tb = ((const int4*)a.adjtab)[(long long)s * kThreads + tid];
+      bent[0] = tb.x;
+      bent[1] = tb.y;
+      bent[2] = tb.z;
+      binfo = tb.w;
+    }
 #pragma unroll
     for (int i = 0; i < kRJ; ++i) {
-      // slot -> sample: the plan deals each chunk's samples to slots so that
-      // consecutive lanes read distinct LDS banks in the row gathers
-      const int q = sc0 + tid + i * kThreads;
-      const bool ok = q < sc1;
-      const int j = ok ? (a.sord ? a.sord[q] : q) : -1;
-      jg[i] = j;
-      jk1[i] = ok ? a.k1[j] : -1;
-      jk2[i] = ok ? a.k2[j] : 0;
-      jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
+      if (bal) {
+        // lane-table entry: row (k1), k2 and the sample within its state
+        const bool ok = bent[i] >= 0;
+        const int j = ok ? o0 + ((bent[i] >> 20) & 1023) : -1;
+        jg[i] = j;
+        jk1[i] = ok ? (bent[i] >> 10) & 1023 : -1;
+        jk2[i] = ok ? bent[i] & 1023 : 0;
+        jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
+      } else {
+        // slot -> sample: the plan deals each chunk's samples to slots so
+        // that consecutive lanes read distinct LDS banks in the row gathers
+        const int q = sc0 + tid + i * kThreads;
+        const bool ok = q < sc1;
+        const int j = ok ? (a.sord ? a.sord[q] : q) : -1;
+        jg[i] = j;
+        jk1[i] = ok ? a.k1[j] : -1;
+        jk2[i] = ok ? a.k2[j] : 0;
+        jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
+      }
       jstep[i] = (16 * jk2[i]) % m2;  // twiddle-index advance per 16-row group
     }
     float esum[kRJ];
@@ -543,7 +582,114 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         }
       }
 
-      if (ADJ) {
+      if (ADJ && bal) {
+        // Balanced adjoint from the lane table: lane L adds its (up to 3)
+        // row-sorted samples into all 16 dim-2 rows of the group, writing
+        // each row segment that closes in the lane straight into the tile
+        // (a row begun in an earlier lane goes to the lane's carry slot
+        // instead); the owner of a row that continues into later lanes adds
+        // their carries in lane order once they are published.  Every lane
+        // runs the same number of samples, and the sum is deterministic.
+        lds_float2* cy = (lds_float2*)adjraw;  // [16][kThreads], stored swapped like the tile
+        const bool deferred = (binfo & kAdjDeferred) != 0;
+        const int ncont = (binfo >> kAdjNcontShift) & 1023;
+        const int zrow = (binfo >> kAdjZrowShift) & 1023;
+        const float fn2 = 1.0f / (float)m2;
+        for (int g = 0; g < ngroups; ++g) {
+          const int r0 = g * 16;
+          const int Gv = min(16, n2 - r0);
+          const bool row_ok = p < Gv;
+          const int i2 = r0 + (row_ok ? p : 0);
+          const long long rowoff = (long long)i2 * N;
+          float2 o[16];
+#pragma unroll
+          for (int k = 0; k < 16; ++k) o[k] = make_float2(0.f, 0.f);
+          for (int c = 0; c < cc; ++c) {
+            __syncthreads();  // the previous FFT's exchange (aliasing the tile) and carry reads are done
+            float2 acc[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
+            int lrow = 0;
+#pragma unroll
+            for (int i = 0; i < kRJ; ++i) {
+              if (jk1[i] >= 0) {
+                float2 w = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int cq = 0; cq < kCC; ++cq)
+                  if (cq == c) w = W[i][cq];
+                const int k2 = jk2[i];
+                const int prod = k2 * (r0 + o2p);  // = k2 (r0 - o2) modulo m2
+                int idx = prod - m2 * (int)((float)prod * fn2);
+                idx += idx < 0 ? m2 : 0;
+                idx -= idx >= m2 ? m2 : 0;
+                // conj(W2^{k2 (r0+q)}) w = (w conj(W2^{k2 r0})) conj(W2^{k2})^q
+                float2 wq = cmulc(w, ld2(tw2 + idx));
+                const float2 pst = ld2(tw2 + k2);
+                // a segment closed at the previous entry restarts the sum
+                const float keep = i > 0 && ((binfo >> (kAdjCloseShift + i - 1)) & 1) ? 0.f : 1.f;
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                  acc[q] = make_float2(fmaf(acc[q].x, keep, wq.x), fmaf(acc[q].y, keep, wq.y));
+                  wq = cmulc(wq, pst);
+                }
+                lrow = jk1[i];
+                if ((binfo >> (kAdjCloseShift + i)) & 1) {
+                  const bool to_carry = (binfo >> (kAdjCarryShift + i)) & 1;
+                  lds_float2* dst = to_carry ? cy + tid : Zt + lrow;
+                  // one stride (immediate store offsets) when the tile is kThreads wide
+                  const int dstride = N == kThreads ? N : (to_carry ? kThreads : N);
+#pragma unroll
+                  for (int q = 0; q < 16; ++q) st2(dst + q * dstride, cswap(acc[q]));
+                }
+              }
+            }
+            if (binfo & kAdjZvalid) {  // an empty row dealt to this lane
+#pragma unroll
+              for (int q = 0; q < 16; ++q) st2(Zt + q * N + zrow, make_float2(0.f, 0.f));
+            }
+            __syncthreads();  // carries published
+            if (deferred) {
+#pragma unroll
+              for (int q = 0; q < 16; ++q) acc[q] = cswap(acc[q]);
+              for (int l = 1; l <= ncont; ++l) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[q] = cadd(acc[q], ld2(cy + q * kThreads + tid + l));
+              }
+#pragma unroll
+              for (int q = 0; q < 16; ++q) st2(Zt + q * N + lrow, acc[q]);
+            }
+            __syncthreads();
+            float2 v[16];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * N + t + 16 * r);
+            __syncthreads();  // the row tile aliases the exchange buffer
+            reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
+            if (t < R1) {
+              const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
+#pragma unroll
+              for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), Sp[t + R1 * k]), inv_sqrt));
+            }
+          }
+          if (a.tadj) {  // first translation of T^H along dim 1 (input layout of reg_fft<16,R1>)
+            reg_fft<16, R1>(o, t, xb, tw1);
+            {
+              float2 ph[16];
+              ramp_in<R1>(ph, ca, t);
+#pragma unroll
+              for (int r = 0; r < R1; ++r) o[r] = cswap(cmul(o[r], ph[r]));
+            }
+            reg_fft<R1, 16>(o, t, xb, tw1);
+#pragma unroll
+            for (int k = 0; k < 16; ++k) o[k] = cscale(cswap(o[k]), inv_n);
+          }
+          if (t < R1 && row_ok) {
+            float2* Vt = Vp + rowoff;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) Vt[t + R1 * k] = first_out ? o[k] : cadd(Vt[t + R1 * k], o[k]);
+          }
+        }
+        first_out = false;
+      } else if (ADJ) {
         // Atomic-free adjoint: the weighted samples of coil c are published
         // in LDS; lane k owns phase-encode row k1 = k and adds each of its
         // samples into all 16 dim-2 rows of the group (exclusive writes),
@@ -681,10 +827,20 @@ __global__ void zero_queue_kernel(int* q) {
   if (threadIdx.x < 8) q[threadIdx.x] = 0;
 }
 
+// the adjoint-only launch takes the lane-table instance when the plan has a
+// table and dim 1 is not folded (q1 = 1)
+static bool lane_table_launch(const CoilArgs& a, int mode) {
+  return mode == COIL_DECODE && a.adjtab != nullptr && (a.fy1 == 0 || a.fy1 == a.n1);
+}
+
 template <int R1>
 hipError_t launch_rcoil(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
   const int nblk = (((a.i3n ? a.i3n : a.n3) + 7) / 8) * 8 * nstates;
   dim3 grid(nblk);
+  if (lane_table_launch(a, mode)) {
+    hipLaunchKernelGGL((rcoil_kernel<R1, COIL_DECODE, false, true>), grid, dim3(kThreads), 0, st, a, nstates);
+    return hipGetLastError();
+  }
   switch (mode) {
     case COIL_ENCODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_ENCODE>), grid, dim3(kThreads), 0, st, a, nstates); break;
     case COIL_DECODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_DECODE>), grid, dim3(kThreads), 0, st, a, nstates); break;
@@ -697,15 +853,17 @@ hipError_t launch_rcoil(const CoilArgs& a, int mode, int nstates, hipStream_t st
 
 template <int R1>
 hipError_t launch_rcoil_persistent(const CoilArgs& a, int mode, int nstates, hipStream_t st) {
-  static int resident[5] = {0, 0, 0, 0, 0};
-  const int mi = mode - 1;
-  if (mi < 0 || mi > 3) return hipErrorInvalidValue;
+  static int resident[5] = {0, 0, 0, 0, 0};  // per mode, [4]: the lane-table adjoint
+  const bool lt = lane_table_launch(a, mode);
+  const int mi = lt ? 4 : mode - 1;
+  if (mi < 0 || mi > 4) return hipErrorInvalidValue;
   if (!resident[mi]) {
     int dev = 0, ncu = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess) {
-      switch (mode) {
+      switch (lt ? 0 : mode) {
+        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_DECODE, true, true>, kThreads, 0); break;
         case COIL_ENCODE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_ENCODE, true>, kThreads, 0); break;
         case COIL_DECODE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_DECODE, true>, kThreads, 0); break;
         case COIL_EHE: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rcoil_kernel<R1, COIL_EHE, true>, kThreads, 0); break;
@@ -719,6 +877,10 @@ hipError_t launch_rcoil_persistent(const CoilArgs& a, int mode, int nstates, hip
   const int nblk = std::max(8, std::min(resident[mi], (items + 7) / 8 * 8));
   hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(64), 0, st, a.queue);
   dim3 grid(nblk);
+  if (lt) {
+    hipLaunchKernelGGL((rcoil_kernel<R1, COIL_DECODE, true, true>), grid, dim3(kThreads), 0, st, a, nstates);
+    return hipGetLastError();
+  }
   switch (mode) {
     case COIL_ENCODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_ENCODE, true>), grid, dim3(kThreads), 0, st, a, nstates); break;
     case COIL_DECODE: hipLaunchKernelGGL((rcoil_kernel<R1, COIL_DECODE, true>), grid, dim3(kThreads), 0, st, a, nstates); break;

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "fft_engine.hpp"
+#include "lanetab.hpp"
 
 namespace dis {
 
@@ -91,6 +92,12 @@ struct CoilArgs {
   const int* csr;         // [Ns] (k2 << 16) | (sample index within its state), sorted by k1 per state
   const int* slist;       // optional [nstates] list of the states handled (U/V indexed by list position)
   const int* sord;        // optional [Ns] register path: sample handled by each slot, per chunk of kCoilRegChunk
+  // optional [S][kThreads] int4 register-path adjoint lane table (states of
+  // at most kCoilRegChunk samples): lane L takes row-sorted samples
+  // cs L .. cs L + cs - 1 (cs = ceil(samples / kThreads)); .x/.y/.z =
+  // k2 | row << 10 | (sample within its state) << 20, or -1; .w = flags
+  // (kAdj* of lanetab.hpp)
+  const int* adjtab;
   int* queue;             // persistent launch: 8 per-XCD work counters (zeroed before the launch), else null
   // SENSE folding: acquired phase-encode grid fy1 x fy2 smaller than n1 x n2
   // (0 = none); k1/k2 then index the acquired grid
@@ -110,6 +117,7 @@ enum CoilMode { COIL_ENCODE = 1, COIL_DECODE = 2, COIL_EHE = 3, COIL_ENERGY = 4 
 
 // Samples per chunk of the register-path coil stage (3 per thread).
 constexpr int kCoilRegChunk = 768;
+static_assert(kAdjLanes == kThreads && kAdjMaxSamples == kCoilRegChunk, "adjoint lane table shape");
 
 hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st);
 // Persistent register-path coil stage: resident workgroups pull (state,

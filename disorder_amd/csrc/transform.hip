@@ -454,6 +454,7 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
     a.rowptr = v.rowptr;
     a.csr = v.csr;
     a.sord = v.sord;
+    a.adjtab = v.adjtab;
     a.Ns = g.nsamples;
     a.yin = yin;
     a.yout = yout;
@@ -468,10 +469,10 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
   };
 
   // E^H E runs the coil stage as its two halves, the forward into a k-space
-  // scratch [Ns x n3 x C] and the adjoint from it: at the benchmark size the
-  // two launches take 14.5 + 20.1 ms against 63.7 ms for the fused on-chip
-  // kernel, for 2 x 805 MB of extra traffic (every plan on the register
-  // path, including plans folded along dim 2 only)
+  // scratch [Ns x n3 x C] and the adjoint from it: at 256 x 192 x 256, 8
+  // coils, 64 shots the two launches take 13.9 + 15.7 ms against 63.7 ms for
+  // the fused on-chip kernel, for 2 x 805 MB of extra traffic (every plan on
+  // the register path, including plans folded along dim 2 only)
   const bool split = mode == COIL_EHE && (v.fold[0] == 0 || coil_fold_register(g.n[0], g.n[1], v.fold[0]));
   float2* yscr = nullptr;
   if (split) {
@@ -890,6 +891,7 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
     a.rowptr = v.rowptr;
     a.csr = v.csr;
     a.sord = v.sord;
+    a.adjtab = v.adjtab;
     a.Ns = g.nsamples;
     a.yin = y;
     a.yout = nullptr;

@@ -194,6 +194,43 @@ def test_encode_decode_ehe(dev, n, C, S, nmotion, weights):
     del Tall
 
 
+def test_adjoint_lane_table_edge_states(dev):
+    """The adjoint's lane table (samples sorted by phase-encode row, dealt in
+    equal runs to the lanes, rows spanning lanes finished from carries) on
+    states that stress it: no samples, one sample, one row holding all 48 of
+    its samples (a row spread over 48 lanes), 768 samples (three per lane),
+    ten full rows plus 33 others (heavy rows, partly empty last lanes) and
+    257 samples (two per lane); decode and E^H E against the oracle."""
+    import disorder_amd as D
+    rng = np.random.default_rng(29)
+    n, C = (64, 48, 4), 2
+    grid = np.array([(a, b) for a in range(n[0]) for b in range(n[1])])
+    states = [np.zeros((0, 2), int), grid[rng.choice(len(grid), 1, replace=False)],
+              np.array([(7, b) for b in range(n[1])]),
+              grid[rng.choice(len(grid), 768, replace=False)]]
+    full = np.array([(a, b) for a in range(10) for b in range(n[1])])
+    rest = grid[grid[:, 0] >= 10]
+    states.append(np.vstack([full, rest[rng.choice(len(rest), 33, replace=False)]]))
+    states.append(grid[rng.choice(len(grid), 257, replace=False)])
+    for st in states:
+        rng.shuffle(st)  # acquisition order within a state is arbitrary
+    k1 = np.concatenate([st[:, 0] for st in states]).astype(np.int64)
+    k2 = np.concatenate([st[:, 1] for st in states]).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum([len(st) for st in states])])
+    S = len(states)
+    We = rng.uniform(0.5, 1.5, off[-1])
+    samp = O.Sampling(k1, k2, off, n[0], n[1], We)
+    x = smooth_volume(rng, n) + 0.1 * crandn(rng, n)
+    Sm = crandn(rng, n + (C,)) * 0.5
+    T = random_motion(rng, S)
+    E = D.Encoding(to_dev(Sm, dev), k1, k2, off, We=We)
+    yin = crandn(rng, (off[-1], n[2], C))
+    xr = O.decode(yin, Sm, samp, T, n)
+    assert rel_l2(to_np(E.decode(to_dev(yin, dev), T)), xr) < TOL
+    er = O.encodedecode(x, Sm, samp, T, n)
+    assert rel_l2(to_np(E.encodedecode(to_dev(x, dev), T)), er) < TOL
+
+
 def test_encode_zero_motion_and_level_grid(dev):
     """T == 0 skips the transform (encode.m:61); level grids (Nfull = 2n)."""
     import disorder_amd as D
