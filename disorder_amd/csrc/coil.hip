@@ -336,17 +336,20 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   constexpr bool ADJ = MODE == COIL_DECODE || MODE == COIL_EHE;
   constexpr int N = R1 * 16;
   constexpr int XB = RegFftSize<R1>::XB;
-  constexpr int XZ = (16 * XB > 16 * N) ? 16 * XB : 16 * N;
+  // lane-table adjoint: the tile carries the lanes' carry slots as kThreads
+  // extra columns, so tile rows and carries share one stride, a multiple of
+  // 64 elements (512 bytes) so that a segment's 16 stores pair into
+  // ds_write2st64 (n1 = 224 with stride 480: 82 single stores, 10 % slower)
+  constexpr int NS = BAL ? (N + kThreads + 63) / 64 * 64 : N;
+  constexpr int XZ = (16 * XB > 16 * NS) ? 16 * XB : 16 * NS;
   constexpr int kEntCap = 2048;
   __shared__ fvec2 xzraw[XZ];                 // FFT exchange, aliased with the 16-row tile
-  // adjoint scratch: the row-owner loop's sample values of the current coil
-  // and the state's CSR by k1, or (adjoint-only launches) the lane table's
-  // carries [16][kThreads]
+  // row-owner adjoint: the sample values of the current coil and the
+  // state's CSR by k1
   constexpr int kWcElems = kRJ * kThreads;
   constexpr int kCsrInts = N + 1 + kEntCap;
   constexpr int kRowOwnerElems = kWcElems + (kCsrInts + 1) / 2;
-  constexpr int kCarryElems = BAL ? 16 * kThreads : 0;
-  __shared__ fvec2 adjraw[kRowOwnerElems > kCarryElems ? kRowOwnerElems : kCarryElems];
+  __shared__ fvec2 adjraw[kRowOwnerElems];
   fvec2* const wcraw = adjraw;
   int* const csrraw = (int*)(adjraw + kWcElems);
   __shared__ fvec2 t1raw[N];
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   lds_float2* tw1 = (lds_float2*)t1raw;
   lds_float2* tw2 = (lds_float2*)t2raw;
   lds_float2* xb = (lds_float2*)xzraw + p * XB;
-  lds_float2* Zt = (lds_float2*)xzraw;  // [16 rows][N] natural order
+  lds_float2* Zt = (lds_float2*)xzraw;  // [16 rows][N] natural order (row stride NS)
   const int n2 = a.n2;
   // dim-2 DFT of the sampled rows: length n2, or under SENSE folding along
   // dim 2 (fy1 == n1) the acquired length m2 = fy2 with the image rows
@@ -565,19 +568,26 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         }
         __syncthreads();  // last sample gathers done before the adjoint reuses Zt
       }
-      if (MODE == COIL_DECODE) {
+      if (MODE == COIL_DECODE && a.Ns > 0) {
+        // every load issued unconditionally (sample 0 and coil cc - 1 stand
+        // in for unused slots) so that all of them are in flight together;
+        // behind the per-sample and per-coil conditions each load was
+        // waited for before the next one was issued
+        float2 yv[kRJ][kCC];
 #pragma unroll
         for (int i = 0; i < kRJ; ++i) {
-          const int j = jg[i];
+          const long long jj = jk1[i] >= 0 ? jg[i] : 0;
+#pragma unroll
+          for (int c = 0; c < kCC; ++c)
+            yv[i][c] = a.yin[jj + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + min(c, cc - 1)))];
+        }
+#pragma unroll
+        for (int i = 0; i < kRJ; ++i) {
           if (jk1[i] >= 0) {
             const float2 ph = fold1_phase(jk1[i]);
 #pragma unroll
-            for (int c = 0; c < kCC; ++c) {
-              if (c < cc) {
-                const float2 yv = a.yin[j + (long long)a.Ns * (i3 + (long long)a.n3 * (c0 + c))];
-                W[i][c] = cscale(q1 > 1 ? cmulc(yv, ph) : yv, jw[i]);
-              }
-            }
+            for (int c = 0; c < kCC; ++c)
+              if (c < cc) W[i][c] = cscale(q1 > 1 ? cmulc(yv[i][c], ph) : yv[i][c], jw[i]);
           }
         }
       }
@@ -590,7 +600,8 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         // instead); the owner of a row that continues into later lanes adds
         // their carries in lane order once they are published.  Every lane
         // runs the same number of samples, and the sum is deterministic.
-        lds_float2* cy = (lds_float2*)adjraw;  // [16][kThreads], stored swapped like the tile
+        // tile [16][NS]: columns < N the rows, N + L lane L's carry (stored
+        // swapped like the rows)
         const bool deferred = (binfo & kAdjDeferred) != 0;
         const int ncont = (binfo >> kAdjNcontShift) & 1023;
         const int zrow = (binfo >> kAdjZrowShift) & 1023;
@@ -635,17 +646,15 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                 lrow = jk1[i];
                 if ((binfo >> (kAdjCloseShift + i)) & 1) {
                   const bool to_carry = (binfo >> (kAdjCarryShift + i)) & 1;
-                  lds_float2* dst = to_carry ? cy + tid : Zt + lrow;
-                  // one stride (immediate store offsets) when the tile is kThreads wide
-                  const int dstride = N == kThreads ? N : (to_carry ? kThreads : N);
+                  lds_float2* dst = Zt + (to_carry ? N + tid : lrow);
 #pragma unroll
-                  for (int q = 0; q < 16; ++q) st2(dst + q * dstride, cswap(acc[q]));
+                  for (int q = 0; q < 16; ++q) st2(dst + q * NS, cswap(acc[q]));
                 }
               }
             }
             if (binfo & kAdjZvalid) {  // an empty row dealt to this lane
 #pragma unroll
-              for (int q = 0; q < 16; ++q) st2(Zt + q * N + zrow, make_float2(0.f, 0.f));
+              for (int q = 0; q < 16; ++q) st2(Zt + q * NS + zrow, make_float2(0.f, 0.f));
             }
             __syncthreads();  // carries published
             if (deferred) {
@@ -653,21 +662,29 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
               for (int q = 0; q < 16; ++q) acc[q] = cswap(acc[q]);
               for (int l = 1; l <= ncont; ++l) {
 #pragma unroll
-                for (int q = 0; q < 16; ++q) acc[q] = cadd(acc[q], ld2(cy + q * kThreads + tid + l));
+                for (int q = 0; q < 16; ++q) acc[q] = cadd(acc[q], ld2(Zt + q * NS + N + tid + l));
               }
 #pragma unroll
-              for (int q = 0; q < 16; ++q) st2(Zt + q * N + lrow, acc[q]);
+              for (int q = 0; q < 16; ++q) st2(Zt + q * NS + lrow, acc[q]);
             }
             __syncthreads();
             float2 v[16];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * N + t + 16 * r);
+            for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * NS + t + 16 * r);
             __syncthreads();  // the row tile aliases the exchange buffer
             reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
             if (t < R1) {
               const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
+              // all 16 map loads in flight before the first use: with the
+              // registers near their limit the scheduler otherwise pairs
+              // every load with its wait (n1 = 224: 16 serial round trips
+              // per row group and coil, the adjoint 15.5 -> 26.5 ms)
+              float2 sv[16];
 #pragma unroll
-              for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), Sp[t + R1 * k]), inv_sqrt));
+              for (int k = 0; k < 16; ++k) sv[k] = Sp[t + R1 * k];
+              __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+              for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), sv[k]), inv_sqrt));
             }
           }
           if (a.tadj) {  // first translation of T^H along dim 1 (input layout of reg_fft<16,R1>)
@@ -766,8 +783,12 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
             reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
             if (t < R1) {
               const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
+              float2 sv[16];  // all map loads in flight before the first use
 #pragma unroll
-              for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), Sp[t + R1 * k]), inv_sqrt));
+              for (int k = 0; k < 16; ++k) sv[k] = Sp[t + R1 * k];
+              __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+              for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), sv[k]), inv_sqrt));
             }
           }
           if (a.tadj) {  // first translation of T^H along dim 1 (input layout of reg_fft<16,R1>)
