@@ -54,6 +54,8 @@ def parse():
                          "known motion, then the 3-level pyramid); fold: E^H E with SENSE folding; stream: device-copy "
                          "ceiling; proposal: learned-proposal acceptance against distance from the solution; lm: "
                          "LMsolver and its partials at 256x192x256")
+    ap.add_argument("--plain-only", dest="plain_only", action="store_true",
+                    help="pyramid workload: one plain solve only, without the reference and learned runs")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--outer", type=int, default=10)
     ap.add_argument("--cg", type=int, default=8)
@@ -266,6 +268,16 @@ def bench_pyramid(args):
                 "motion_err_rms_vox": float(np.sqrt(np.mean(T_err[:, :3] ** 2))),
                 "motion_err_rms_deg": float(np.sqrt(np.mean(T_err[:, 3:] ** 2)) * 180 / math.pi)}
 
+    if args.plain_only:
+        # one timed plain solve and nothing else (for kernel traces of the solve)
+        xp, Tp, ip, sp, _ = _run_pyramid(P, args)
+        return {"metric": "joint CG+LM pyramid solve time to converged motion (plain LM), s", "value": ip["solve_s"],
+                "unit": "s", "n_gpus": 1, "higher_is_better": False, "dtype": "complex64",
+                "data": "synthetic (seeded phantom, 32 Gaussian coils compressed to 8, DISORDER 8x8 sampling, "
+                        "random-walk motion, 25 dB noise)",
+                "config": {"workload": f"joint CG+LM pyramid at {n[0]}x{n[1]}x{n[2]}, 8 coils, {args.shots} shots, "
+                                       f"{args.levels} levels, plain solve only"},
+                "data_gen_s": gen_s, "plain": summary(xp, Tp, ip, sp)}
     xr, Tr, ir, sr, _ = _run_pyramid(P, args, tolerSolve=1e-6, label="reference")
     reference = summary(xr, Tr, ir, sr)
     xp, Tp, ip, sp, cp = _run_pyramid(P, args, x_ref=xr)

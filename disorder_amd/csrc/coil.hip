@@ -932,7 +932,9 @@ hipError_t launch_coil_persistent(const CoilArgs& a, int mode, int nstates, hipS
     case 10: return launch_rcoil_persistent<10>(a, mode, nstates, st);
     case 9: return launch_rcoil_persistent<9>(a, mode, nstates, st);
     case 8: return launch_rcoil_persistent<8>(a, mode, nstates, st);
+    case 6: return launch_rcoil_persistent<6>(a, mode, nstates, st);
     case 4: return launch_rcoil_persistent<4>(a, mode, nstates, st);
+    case 3: return launch_rcoil_persistent<3>(a, mode, nstates, st);
     case 2: return launch_rcoil_persistent<2>(a, mode, nstates, st);
     default: return hipErrorInvalidValue;
   }
@@ -956,7 +958,9 @@ hipError_t launch_coil(const CoilArgs& a, int mode, int nstates, hipStream_t st)
       case 10: return launch_rcoil<10>(a, mode, nstates, st);
       case 9: return launch_rcoil<9>(a, mode, nstates, st);
       case 8: return launch_rcoil<8>(a, mode, nstates, st);
+      case 6: return launch_rcoil<6>(a, mode, nstates, st);
       case 4: return launch_rcoil<4>(a, mode, nstates, st);
+      case 3: return launch_rcoil<3>(a, mode, nstates, st);
       case 2: return launch_rcoil<2>(a, mode, nstates, st);
       default: break;
     }

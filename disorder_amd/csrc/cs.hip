@@ -403,6 +403,8 @@ int run_rsh(dis_ctx* c, Shearlet* sh, const Pass& ps, AxisArgs& a, int R1, bool 
   switch (R1) {
     DIS_RR(2)
     DIS_RR(4)
+    DIS_RR(3)
+    DIS_RR(6)
     DIS_RR(8)
     DIS_RR(9)
     DIS_RR(10)

@@ -564,7 +564,9 @@ int reg_radix(int n) {
     case 160: return 10;
     case 144: return 9;
     case 128: return 8;
+    case 96: return 6;
     case 64: return 4;
+    case 48: return 3;
     case 32: return 2;
     default: return 0;
   }
@@ -602,7 +604,9 @@ hipError_t launch_pass(const PassArgs& a, int mode, int grid_states, hipStream_t
       case 10: return launch_rpass<10>(a, mode, grid_states, st);
       case 9: return launch_rpass<9>(a, mode, grid_states, st);
       case 8: return launch_rpass<8>(a, mode, grid_states, st);
+      case 6: return launch_rpass<6>(a, mode, grid_states, st);
       case 4: return launch_rpass<4>(a, mode, grid_states, st);
+      case 3: return launch_rpass<3>(a, mode, grid_states, st);
       case 2: return launch_rpass<2>(a, mode, grid_states, st);
     }
   }

@@ -47,6 +47,23 @@ template <> struct Dft<12> {
   }
 };
 
+template <> struct Dft<6> {
+  // r = 2a + b, k = c + 3d:  W6^{rk} = W3^{ac} W6^{bc} W2^{bd}
+  static __device__ __forceinline__ void run(float2* v) {
+    float2 e[3] = {v[0], v[2], v[4]};
+    float2 o[3] = {v[1], v[3], v[5]};
+    DftOdd<3>::run(e);
+    DftOdd<3>::run(o);
+    o[1] = wconst(o[1], 0.5f, DIS_SQRT3_2);   // W6^1
+    o[2] = wconst(o[2], -0.5f, DIS_SQRT3_2);  // W6^2
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      v[c] = cadd(e[c], o[c]);
+      v[c + 3] = csub(e[c], o[c]);
+    }
+  }
+};
+
 template <> struct Dft<9> {
   // r = 3a + b, k = c + 3d:  W9^{rk} = W3^{ac} W9^{bc} W3^{bd}
   static __device__ __forceinline__ void run(float2* v) {

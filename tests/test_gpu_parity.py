@@ -32,6 +32,8 @@ def to_np(t):
                                      ((224, 16, 10), None),
                                      # odd radix-9 register path (144) on every axis
                                      ((144, 12, 10), None), ((16, 144, 12), (32, 144, 24)), ((12, 16, 144), None),
+                                     # radix-6 and odd radix-3 register paths (96, 48) on every axis
+                                     ((96, 16, 12), None), ((16, 96, 48), (32, 192, 96)), ((48, 12, 96), None),
                                      # prime factors 11 and 13 (runtime-radix stage): 11, 13, 22, 26, 176
                                      # on every axis, and a prime length 97
                                      ((11, 13, 22), None), ((26, 11, 13), (52, 22, 26)), ((22, 26, 11), None),
@@ -165,6 +167,9 @@ def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
                                                     # register coil stage at n1 = 160 and 224
                                                     ((160, 16, 4), 4, 4, None, True),
                                                     ((224, 12, 4), 3, 3, None, False),
+                                                    # register coil stage at n1 = 96 and 48
+                                                    ((96, 20, 8), 3, 4, None, True),
+                                                    ((48, 24, 6), 2, 5, None, False),
                                                     # prime factors 11 and 13 on every axis (runtime-radix
                                                     # stage in the passes and the generic coil stage)
                                                     ((11, 13, 22), 3, 5, None, True),
