@@ -486,59 +486,68 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
             for (int r = 0; r < R1; ++r) u[r] = cscale(cswap(u[r]), inv_n);
           }
-          for (int c = 0; c < cc; ++c) {
-            const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
-            float2 v[16];
+          // W[i][0] is the current coil's accumulator: the row of eight is
+          // rotated by one after every coil (eight rotations per group, the
+          // identity), so the sums take a static register index instead of a
+          // select over all eight
+          for (int c = 0; c < kCC; ++c) {
+            if (c < cc) {
+              const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
+              float2 v[16];
 #pragma unroll
-            for (int r = 0; r < R1; ++r) v[r] = cmul(u[r], Sp[t + 16 * r]);
-            reg_fft<R1, 16>(v, t, xb, tw1);
-            if (t < R1) {
+              for (int r = 0; r < R1; ++r) v[r] = cmul(u[r], Sp[t + 16 * r]);
+              reg_fft<R1, 16>(v, t, xb, tw1);
+              if (t < R1) {
 #pragma unroll
-              for (int k = 0; k < 16; ++k) st2(Zt + p * N + t + R1 * k, v[k]);
-            }
-            __syncthreads();
-            if (Gv == 16) {
-              // full group: no per-row guard, so the LDS reads of a sample's
-              // Horner chain are issued kHornerBatch at a time
-#pragma unroll
-              for (int i = 0; i < kRJ; ++i) {
-                if (jk1[i] >= 0) {
-                  const float2 a1 = ld2(tw2 + jk2[i]);
-                  float2 acc = ld2(Zt + 15 * N + jk1[i] * q1);
-#pragma unroll
-                  for (int q = 14; q >= 0; --q) {
-                    const float2 z = ld2(Zt + q * N + jk1[i] * q1);
-                    acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
-                    if ((q & (kHornerBatch - 1)) == 0) __builtin_amdgcn_sched_barrier(0);
-                  }
-                  acc = cmul(acc, ld2(tw2 + ib[i]));
-#pragma unroll
-                  for (int cq = 0; cq < kCC; ++cq)
-                    if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
-                }
+                for (int k = 0; k < 16; ++k) st2(Zt + p * N + t + R1 * k, v[k]);
               }
-            } else {
+              __syncthreads();
+              if (Gv == 16) {
+                // full group: no per-row guard, so the LDS reads of a sample's
+                // Horner chain are issued kHornerBatch at a time
 #pragma unroll
-              for (int i = 0; i < kRJ; ++i) {
-                if (jk1[i] >= 0) {
-                  // sum_q W2^{k2 (r0 + q)} Z[q] = W2^{k2 r0} * (Horner in a = W2^{k2})
-                  const float2 a1 = ld2(tw2 + jk2[i]);
-                  float2 acc = make_float2(0.f, 0.f);
+                for (int i = 0; i < kRJ; ++i) {
+                  if (jk1[i] >= 0) {
+                    const float2 a1 = ld2(tw2 + jk2[i]);
+                    float2 acc = ld2(Zt + 15 * N + jk1[i] * q1);
 #pragma unroll
-                  for (int q = 15; q >= 0; --q) {
-                    if (q < Gv) {
+                    for (int q = 14; q >= 0; --q) {
                       const float2 z = ld2(Zt + q * N + jk1[i] * q1);
                       acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
+                      if ((q & (kHornerBatch - 1)) == 0) __builtin_amdgcn_sched_barrier(0);
                     }
+                    acc = cmul(acc, ld2(tw2 + ib[i]));
+                    W[i][0] = cadd(W[i][0], acc);
                   }
-                  acc = cmul(acc, ld2(tw2 + ib[i]));
+                }
+              } else {
 #pragma unroll
-                  for (int cq = 0; cq < kCC; ++cq)
-                    if (cq == c) W[i][cq] = cadd(W[i][cq], acc);
+                for (int i = 0; i < kRJ; ++i) {
+                  if (jk1[i] >= 0) {
+                    // sum_q W2^{k2 (r0 + q)} Z[q] = W2^{k2 r0} * (Horner in a = W2^{k2})
+                    const float2 a1 = ld2(tw2 + jk2[i]);
+                    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+                    for (int q = 15; q >= 0; --q) {
+                      if (q < Gv) {
+                        const float2 z = ld2(Zt + q * N + jk1[i] * q1);
+                        acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
+                      }
+                    }
+                    acc = cmul(acc, ld2(tw2 + ib[i]));
+                    W[i][0] = cadd(W[i][0], acc);
+                  }
                 }
               }
+              __syncthreads();  // the row tile aliases the next FFT's exchange buffer
             }
-            __syncthreads();  // the row tile aliases the next FFT's exchange buffer
+#pragma unroll
+            for (int i = 0; i < kRJ; ++i) {
+              const float2 w0 = W[i][0];
+#pragma unroll
+              for (int cq = 0; cq + 1 < kCC; ++cq) W[i][cq] = W[i][cq + 1];
+              W[i][kCC - 1] = w0;
+            }
           }
 #pragma unroll
           for (int i = 0; i < kRJ; ++i) {
@@ -615,76 +624,84 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
           float2 o[16];
 #pragma unroll
           for (int k = 0; k < 16; ++k) o[k] = make_float2(0.f, 0.f);
-          for (int c = 0; c < cc; ++c) {
-            __syncthreads();  // the previous FFT's exchange (aliasing the tile) and carry reads are done
-            float2 acc[16];
+          // W[i][0] holds coil c: the row of eight is rotated by one after
+          // every coil (eight rotations per group, the identity)
+          for (int c = 0; c < kCC; ++c) {
+            if (c < cc) {
+              __syncthreads();  // the previous FFT's exchange (aliasing the tile) and carry reads are done
+              float2 acc[16];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
-            int lrow = 0;
+              for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
+              int lrow = 0;
+#pragma unroll
+              for (int i = 0; i < kRJ; ++i) {
+                if (jk1[i] >= 0) {
+                  const float2 w = W[i][0];  // coil c (the row is rotated after every coil)
+                  const int k2 = jk2[i];
+                  const int prod = k2 * (r0 + o2p);  // = k2 (r0 - o2) modulo m2
+                  int idx = prod - m2 * (int)((float)prod * fn2);
+                  idx += idx < 0 ? m2 : 0;
+                  idx -= idx >= m2 ? m2 : 0;
+                  // conj(W2^{k2 (r0+q)}) w = (w conj(W2^{k2 r0})) conj(W2^{k2})^q
+                  float2 wq = cmulc(w, ld2(tw2 + idx));
+                  const float2 pst = ld2(tw2 + k2);
+                  // a segment closed at the previous entry restarts the sum
+                  const float keep = i > 0 && ((binfo >> (kAdjCloseShift + i - 1)) & 1) ? 0.f : 1.f;
+#pragma unroll
+                  for (int q = 0; q < 16; ++q) {
+                    acc[q] = make_float2(fmaf(acc[q].x, keep, wq.x), fmaf(acc[q].y, keep, wq.y));
+                    wq = cmulc(wq, pst);
+                  }
+                  lrow = jk1[i];
+                  if ((binfo >> (kAdjCloseShift + i)) & 1) {
+                    const bool to_carry = (binfo >> (kAdjCarryShift + i)) & 1;
+                    lds_float2* dst = Zt + (to_carry ? N + tid : lrow);
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) st2(dst + q * NS, cswap(acc[q]));
+                  }
+                }
+              }
+              if (binfo & kAdjZvalid) {  // an empty row dealt to this lane
+#pragma unroll
+                for (int q = 0; q < 16; ++q) st2(Zt + q * NS + zrow, make_float2(0.f, 0.f));
+              }
+              __syncthreads();  // carries published
+              if (deferred) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) acc[q] = cswap(acc[q]);
+                for (int l = 1; l <= ncont; ++l) {
+#pragma unroll
+                  for (int q = 0; q < 16; ++q) acc[q] = cadd(acc[q], ld2(Zt + q * NS + N + tid + l));
+                }
+#pragma unroll
+                for (int q = 0; q < 16; ++q) st2(Zt + q * NS + lrow, acc[q]);
+              }
+              __syncthreads();
+              float2 v[16];
+#pragma unroll
+              for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * NS + t + 16 * r);
+              __syncthreads();  // the row tile aliases the exchange buffer
+              reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
+              if (t < R1) {
+                const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
+                // all 16 map loads in flight before the first use: with the
+                // registers near their limit the scheduler otherwise pairs
+                // every load with its wait (n1 = 224: 16 serial round trips
+                // per row group and coil, the adjoint 15.5 -> 26.5 ms)
+                float2 sv[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) sv[k] = Sp[t + R1 * k];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), sv[k]), inv_sqrt));
+              }
+            }
 #pragma unroll
             for (int i = 0; i < kRJ; ++i) {
-              if (jk1[i] >= 0) {
-                float2 w = make_float2(0.f, 0.f);
+              const float2 w0 = W[i][0];
 #pragma unroll
-                for (int cq = 0; cq < kCC; ++cq)
-                  if (cq == c) w = W[i][cq];
-                const int k2 = jk2[i];
-                const int prod = k2 * (r0 + o2p);  // = k2 (r0 - o2) modulo m2
-                int idx = prod - m2 * (int)((float)prod * fn2);
-                idx += idx < 0 ? m2 : 0;
-                idx -= idx >= m2 ? m2 : 0;
-                // conj(W2^{k2 (r0+q)}) w = (w conj(W2^{k2 r0})) conj(W2^{k2})^q
-                float2 wq = cmulc(w, ld2(tw2 + idx));
-                const float2 pst = ld2(tw2 + k2);
-                // a segment closed at the previous entry restarts the sum
-                const float keep = i > 0 && ((binfo >> (kAdjCloseShift + i - 1)) & 1) ? 0.f : 1.f;
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                  acc[q] = make_float2(fmaf(acc[q].x, keep, wq.x), fmaf(acc[q].y, keep, wq.y));
-                  wq = cmulc(wq, pst);
-                }
-                lrow = jk1[i];
-                if ((binfo >> (kAdjCloseShift + i)) & 1) {
-                  const bool to_carry = (binfo >> (kAdjCarryShift + i)) & 1;
-                  lds_float2* dst = Zt + (to_carry ? N + tid : lrow);
-#pragma unroll
-                  for (int q = 0; q < 16; ++q) st2(dst + q * NS, cswap(acc[q]));
-                }
-              }
-            }
-            if (binfo & kAdjZvalid) {  // an empty row dealt to this lane
-#pragma unroll
-              for (int q = 0; q < 16; ++q) st2(Zt + q * NS + zrow, make_float2(0.f, 0.f));
-            }
-            __syncthreads();  // carries published
-            if (deferred) {
-#pragma unroll
-              for (int q = 0; q < 16; ++q) acc[q] = cswap(acc[q]);
-              for (int l = 1; l <= ncont; ++l) {
-#pragma unroll
-                for (int q = 0; q < 16; ++q) acc[q] = cadd(acc[q], ld2(Zt + q * NS + N + tid + l));
-              }
-#pragma unroll
-              for (int q = 0; q < 16; ++q) st2(Zt + q * NS + lrow, acc[q]);
-            }
-            __syncthreads();
-            float2 v[16];
-#pragma unroll
-            for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * NS + t + 16 * r);
-            __syncthreads();  // the row tile aliases the exchange buffer
-            reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
-            if (t < R1) {
-              const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
-              // all 16 map loads in flight before the first use: with the
-              // registers near their limit the scheduler otherwise pairs
-              // every load with its wait (n1 = 224: 16 serial round trips
-              // per row group and coil, the adjoint 15.5 -> 26.5 ms)
-              float2 sv[16];
-#pragma unroll
-              for (int k = 0; k < 16; ++k) sv[k] = Sp[t + R1 * k];
-              __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-              for (int k = 0; k < 16; ++k) o[k] = cadd(o[k], cscale(cmulc(cswap(v[k]), sv[k]), inv_sqrt));
+              for (int cq = 0; cq + 1 < kCC; ++cq) W[i][cq] = W[i][cq + 1];
+              W[i][kCC - 1] = w0;
             }
           }
           if (a.tadj) {  // first translation of T^H along dim 1 (input layout of reg_fft<16,R1>)
