@@ -64,7 +64,8 @@ struct dis_plan {
   int* rowptr = nullptr;  // [S][n1+1] per-state CSR by k1
   int* csr = nullptr;     // [Ns]
   int* sord = nullptr;    // [Ns] register coil stage: sample of each slot
-  int* adjtab = nullptr;  // [S][kThreads] int4 register-path adjoint lane table, or null
+  int* adjtab = nullptr;  // register-path adjoint lane table: blocks of kThreads int4, or null
+  int* adjoff = nullptr;  // [S+1] first table block of each state
   std::vector<int> hoff;
   int fold[2] = {0, 0};   // acquired phase-encode grid under SENSE folding
   int roi[2] = {0, 0};    // readout range [lo, hi) of the LM sums (E.nF); {0, 0} = all
@@ -472,10 +473,12 @@ int dis_plan_create(dis_ctx* c, const dis_geom* g, const void* S_maps, const int
          hipMalloc(&p->csr, ent.size() * sizeof(int)) == hipSuccess &&
          hipMemcpy(p->rowptr, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(p->csr, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
-    std::vector<int> tab;
-    if (ok && adjoint_lane_table(g->n[0], g->n[1], g->nstates, state_off, rp, ent, &tab))
+    std::vector<int> tab, toff;
+    if (ok && adjoint_lane_table(g->n[0], g->n[1], g->nstates, state_off, rp, ent, &tab, &toff))
       ok = hipMalloc(&p->adjtab, tab.size() * sizeof(int)) == hipSuccess &&
-           hipMemcpy(p->adjtab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+           hipMalloc(&p->adjoff, toff.size() * sizeof(int)) == hipSuccess &&
+           hipMemcpy(p->adjtab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(p->adjoff, toff.data(), toff.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
   }
   // slot -> sample table of the register coil stage: per state and chunk of
   // kCoilRegChunk samples, slot q takes a sample with k1 = q (mod 32) while
@@ -526,6 +529,7 @@ int dis_plan_destroy(dis_plan* p) {
   if (p->csr) (void)hipFree(p->csr);
   if (p->sord) (void)hipFree(p->sord);
   if (p->adjtab) (void)hipFree(p->adjtab);
+  if (p->adjoff) (void)hipFree(p->adjoff);
   if (p->Fd) (void)hipFree(p->Fd);
   delete p;
   return DIS_OK;
@@ -625,6 +629,7 @@ OpView plan_view(dis_plan* p) {
   v.csr = p->csr;
   v.sord = p->sord;
   v.adjtab = p->adjtab;
+  v.adjoff = p->adjoff;
   v.g = p->g;
   v.hoff = p->hoff.data();
   v.fold[0] = p->fold[0];

@@ -418,7 +418,8 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
     float jw[kRJ];
     int bent[kRJ];
     if (bal) {
-      const int4 tb = ((const int4*)a.adjtab)[(long long)s * kThreads + tid];
+      const int ch = (sc0 - o0) / (kRJ * kThreads);  // this chunk's table block
+      const int4 tb = ((const int4*)a.adjtab)[((long long)a.adjoff[s] + ch) * kThreads + tid];
       bent[0] = tb.x;
       bent[1] = tb.y;
       bent[2] = tb.z;
@@ -428,10 +429,10 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
     for (int i = 0; i < kRJ; ++i) {
       if (bal) {
         // lane-table entry: row (k1), k2 and the sample within its state
-        const bool ok = bent[i] >= 0;
-        const int j = ok ? o0 + ((bent[i] >> 20) & 1023) : -1;
+        const bool ok = (binfo >> (kAdjValidShift + i)) & 1;
+        const int j = ok ? o0 + ((bent[i] >> kAdjSampleShift) & 0x3fff) : -1;
         jg[i] = j;
-        jk1[i] = ok ? (bent[i] >> 10) & 1023 : -1;
+        jk1[i] = ok ? (bent[i] >> kAdjRowShift) & 0xff : -1;
         jk2[i] = ok ? bent[i] & 1023 : 0;
         jw[i] = ok ? (a.We ? a.We[j] : 1.0f) : 0.f;
       } else {

@@ -92,12 +92,12 @@ struct CoilArgs {
   const int* csr;         // [Ns] (k2 << 16) | (sample index within its state), sorted by k1 per state
   const int* slist;       // optional [nstates] list of the states handled (U/V indexed by list position)
   const int* sord;        // optional [Ns] register path: sample handled by each slot, per chunk of kCoilRegChunk
-  // optional [S][kThreads] int4 register-path adjoint lane table (states of
-  // at most kCoilRegChunk samples): lane L takes row-sorted samples
-  // cs L .. cs L + cs - 1 (cs = ceil(samples / kThreads)); .x/.y/.z =
-  // k2 | row << 10 | (sample within its state) << 20, or -1; .w = flags
-  // (kAdj* of lanetab.hpp)
+  // optional register-path adjoint lane table (lanetab.hpp): one block of
+  // kThreads int4 per state and chunk of kCoilRegChunk row-sorted samples,
+  // lane L taking chunk samples cs L .. cs L + cs - 1 (cs = ceil(samples in
+  // the chunk / kThreads)); adjoff [S+1] gives each state's first block
   const int* adjtab;
+  const int* adjoff;
   int* queue;             // persistent launch: 8 per-XCD work counters (zeroed before the launch), else null
   // SENSE folding: acquired phase-encode grid fy1 x fy2 smaller than n1 x n2
   // (0 = none); k1/k2 then index the acquired grid

@@ -455,6 +455,7 @@ int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, cons
     a.csr = v.csr;
     a.sord = v.sord;
     a.adjtab = v.adjtab;
+    a.adjoff = v.adjoff;
     a.Ns = g.nsamples;
     a.yin = yin;
     a.yout = yout;
@@ -892,6 +893,7 @@ int run_lm_impl(dis_ctx* c, const OpView& v, double* T, double* w, unsigned char
     a.csr = v.csr;
     a.sord = v.sord;
     a.adjtab = v.adjtab;
+    a.adjoff = v.adjoff;
     a.Ns = g.nsamples;
     a.yin = y;
     a.yout = nullptr;

@@ -41,6 +41,7 @@ struct OpView {
   const int* csr;
   const int* sord;  // register coil stage: sample of each slot (or null)
   const int* adjtab = nullptr;  // register coil stage: adjoint lane table (or null)
+  const int* adjoff = nullptr;  // its first block per state
   const int* hoff;  // host copy of the offsets
   int fold[2];      // acquired phase-encode grid when SENSE folding is on, else {0, 0}
   int roi[2];       // readout planes [roi[0], roi[1]) of LMsolver (E.nF), or {0, 0} for all

@@ -9,8 +9,9 @@
 //
 //   lanetab_harness TRIALS SEED [plant]
 // prints "ok <rows>" and exits 0, or names the first violation and exits 1.
-// With "plant", one close bit of one lane per state is flipped after the
-// table is built, and the run must report a violation.
+// With "plant", one close bit of one lane per table block is flipped after
+// the table is built, and the run must report a violation.  States hold up
+// to 2,000 samples, so that they span up to three blocks of 768.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -20,22 +21,24 @@
 
 using namespace dis;
 
-static int check_state(const int* T, int n1, const std::vector<int>& k1, const std::vector<int>& k2, int o0, int o1,
-                       std::mt19937& rng, long long* rows) {
-  std::vector<long long> val(o1 - o0), Z(n1, 0), direct(n1, 0), carry(kAdjLanes, 0), acc_end(kAdjLanes, 0);
-  std::vector<int> writes(n1, 0), lrow_end(kAdjLanes, 0);
-  for (auto& v : val) v = 1 + rng() % 1000003;
-  for (int j = o0; j < o1; ++j) direct[k1[j]] += val[j - o0];
+// One table block: `chunk` lists the samples (within the state) of the
+// block's chunk; `val` holds every sample value of the state.
+static int check_block(const int* T, int n1, const std::vector<int>& k1, const std::vector<int>& k2, int o0,
+                       const std::vector<int>& chunk, const std::vector<long long>& val, long long* rows) {
+  std::vector<long long> Z(n1, 0), direct(n1, 0), carry(kAdjLanes, 0), acc_end(kAdjLanes, 0);
+  std::vector<int> writes(n1, 0), lrow_end(kAdjLanes, 0), seen(val.size(), 0);
+  for (int jl : chunk) direct[k1[o0 + jl]] += val[jl];
   for (int L = 0; L < kAdjLanes; ++L) {
     const int info = T[4 * L + 3];
     long long acc = 0;
     int lrow = 0, last = -1;
     for (int i = 0; i < 3; ++i) {
+      if (!((info >> (kAdjValidShift + i)) & 1)) continue;
       const int e = T[4 * L + i];
-      if (e < 0) continue;
       last = i;
-      const int kk2 = e & 1023, row = (e >> 10) & 1023, jl = (e >> 20) & 1023;
-      if (jl >= o1 - o0 || k1[o0 + jl] != row || k2[o0 + jl] != kk2) return printf("entry %d of lane %d\n", i, L), 1;
+      const int kk2 = e & 1023, row = (e >> kAdjRowShift) & 0xff, jl = (e >> kAdjSampleShift) & 0x3fff;
+      if (jl >= (int)val.size() || k1[o0 + jl] != row || k2[o0 + jl] != kk2) return printf("entry %d of lane %d\n", i, L), 1;
+      if (seen[jl]++) return printf("sample %d dealt twice\n", jl), 1;
       const long long keep = i > 0 && ((info >> (kAdjCloseShift + i - 1)) & 1) ? 0 : 1;
       acc = acc * keep + val[jl];
       lrow = row;
@@ -73,6 +76,8 @@ static int check_state(const int* T, int n1, const std::vector<int>& k1, const s
     if (writes[k] != 1) return printf("row %d written %d times\n", k, writes[k]), 1;
     if (Z[k] != direct[k]) return printf("row %d: %lld against %lld\n", k, Z[k], direct[k]), 1;
   }
+  for (int jl : chunk)
+    if (!seen[jl]) return printf("sample %d of the chunk not dealt\n", jl), 1;
   *rows += n1;
   return 0;
 }
@@ -90,7 +95,8 @@ int main(int argc, char** argv) {
     for (int s = 0; s < S; ++s) {
       // empty, a few samples, exactly 768, any count, or half of them in one row
       const int mode = rng() % 5;
-      const int ns = mode == 0 ? 0 : mode == 1 ? 1 + rng() % 10 : mode == 2 ? kAdjMaxSamples : (int)(rng() % (kAdjMaxSamples + 1));
+      const int ns = mode == 0 ? 0 : mode == 1 ? 1 + rng() % 10 : mode == 2 ? kAdjMaxSamples * (1 + rng() % 2)
+                                                                                  : (int)(rng() % 2001);
       const int hot = rng() % n1;
       for (int j = 0; j < ns; ++j) {
         k1.push_back((mode == 4 && rng() % 2) ? hot : (int)(rng() % n1));
@@ -107,20 +113,31 @@ int main(int argc, char** argv) {
       std::vector<int> fill(r, r + n1);
       for (int j = off[s]; j < off[s + 1]; ++j) ent[off[s] + fill[k1[j]]++] = (k2[j] << 16) | (j - off[s]);
     }
-    std::vector<int> tab;
-    if (!adjoint_lane_table(n1, n2, S, off.data(), rp, ent, &tab)) return printf("no table\n"), 1;
+    std::vector<int> tab, toff;
+    if (!adjoint_lane_table(n1, n2, S, off.data(), rp, ent, &tab, &toff)) return printf("no table\n"), 1;
     for (int s = 0; s < S; ++s) {
-      int* T = tab.data() + (size_t)s * kAdjLanes * 4;
-      bool flipped = false;
-      if (plant && off[s + 1] > off[s]) {
-        const int L = rng() % ((off[s + 1] - off[s] + 2) / 3);  // a lane that holds samples
-        T[4 * L + 3] ^= 1 << kAdjCloseShift;
-        flipped = true;
-        ++planted;
+      const int ns = off[s + 1] - off[s];
+      std::vector<long long> val(ns), total(n1, 0), direct(n1, 0);
+      for (auto& v : val) v = 1 + rng() % 1000003;
+      const int nch = toff[s + 1] - toff[s];
+      if (nch != std::max(1, (ns + kAdjMaxSamples - 1) / kAdjMaxSamples)) return printf("state %d: %d blocks\n", s, nch), 1;
+      for (int ch = 0; ch < nch; ++ch) {
+        // the chunk: row-sorted entries [768 ch, 768 ch + 768) of the state's CSR
+        std::vector<int> chunk;
+        for (int e = ch * kAdjMaxSamples; e < std::min(ns, (ch + 1) * kAdjMaxSamples); ++e)
+          chunk.push_back(ent[off[s] + e] & 0xffff);
+        int* T = tab.data() + (size_t)(toff[s] + ch) * kAdjLanes * 4;
+        bool flipped = false;
+        if (plant && !chunk.empty()) {
+          const int L = rng() % ((chunk.size() + 2) / 3);  // a lane that holds samples
+          T[4 * L + 3] ^= 1 << kAdjCloseShift;
+          flipped = true;
+          ++planted;
+        }
+        const int bad = check_block(T, n1, k1, k2, off[s], chunk, val, &rows);
+        if (bad && !flipped) return 1;
+        if (bad) ++caught;
       }
-      const int bad = check_state(T, n1, k1, k2, off[s], off[s + 1], rng, &rows);
-      if (bad && !flipped) return 1;
-      if (bad) ++caught;
     }
   }
   if (plant) {

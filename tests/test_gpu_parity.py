@@ -164,6 +164,8 @@ def _problem(rng, n, C, S, nmotion=None, weights=False, Nfull=None):
                                                     # chunks per workgroup on the register path
                                                     ((64, 28, 4), 8, 2, None, True),
                                                     ((64, 40, 4), 8, 2, None, False),
+                                                    # one state of 3,072 samples: four adjoint table blocks
+                                                    ((64, 48, 4), 3, 1, None, True),
                                                     # register coil stage at n1 = 160 and 224
                                                     ((160, 16, 4), 4, 4, None, True),
                                                     ((224, 12, 4), 3, 3, None, False),

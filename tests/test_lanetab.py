@@ -1,10 +1,11 @@
 """The register coil stage's adjoint lane table (disorder_amd/csrc/lanetab.hpp)
 on the CPU: tests/native/lanetab_harness.cpp (built here with g++) builds the
-table for random states (empty, one to ten samples, exactly 768, any count,
-half of the samples in one row) and runs the kernel's use of it on integer
-sample values; every row must equal the direct sum of its samples and be
-written once.  A flipped close bit must be caught in every state it is
-planted in."""
+table for random states (empty, one to ten samples, 768 or 1,536, any count
+up to 2,000, half of the samples in one row; one block per chunk of 768
+row-sorted samples) and runs the kernel's use of each block on integer
+sample values; every row must equal the direct sum of its chunk's samples
+and be written once, every sample dealt once.  A flipped close bit must be
+caught in every block it is planted in."""
 import os
 import shutil
 import subprocess
