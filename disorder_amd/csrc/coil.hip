@@ -341,7 +341,13 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   // 64 elements (512 bytes) so that a segment's 16 stores pair into
   // ds_write2st64 (n1 = 224 with stride 480: 82 single stores, 10 % slower)
   constexpr int NS = BAL ? (N + kThreads + 63) / 64 * 64 : N;
-  constexpr int XZ = (16 * XB > 16 * NS) ? 16 * XB : 16 * NS;
+  // forward: the 16-row tile is stored column-major (spectrum column k1's
+  // 16 rows contiguous, columns ZS = 18 elements apart, 16-byte aligned), so
+  // a sample's Horner chain reads its column with 16-byte loads
+  constexpr int ZS = 18;
+  constexpr int XZF = FWD ? N * ZS : 0;
+  constexpr int XZ0 = (16 * XB > 16 * NS) ? 16 * XB : 16 * NS;
+  constexpr int XZ = XZ0 > XZF ? XZ0 : XZF;
   constexpr int kEntCap = 2048;
   __shared__ fvec2 xzraw[XZ];                 // FFT exchange, aliased with the 16-row tile
   // row-owner adjoint: the sample values of the current coil and the
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
               reg_fft<R1, 16>(v, t, xb, tw1);
               if (t < R1) {
 #pragma unroll
-                for (int k = 0; k < 16; ++k) st2(Zt + p * N + t + R1 * k, v[k]);
+                for (int k = 0; k < 16; ++k) st2(Zt + (t + R1 * k) * ZS + p, v[k]);
               }
               __syncthreads();
               if (Gv == 16) {
@@ -510,10 +516,11 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                 for (int i = 0; i < kRJ; ++i) {
                   if (jk1[i] >= 0) {
                     const float2 a1 = ld2(tw2 + jk2[i]);
-                    float2 acc = ld2(Zt + 15 * N + jk1[i] * q1);
+                    const lds_float2* zc = Zt + jk1[i] * q1 * ZS;  // the sample's column
+                    float2 acc = ld2(zc + 15);
 #pragma unroll
                     for (int q = 14; q >= 0; --q) {
-                      const float2 z = ld2(Zt + q * N + jk1[i] * q1);
+                      const float2 z = ld2(zc + q);
                       acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
                       if ((q & (kHornerBatch - 1)) == 0) __builtin_amdgcn_sched_barrier(0);
                     }
@@ -531,7 +538,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
 #pragma unroll
                     for (int q = 15; q >= 0; --q) {
                       if (q < Gv) {
-                        const float2 z = ld2(Zt + q * N + jk1[i] * q1);
+                        const float2 z = ld2(Zt + jk1[i] * q1 * ZS + q);
                         acc = make_float2(fmaf(acc.x, a1.x, fmaf(-acc.y, a1.y, z.x)), fmaf(acc.x, a1.y, fmaf(acc.y, a1.x, z.y)));
                       }
                     }
