@@ -37,6 +37,18 @@ struct dis_ctx {
   int coil_schedule = 0;  // 0 auto, 1 static grid, 2 persistent work queue
   long long cache_budget = 208ll << 20;  // bytes of state volumes a transform keeps in flight (0 = all states)
   int tape_mode = 0;  // 0 off, 1 record, 2 replay
+  // pinned host staging of the coefficient uploads: a ring of slots, each
+  // reused once the event recorded behind its last copy has completed, so an
+  // upload does not wait for the stream (the device buffer dcoef is rewritten
+  // in stream order, after the kernels that read its previous contents)
+  struct PinSlot {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool used = false;
+  };
+  std::array<PinSlot, 4> pins;
+  int pin_next = 0;
   std::vector<std::pair<void*, size_t>> tape;
   size_t tape_pos = 0;
   // profiling
@@ -182,9 +194,24 @@ int ctx_upload_bytes(dis_ctx* c, const Staging& stg, const unsigned char** dptr)
     HIPCHK(c, hipMalloc(&c->dcoef, cap));
     c->dcoef_cap = cap;
   }
-  if (!stg.bytes.empty())
-    HIPCHK(c, hipMemcpyAsync(c->dcoef, stg.bytes.data(), stg.bytes.size(), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!stg.bytes.empty()) {
+    dis_ctx::PinSlot& ps = c->pins[c->pin_next];
+    c->pin_next = (c->pin_next + 1) % (int)c->pins.size();
+    if (ps.used) HIPCHK(c, hipEventSynchronize(ps.ev));  // the slot's previous copy has read it
+    if (ps.cap < stg.bytes.size()) {
+      if (ps.p) HIPCHK(c, hipHostFree(ps.p));
+      ps.p = nullptr;
+      ps.cap = 0;
+      const size_t cap = std::max<size_t>(stg.bytes.size(), 1 << 16);
+      HIPCHK(c, hipHostMalloc(&ps.p, cap));
+      ps.cap = cap;
+    }
+    if (!ps.ev) HIPCHK(c, hipEventCreateWithFlags(&ps.ev, hipEventDisableTiming));
+    std::memcpy(ps.p, stg.bytes.data(), stg.bytes.size());
+    HIPCHK(c, hipMemcpyAsync(c->dcoef, ps.p, stg.bytes.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(ps.ev, c->stream));
+    ps.used = true;
+  }
   *dptr = (const unsigned char*)c->dcoef;
   return DIS_OK;
 }
@@ -331,6 +358,10 @@ int dis_ctx_destroy(dis_ctx* c) {
   for (auto& kv : c->bufs)
     if (kv.second.first) (void)hipFree(kv.second.first);
   if (c->dcoef) (void)hipFree(c->dcoef);
+  for (auto& ps : c->pins) {
+    if (ps.p) (void)hipHostFree(ps.p);
+    if (ps.ev) (void)hipEventDestroy(ps.ev);
+  }
   for (auto& b : c->tape) (void)hipFree(b.first);
   (void)hipStreamDestroy(c->stream);
   delete c;
