@@ -63,6 +63,36 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
     return DIS_OK;
   };
 
+  // 'Energy' (CGsolver.m:87, computeEnergy of every odd iterate): E x of the
+  // iterate is kept in k-space, E x_{k+1} = E x_k + g_k E p_k with E p_k the
+  // forward half that E^H E leaves in its k-space scratch, so the fidelity
+  // energy sum |E x - y|^2 is a reduction instead of another forward encode
+  // (operators whose E^H E keeps no k-space scratch re-encode)
+  float2* Ep = tolType == 6 ? ehe_kspace(c, v) : nullptr;
+  float2* Ex = nullptr;
+  const long long ny = (long long)v.g.nsamples * v.g.n[2] * v.g.ncoils;
+  if (Ep && ny > 0) {
+    void* eb = nullptr;
+    rc = ctx_buffer(c, 9, sizeof(float2) * ny, &eb);
+    if (rc) return rc;
+    Ex = (float2*)eb;
+  } else {
+    Ep = nullptr;
+  }
+  auto kenergy = [&](double* out2, double fid) -> int {
+    double reg = 0;
+    if (lam != 0) {
+      CCHK(c, cg_dot(x, x, n, part, &sc->dot, st));
+      double2 h;
+      CCHK(c, hipMemcpyAsync(&h, &sc->dot, sizeof(h), hipMemcpyDeviceToHost, st));
+      CCHK(c, hipStreamSynchronize(st));
+      reg = std::fabs(lam * h.x);
+    }
+    out2[0] = fid;
+    out2[1] = reg;
+    return DIS_OK;
+  };
+
   // b = C(decode(y)) (CGsolver.m:46) into r
   rc = op_apply(c, v, T, nullptr, y, nullptr, r, COIL_DECODE, nullptr);
   if (rc) return rc;
@@ -73,10 +103,12 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
     rc = op_apply(c, v, T, x, nullptr, nullptr, q, COIL_EHE, nullptr);
     if (!rc && sh) rc = shearlet_apply(c, sh, x, sh_scale, q);
     if (rc) return rc;
+    if (Ex) CCHK(c, hipMemcpyAsync(Ex, Ep, sizeof(float2) * ny, hipMemcpyDeviceToDevice, st));  // E x_0
     CCHK(c, cg_constrain_dot(q, x, Ma, flam, n, part, &sc->dot, st));
     CCHK(c, cg_residual(r, q, r, z, p, Se, n, part, need_b ? part2 : nullptr, &sc->rsold, &sc->bnorm2, st));
   } else {
     CCHK(c, hipMemsetAsync(x, 0, vb, st));
+    if (Ex) CCHK(c, hipMemsetAsync(Ex, 0, sizeof(float2) * ny, st));
     CCHK(c, cg_residual(r, nullptr, r, z, p, Se, n, part, need_b ? part2 : nullptr, &sc->rsold, &sc->bnorm2, st));
   }
   CgScalars h;
@@ -99,9 +131,10 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
     CCHK(c, cg_constrain_dot(q, p, Ma, flam, n, part, &sc->pq, st));
     CCHK(c, cg_step(sc, st));
     CCHK(c, cg_update(x, r, z, p, q, Se, sc, n, part, part2, st));
+    if (Ex) CCHK(c, cg_kspace_update(Ex, Ep, y, sc, ny, it % 2 == 1, part, st));
     if (tolType == 8) CCHK(c, cg_maxabs2(p, n, part, &sc->dot, st));  // before p changes
     CCHK(c, cg_direction(p, z, sc, n, st));
-    if (tolType == 6 && it % 2 == 1) {
+    if (tolType == 6 && it % 2 == 1 && !Ex) {
       double e2[2];
       rc = energy(e2);
       if (rc) return rc;
@@ -110,6 +143,13 @@ int run_cg(dis_ctx* c, const OpView& v, const double* T, const float2* y, const 
     }
     CCHK(c, hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st));
     CCHK(c, hipStreamSynchronize(st));
+    if (tolType == 6 && it % 2 == 1 && Ex) {
+      double e2[2];
+      rc = kenergy(e2, h.fid.x);
+      if (rc) return rc;
+      Ehist.push_back(e2[0]);
+      Ehist.push_back(e2[1]);
+    }
     if (tolType == 8) {
       const double g2 = h.g.x * h.g.x + h.g.y * h.g.y;
       if (g2 * h.dot.x < tol) break;                 // max |alpha p|^2 < toler

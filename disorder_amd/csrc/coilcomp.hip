@@ -48,7 +48,6 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 constexpr int kGramTile = 32;     // voxels staged per step
 constexpr int kGramLd = 66;       // padded coil stride of the staging tiles
 constexpr int kGramFlush = 8;     // FP32 steps (of kGramTile voxels) per FP64 flush
-constexpr int kGramBlocksMax = 1024;
 
 // One block accumulates a contiguous chunk of voxels.  Thread (ti, tj) owns a
 // 4x4 block of P (256 threads cover 64 x 64 coils).

@@ -184,6 +184,28 @@ __global__ __launch_bounds__(kThreads) void update_kernel(float2* __restrict__ x
   block_store(ab, 0.0, partial_abs);
 }
 
+// Ex += g Ep (the same FP32 step as x += g p); partial of sum |Ex - y|^2
+template <bool ENERGY>
+__global__ __launch_bounds__(kThreads) void kspace_update_kernel(float2* __restrict__ Ex, const float2* __restrict__ Ep,
+                                                                 const float2* __restrict__ y, const CgScalars* sc,
+                                                                 long long n, double2* partial) {
+  const float gx = (float)sc->g.x, gy = (float)sc->g.y;
+  double e = 0;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kThreads) {
+    const float2 pv = Ep[i];
+    float2 xv = Ex[i];
+    xv.x += gx * pv.x - gy * pv.y;
+    xv.y += gx * pv.y + gy * pv.x;
+    Ex[i] = xv;
+    if (ENERGY) {
+      const float2 yv = y[i];
+      const double dx = (double)xv.x - yv.x, dy = (double)xv.y - yv.y;
+      e += dx * dx + dy * dy;
+    }
+  }
+  if (ENERGY) block_store(e, 0.0, partial);
+}
+
 // p = z + d p
 __global__ __launch_bounds__(kThreads) void direction_kernel(float2* __restrict__ p, const float2* __restrict__ z,
                                                              const CgScalars* sc, long long n) {
@@ -300,6 +322,18 @@ hipError_t cg_update(float2* x, float2* r, float2* z, const float2* p, const flo
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, st, partial, nb, &sc->rs);
   hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, st, partial_abs, nb, &sc->zr_abs);
   hipLaunchKernelGGL(beta_kernel, dim3(1), dim3(1), 0, st, sc);
+  return hipGetLastError();
+}
+
+hipError_t cg_kspace_update(float2* Ex, const float2* Ep, const float2* y, CgScalars* sc, long long n, bool energy,
+                            double2* partial, hipStream_t st) {
+  const int nb = grid_for(n);
+  if (energy) {
+    hipLaunchKernelGGL(kspace_update_kernel<true>, dim3(nb), dim3(kThreads), 0, st, Ex, Ep, y, sc, n, partial);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kThreads), 0, st, partial, nb, &sc->fid);
+  } else {
+    hipLaunchKernelGGL(kspace_update_kernel<false>, dim3(nb), dim3(kThreads), 0, st, Ex, Ep, y, sc, n, partial);
+  }
   return hipGetLastError();
 }
 

@@ -14,11 +14,16 @@ struct CgScalars {
   double2 zr_abs;  // sum |conj(z) r|  (RelativeResidual2Error)
   double2 bnorm2;  // sum |conj(P b) b|
   double2 dot;     // scratch result
+  double2 fid;     // sum |E x - y|^2 from the k-space iterate (x: the fidelity energy)
 };
 
 constexpr int kCgPartials = 1024;
 
 hipError_t cg_dot(const float2* a, const float2* b, long long n, double2* partial, double2* out, hipStream_t st);
+// Ex += g Ep with the step g of the x update (sc->g, in FP32 like x); with
+// energy, sc->fid.x = sum |Ex - y|^2 in FP64 (fixed-order reduction).
+hipError_t cg_kspace_update(float2* Ex, const float2* Ep, const float2* y, CgScalars* sc, long long n, bool energy,
+                            double2* partial, hipStream_t st);
 hipError_t cg_mask(float2* v, const float* Ma, long long n, hipStream_t st);
 // out->x = max_i |a_i|^2
 hipError_t cg_maxabs2(const float2* a, long long n, double2* partial, double2* out, hipStream_t st);

@@ -372,6 +372,14 @@ int fold_setup(dis_ctx* c, const OpView& v, int maxnec, CoilArgs* a) {
   return DIS_OK;
 }
 
+float2* ehe_kspace(dis_ctx* c, const OpView& v) {
+  const dis_geom& g = v.g;
+  if (!(v.fold[0] == 0 || coil_fold_register(g.n[0], g.n[1], v.fold[0]))) return nullptr;
+  void* yb = nullptr;
+  if (ctx_buffer(c, 7, sizeof(float2) * (size_t)std::max(1, g.nsamples) * g.n[2] * g.ncoils, &yb)) return nullptr;
+  return (float2*)yb;
+}
+
 int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* yin, float2* yout,
              float2* xout, int mode, EnergyOut* eo) {
   const dis_geom& g = v.g;

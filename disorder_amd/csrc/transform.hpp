@@ -142,6 +142,10 @@ int rigid_inverse(dis_ctx* c, const Grid& gr, const double* T, int S, int Sld, c
 // E, E^H, E^H E and the residual energy through one coil-stage mode.
 int op_apply(dis_ctx* c, const OpView& v, const double* T, const float2* x, const float2* yin, float2* yout,
              float2* xout, int mode, EnergyOut* eo);
+// The k-space scratch [Ns x n3 x C] in which op_apply's E^H E leaves E x
+// (its forward half), or null where E^H E runs as one fused or folded
+// stage without it.
+float2* ehe_kspace(dis_ctx* c, const OpView& v);
 
 // Tx[s] = T_s x and G[g][s] = d(T_s x)/dT_g (g < 6) for states [0,S) of a
 // [Sld x 6] motion array.  Buffers: Tx [S], G [6 S], X0 [3 S], Y [2 S] volumes.
