@@ -336,18 +336,29 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   constexpr bool ADJ = MODE == COIL_DECODE || MODE == COIL_EHE;
   constexpr int N = R1 * 16;
   constexpr int XB = RegFftSize<R1>::XB;
-  // lane-table adjoint: the tile carries the lanes' carry slots as kThreads
-  // extra columns, so tile rows and carries share one stride, a multiple of
-  // 64 elements (512 bytes) so that a segment's 16 stores pair into
-  // ds_write2st64 (n1 = 224 with stride 480: 82 single stores, 10 % slower)
-  constexpr int NS = BAL ? (N + kThreads + 63) / 64 * 64 : N;
+  // lane-table adjoint: tile rows kThreads elements apart, and the lanes'
+  // carry slots (one column per lane) in a region of their own at the same
+  // stride, a multiple of 64 elements (512 bytes) so that a segment's 16
+  // stores pair into ds_write2st64 (n1 = 224 with stride 480: 82 single
+  // stores, 10 % slower).  The carries live from the scatter to the deferred
+  // adds and the FFT exchange only after them, so the two share the region
+  // and the exchange needs no workgroup barrier
+  static_assert(!BAL || N <= kThreads, "the lane table covers n1 <= kThreads");
+  constexpr int NS = BAL ? kThreads : N;
+  constexpr int XCB = 16 * XB > 16 * kThreads ? 16 * XB : 16 * kThreads;
   // forward: the 16-row tile is stored column-major (spectrum column k1's
   // 16 rows contiguous, columns ZS = 18 elements apart, 16-byte aligned), so
   // a sample's Horner chain reads its column with 16-byte loads
   constexpr int ZS = 18;
   constexpr int XZF = FWD ? N * ZS : 0;
   constexpr int XZ0 = (16 * XB > 16 * NS) ? 16 * XB : 16 * NS;
-  constexpr int XZ = XZ0 > XZF ? XZ0 : XZF;
+  // forward-only launches: the FFT exchange has an LDS region of its own
+  // behind the tile (80 KiB at N = 256, two workgroups per CU), so a
+  // pencil's exchange is ordered within its wavefront only, and a wave's
+  // next FFT runs while other waves still read the tile; one workgroup
+  // barrier before each tile store and one after it
+  constexpr bool SEPX = MODE == COIL_ENCODE || MODE == COIL_ENERGY;
+  constexpr int XZ = SEPX ? XZF + 16 * XB : BAL ? 16 * NS + XCB : (XZ0 > XZF ? XZ0 : XZF);
   constexpr int kEntCap = 2048;
   __shared__ fvec2 xzraw[XZ];                 // FFT exchange, aliased with the 16-row tile
   // row-owner adjoint: the sample values of the current coil and the
@@ -364,8 +375,9 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   const int p = tid >> 4, t = tid & 15;
   lds_float2* tw1 = (lds_float2*)t1raw;
   lds_float2* tw2 = (lds_float2*)t2raw;
-  lds_float2* xb = (lds_float2*)xzraw + p * XB;
+  lds_float2* xb = (lds_float2*)xzraw + (SEPX ? XZF : BAL ? 16 * NS : 0) + p * XB;
   lds_float2* Zt = (lds_float2*)xzraw;  // [16 rows][N] natural order (row stride NS)
+  lds_float2* Cr = (lds_float2*)xzraw + 16 * NS;  // lane-table carries [16][kThreads] (BAL)
   const int n2 = a.n2;
   // dim-2 DFT of the sampled rows: length n2, or under SENSE folding along
   // dim 2 (fy1 == n1) the acquired length m2 = fy2 with the image rows
@@ -482,14 +494,14 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
             u[r] = row_ok ? ld : make_float2(0.f, 0.f);
           }
           if (a.tfwd) {  // last translation of T along dim 1
-            reg_fft<R1, 16>(u, t, xb, tw1);
+            reg_fft<R1, 16, SEPX>(u, t, xb, tw1);
             if (t < R1) {
               float2 ph[16];
               ramp16<R1>(ph, cf, t);
 #pragma unroll
               for (int k = 0; k < 16; ++k) u[k] = cswap(cmul(u[k], ph[k]));
             }
-            reg_fft<16, R1>(u, t, xb, tw1);
+            reg_fft<16, R1, SEPX>(u, t, xb, tw1);
 #pragma unroll
             for (int r = 0; r < R1; ++r) u[r] = cscale(cswap(u[r]), inv_n);
           }
@@ -503,7 +515,8 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
               float2 v[16];
 #pragma unroll
               for (int r = 0; r < R1; ++r) v[r] = cmul(u[r], Sp[t + 16 * r]);
-              reg_fft<R1, 16>(v, t, xb, tw1);
+              reg_fft<R1, 16, SEPX>(v, t, xb, tw1);
+              if (SEPX) __syncthreads();  // every wave's reads of the previous tile are done
               if (t < R1) {
 #pragma unroll
                 for (int k = 0; k < 16; ++k) st2(Zt + (t + R1 * k) * ZS + p, v[k]);
@@ -547,7 +560,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                   }
                 }
               }
-              __syncthreads();  // the row tile aliases the next FFT's exchange buffer
+              if (!SEPX) __syncthreads();  // the row tile aliases the next FFT's exchange buffer
             }
 #pragma unroll
             for (int i = 0; i < kRJ; ++i) {
@@ -617,8 +630,8 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
         // instead); the owner of a row that continues into later lanes adds
         // their carries in lane order once they are published.  Every lane
         // runs the same number of samples, and the sum is deterministic.
-        // tile [16][NS]: columns < N the rows, N + L lane L's carry (stored
-        // swapped like the rows)
+        // tile [16][NS]: columns < N the rows; Cr [16][kThreads]: column L
+        // lane L's carry (both stored swapped)
         const bool deferred = (binfo & kAdjDeferred) != 0;
         const int ncont = (binfo >> kAdjNcontShift) & 1023;
         const int zrow = (binfo >> kAdjZrowShift) & 1023;
@@ -636,7 +649,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
           // every coil (eight rotations per group, the identity)
           for (int c = 0; c < kCC; ++c) {
             if (c < cc) {
-              __syncthreads();  // the previous FFT's exchange (aliasing the tile) and carry reads are done
+              __syncthreads();  // every wave's tile loads and FFT exchanges (sharing Cr) of the previous coil are done
               float2 acc[16];
 #pragma unroll
               for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
@@ -663,7 +676,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                   lrow = jk1[i];
                   if ((binfo >> (kAdjCloseShift + i)) & 1) {
                     const bool to_carry = (binfo >> (kAdjCarryShift + i)) & 1;
-                    lds_float2* dst = Zt + (to_carry ? N + tid : lrow);
+                    lds_float2* dst = to_carry ? Cr + tid : Zt + lrow;  // same stride NS
 #pragma unroll
                     for (int q = 0; q < 16; ++q) st2(dst + q * NS, cswap(acc[q]));
                   }
@@ -679,7 +692,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
                 for (int q = 0; q < 16; ++q) acc[q] = cswap(acc[q]);
                 for (int l = 1; l <= ncont; ++l) {
 #pragma unroll
-                  for (int q = 0; q < 16; ++q) acc[q] = cadd(acc[q], ld2(Zt + q * NS + N + tid + l));
+                  for (int q = 0; q < 16; ++q) acc[q] = cadd(acc[q], ld2(Cr + q * NS + tid + l));
                 }
 #pragma unroll
                 for (int q = 0; q < 16; ++q) st2(Zt + q * NS + lrow, acc[q]);
@@ -688,8 +701,8 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
               float2 v[16];
 #pragma unroll
               for (int r = 0; r < R1; ++r) v[r] = ld2(Zt + p * NS + t + 16 * r);
-              __syncthreads();  // the row tile aliases the exchange buffer
-              reg_fft<R1, 16>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
+              // the exchange region holds only carries, read before the barrier above
+              reg_fft<R1, 16, true>(v, t, xb, tw1);  // inverse (swapped data): lane k1 holds k1 + R1 k2
               if (t < R1) {
                 const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
                 // all 16 map loads in flight before the first use: with the
@@ -713,14 +726,14 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
             }
           }
           if (a.tadj) {  // first translation of T^H along dim 1 (input layout of reg_fft<16,R1>)
-            reg_fft<16, R1>(o, t, xb, tw1);
+            reg_fft<16, R1, true>(o, t, xb, tw1);
             {
               float2 ph[16];
               ramp_in<R1>(ph, ca, t);
 #pragma unroll
               for (int r = 0; r < R1; ++r) o[r] = cswap(cmul(o[r], ph[r]));
             }
-            reg_fft<R1, 16>(o, t, xb, tw1);
+            reg_fft<R1, 16, true>(o, t, xb, tw1);
 #pragma unroll
             for (int k = 0; k < 16; ++k) o[k] = cscale(cswap(o[k]), inv_n);
           }

@@ -152,11 +152,23 @@ template <int R1> struct RegFftSize {
   static constexpr int XB_STRIDED = pad_to(BASE, 18);
 };
 
+// Orders a wavefront's LDS accesses around an exchange among its own lanes.
+// The LDS executes one wavefront's instructions in issue order, so only the
+// compiler has to be kept from moving accesses across this point (the
+// wavefront-scope fences emit no instructions).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One two-level FFT of a pencil: RA x RB = N, lanes t < RB enter with
 // v[r] = x[t + RB r] (r < RA); lanes k1 < RA leave with v[k2] = X[k1 + RA k2]
 // (k2 < RB).  xb: this pencil's exchange region; tw: W_N^i, i < N.  Contains
-// two workgroup barriers (every lane of the workgroup must call it).
-template <int RA, int RB>
+// two workgroup barriers (every lane of the workgroup must call it), or with
+// WAVE two wavefront-order points: WAVE requires the pencil's lanes to sit in
+// one wavefront (lane = 16 p + t) and xb to be touched by no other pencil.
+template <int RA, int RB, bool WAVE = false>
 __device__ __forceinline__ void reg_fft(float2 (&v)[16], int t, lds_float2* xb, const lds_float2* tw) {
   constexpr int LD = RB + 1;  // padded exchange row
   if (t < RB) {
@@ -166,13 +178,13 @@ __device__ __forceinline__ void reg_fft(float2 (&v)[16], int t, lds_float2* xb, 
 #pragma unroll
     for (int k1 = 0; k1 < RA; ++k1) st2(xb + k1 * LD + t, v[k1]);
   }
-  __syncthreads();
+  if (WAVE) wave_lds_sync(); else __syncthreads();
   if (t < RA) {
 #pragma unroll
     for (int tt = 0; tt < RB; ++tt) v[tt] = ld2(xb + t * LD + tt);
     Dft<RB>::run(v);
   }
-  __syncthreads();
+  if (WAVE) wave_lds_sync(); else __syncthreads();
 }
 
 // Phase ramp exp(2 pi i c m) over the forward output of lane k1 (< R1):
