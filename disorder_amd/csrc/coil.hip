@@ -509,6 +509,7 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
           // rotated by one after every coil (eight rotations per group, the
           // identity), so the sums take a static register index instead of a
           // select over all eight
+#pragma unroll 2  // halves the accumulator rotation's moves; full unrolling overflows the instruction cache
           for (int c = 0; c < kCC; ++c) {
             if (c < cc) {
               const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
