@@ -356,8 +356,9 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   // behind the tile (80 KiB at N = 256, two workgroups per CU), so a
   // pencil's exchange is ordered within its wavefront only, and a wave's
   // next FFT runs while other waves still read the tile; one workgroup
-  // barrier before each tile store and one after it
-  constexpr bool SEPX = MODE == COIL_ENCODE || MODE == COIL_ENERGY;
+  // barrier before each tile store and one after it (not in persistent
+  // launches, whose queue slot would take the LDS past two workgroups per CU)
+  constexpr bool SEPX = (MODE == COIL_ENCODE || MODE == COIL_ENERGY) && !PERSIST;
   constexpr int XZ = SEPX ? XZF + 16 * XB : BAL ? 16 * NS + XCB : (XZ0 > XZF ? XZ0 : XZF);
   constexpr int kEntCap = 2048;
   __shared__ fvec2 xzraw[XZ];                 // FFT exchange, aliased with the 16-row tile
@@ -509,7 +510,11 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
           // rotated by one after every coil (eight rotations per group, the
           // identity), so the sums take a static register index instead of a
           // select over all eight
-#pragma unroll 2  // halves the accumulator rotation's moves; full unrolling overflows the instruction cache
+          // encode launches: unrolled twice, half the rotation's moves (11.8 ->
+          // 11.5 ms at 256x192x256); the energy instance unrolled so ran 2.5x
+          // slower, and full unrolling overflows the instruction cache
+          constexpr int kCoilUnroll = MODE == COIL_ENCODE ? 2 : 1;
+#pragma unroll kCoilUnroll
           for (int c = 0; c < kCC; ++c) {
             if (c < cc) {
               const float2* Sp = a.Smaps + (long long)(c0 + c) * nvox + i3 * nplane + rowoff;
