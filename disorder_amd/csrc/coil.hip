@@ -361,7 +361,9 @@ __global__ __launch_bounds__(kThreads, 2) void rcoil_kernel(CoilArgs a, int nsta
   constexpr bool SEPX = (MODE == COIL_ENCODE || MODE == COIL_ENERGY) && !PERSIST;
   constexpr int XZ = SEPX ? XZF + 16 * XB : BAL ? 16 * NS + XCB : (XZ0 > XZF ? XZ0 : XZF);
   constexpr int kEntCap = 2048;
-  __shared__ fvec2 xzraw[XZ];                 // FFT exchange, aliased with the 16-row tile
+  // the 16-row tile and the FFT exchange: aliased, or the exchange behind
+  // the tile (SEPX), or behind the rows in the carries' region (BAL)
+  __shared__ fvec2 xzraw[XZ];
   // row-owner adjoint: the sample values of the current coil and the
   // state's CSR by k1
   constexpr int kWcElems = kRJ * kThreads;
